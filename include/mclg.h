@@ -1,0 +1,156 @@
+/* mclg.h — C ABI of the MI355X-native collision pipeline.
+ *
+ * This is the boundary under the `mcl::BVHTree<double,DIM>` facade
+ * (include/MCL/BVHTree.hpp): plain pointers and sizes, status codes, no C++ or
+ * torch types.  Each entry point names the reference interface it replaces
+ * (paths relative to the mattoverby/mclccd repository).
+ *
+ * Semantics shared by all entry points:
+ *   - Every function returns MCLG_OK (0) or a negative MCLG_ERR_* code; the
+ *     message of the last error is available from mclg_last_error().
+ *   - Host-pointer entry points copy their inputs to the GPU; *_device entry
+ *     points take device pointers (HBM-resident inputs) and copy nothing in.
+ *   - update/traverse are synchronous: when they return, results are on the
+ *     device and their counts are known.  Results stay valid until the next
+ *     traverse on the same context.
+ *   - A context is bound to the HIP device current at mclg_create and is not
+ *     thread-safe: calls on one context must not overlap.
+ */
+#ifndef MCLG_H
+#define MCLG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MCLG_OK = 0,
+  MCLG_ERR_ARG = -1,         /* invalid argument (sizes, out-of-range indices, dim) */
+  MCLG_ERR_HIP = -2,         /* a HIP runtime call failed */
+  MCLG_ERR_NOMEM = -3,       /* device allocation failed */
+  MCLG_ERR_STATE = -4,       /* call order (e.g. traverse before update) */
+  MCLG_ERR_UNSUPPORTED = -5, /* option with no GPU path (vf_one_sided CTCD) */
+  MCLG_ERR_CAPACITY = -6     /* caller buffer smaller than the result */
+};
+
+typedef struct mclg_ctx mclg_ctx;
+
+/* mcl::BVHTree<T,DIM>::Options — include/MCL/BVHTree.hpp:42-64 (same fields,
+ * same defaults; booleans as int). */
+typedef struct mclg_options {
+  double box_eta;    /* padding of tree boxes (default 1e-6) */
+  double vf_ccd_eta; /* VF gap; update() clamps it to box_eta */
+  double ee_ccd_eta; /* EE gap; update() clamps it to box_eta */
+  int reptri;        /* representative triangles (default 1) */
+  int threaded;      /* accepted for compatibility; the GPU path is always parallel */
+  int discrete;      /* discrete tri-tri (3D) / edge-edge (2D) at V1 (default 1) */
+  int continuous;    /* ACCD from V0 to V1 (default 1) */
+  int vf_one_sided;  /* CTCD VF path: no GPU implementation -> MCLG_ERR_UNSUPPORTED */
+  int ee_robust;     /* unused on this path (as in the reference) */
+} mclg_options;
+
+/* Fills the reference defaults (BVHTree.hpp:53-63). */
+void mclg_default_options(mclg_options* o);
+
+/* mcl::BVHTree<T,DIM>::BVHTree() — include/MCL/BVHTree.cpp:20-24.  dim is 2 or 3. */
+int mclg_create(int dim, mclg_ctx** out);
+void mclg_destroy(mclg_ctx* ctx);
+
+/* The public `options` member — BVHTree.hpp:64. */
+int mclg_set_options(mclg_ctx* ctx, const mclg_options* o);
+int mclg_get_options(const mclg_ctx* ctx, mclg_options* o);
+
+const char* mclg_last_error(const mclg_ctx* ctx);
+
+/* Runs the context's work on a caller-provided hipStream_t (NULL: the context's
+ * own stream).  No reference counterpart; used by the PyTorch wrapper. */
+int mclg_set_stream(mclg_ctx* ctx, void* hip_stream);
+
+/* BVHTree<T,DIM>::update(const T* V0, const T* V1, const int* P, int np, int pdim,
+ * const int* active) — include/MCL/BVHTree.hpp:81-82, BVHTree.cpp:27-126.
+ * V0, V1: nv x dim row-major; P: np x pdim row-major; active: nv ints or NULL.
+ * nv is explicit here (the reference implies it from P).  np == 0 clears the
+ * tree.  Indices outside [0, nv) -> MCLG_ERR_ARG. */
+int mclg_update(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
+                const int* active);
+int mclg_update_device(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
+                       const int* active);
+
+/* BVHTree<T,DIM>::traverse(const T* V0, const T* V1, const int* P) const —
+ * include/MCL/BVHTree.hpp:86, BVHTree.cpp:129-159 (+ collide, get_candidates,
+ * default_narrow_phase, default_discrete_test: :230-500).  Uses nv from the last
+ * update; P must have pdim == dim.  Instead of invoking callbacks it leaves the
+ * continuous hits (stencil, is_vf, toi), the discrete hits (p0, p1) and, on
+ * request, the candidate stencils and leaf pairs on the device. */
+#define MCLG_KEEP_CANDIDATES 1u /* keep every candidate stencil (for filter_pair / narrow_phase replay) */
+#define MCLG_KEEP_LEAF_PAIRS 2u /* keep the broadphase leaf pairs */
+int mclg_traverse(mclg_ctx* ctx, const double* V0, const double* V1, const int* P, unsigned flags);
+int mclg_traverse_device(mclg_ctx* ctx, const double* V0, const double* V1, const int* P, unsigned flags);
+
+/* Sizes of the last traverse.  candidates and dist_evals are always counted. */
+typedef struct mclg_counts {
+  int64_t leaf_pairs;
+  int64_t candidates;
+  int64_t hits;
+  int64_t discrete;
+  int64_t dist_evals; /* ACCD pair_distance evaluations performed */
+} mclg_counts;
+int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
+
+/* Copies of the last traverse's results to host memory.  Each writes
+ * min(count, cap) records and fails with MCLG_ERR_CAPACITY if cap < count.
+ *   hits:       sten4 = int[4*cap], is_vf = uint8[cap], toi = double[cap]
+ *               (the arguments of append_pair, BVHTree.hpp:102)
+ *   discrete:   int[2*cap] (the arguments of append_discrete, BVHTree.hpp:108)
+ *   candidates: sten4 / is_vf as above plus the ACCD toi of each candidate
+ *               (-1 when it does not hit), every candidate
+ *               (MCLG_KEEP_CANDIDATES)
+ * Records come in no particular order (the reference's callback order is
+ * not deterministic either).
+ *   leaf pairs: int[2*cap] (MCLG_KEEP_LEAF_PAIRS)
+ * Stencils follow BVHTree.cpp:336-395: VF (v, f0, f1, f2) with -1 padding in
+ * 2D, EE (a0, a1, b0, b1).  Pair orientation: the lower primitive index
+ * first, or the lower rank when a rank is set (mclg_set_rank). */
+int mclg_copy_hits(mclg_ctx* ctx, int* sten4, unsigned char* is_vf, double* toi, int64_t cap);
+int mclg_copy_discrete(mclg_ctx* ctx, int* pairs2, int64_t cap);
+int mclg_copy_candidates(mclg_ctx* ctx, int* sten4, unsigned char* is_vf, double* toi, int64_t cap);
+int mclg_copy_leaf_pairs(mclg_ctx* ctx, int* pairs2, int64_t cap);
+
+/* Device pointers to the last traverse's results (valid until the next traverse). */
+typedef struct mclg_device_results {
+  const int* hit_sten4;
+  const unsigned char* hit_is_vf;
+  const double* hit_toi;
+  const int* discrete2;
+  const int* cand_sten4;
+  const unsigned char* cand_is_vf;
+  const double* cand_toi;
+  const int* leaf_pairs2;
+} mclg_device_results;
+int mclg_get_device_results(const mclg_ctx* ctx, mclg_device_results* out);
+
+/* Orientation of leaf pairs.  The reference orients each pair by the in-order
+ * position of its leaves in its KdBVH (BVHTree.cpp:288-301, KdBVH.hpp:202,212);
+ * that order decides the EE stencil order and the tri-tri argument order.
+ * rank[prim] (np ints) makes the GPU orient pairs the same way (lower rank
+ * first); NULL restores the default (lower primitive index first).  Cleared
+ * when the primitive count changes. */
+int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
+
+/* Device milliseconds of the phases of the last update / traverse, measured
+ * with HIP events on the context's stream:
+ *   0 validate+pack  1 reptri  2 leaf boxes  3 morton  4 sort  5 lbvh (update)
+ *   6 validate+pack  7 traversal kernel  8 narrow-phase kernel  (traverse;
+ *   7 and 8 bracket the final launch of the kernel only)
+ * Writes min(n, MCLG_NUM_TIMINGS) values. */
+#define MCLG_NUM_TIMINGS 9
+int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCLG_H */

@@ -1,0 +1,512 @@
+// BVH build kernels for gfx950: vertex packing, representative-triangle flags,
+// leaf boxes, Morton codes, LSD radix sort, bottom-up LBVH (Apetrei 2014) with
+// FP32 outward-rounded node boxes, and escape links for stackless traversal.
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mclg {
+
+// ---------------------------------------------------------------------------
+// Vertex records (both time steps of a vertex in one 64 B / 32 B record)
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void k_pack(const double* __restrict__ V0, const double* __restrict__ V1, int nv, double* __restrict__ rec) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double* r = rec + (size_t)v * VRec<D>::stride;
+  if constexpr (D == 3) {
+    r[0] = V0[3 * (size_t)v]; r[1] = V0[3 * (size_t)v + 1]; r[2] = V0[3 * (size_t)v + 2]; r[3] = 0.0;
+    r[4] = V1[3 * (size_t)v]; r[5] = V1[3 * (size_t)v + 1]; r[6] = V1[3 * (size_t)v + 2]; r[7] = 0.0;
+  } else {
+    r[0] = V0[2 * (size_t)v]; r[1] = V0[2 * (size_t)v + 1];
+    r[2] = V1[2 * (size_t)v]; r[3] = V1[2 * (size_t)v + 1];
+  }
+}
+
+void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s) {
+  if (nv <= 0) return;
+  const int g = (nv + 255) / 256;
+  if (dim == 3) k_pack<3><<<g, 256, 0, s>>>(V0, V1, nv, rec);
+  else k_pack<2><<<g, 256, 0, s>>>(V0, V1, nv, rec);
+}
+
+// ---------------------------------------------------------------------------
+// Index validation: min and max of an int array (one block per 64K items).
+// ---------------------------------------------------------------------------
+__global__ void k_minmax_i32(const int* __restrict__ a, long long n, int* out) {
+  int lo = 0x7fffffff, hi = -0x7fffffff - 1;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int v = a[i];
+    lo = v < lo ? v : lo;
+    hi = v > hi ? v : hi;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const int l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
+  }
+  if (lane_id() == 0) { atomicMin(out, lo); atomicMax(out + 1, hi); }
+}
+
+void launch_minmax_i32(const int* a, long long n, int* out2, hipStream_t s) {
+  if (n <= 0) return;
+  long long g = (n + 255) / 256;
+  if (g > 2048) g = 2048;
+  k_minmax_i32<<<(int)g, 256, 0, s>>>(a, n, out2);
+}
+
+// ---------------------------------------------------------------------------
+// Fills
+// ---------------------------------------------------------------------------
+template <typename T> __global__ void k_fill(T* p, long long n, T v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) p[i] = v;
+}
+template <typename T> static void fill(T* p, long long n, T v, hipStream_t s) {
+  if (n <= 0) return;
+  long long g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  k_fill<T><<<(int)g, 256, 0, s>>>(p, n, v);
+}
+void launch_fill_u8(uint8_t* p, long long n, uint8_t v, hipStream_t s) { fill<uint8_t>(p, n, v, s); }
+void launch_fill_i32(int* p, long long n, int v, hipStream_t s) { fill<int>(p, n, v, s); }
+void launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t s) { fill<unsigned long long>(p, n, v, s); }
+
+// ---------------------------------------------------------------------------
+// Representative-triangle flags (BVHTree.cpp:53-79): a vertex belongs to its
+// first (face, slot) occurrence, an edge {a,b} to the first (face, slot) whose
+// slot edge (P[j], P[(j+1)%3]) it is.  "First" is the minimum of face*pdim+slot,
+// found with atomicMin (vertices) and an open-addressing hash of edge keys.
+// ---------------------------------------------------------------------------
+__global__ void k_vertex_owner(const int* __restrict__ P, long long nslots, int* owner) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  atomicMin(owner + P[i], (int)i);
+}
+
+__device__ __forceinline__ uint32_t hash64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdull;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ull;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+__device__ __forceinline__ uint64_t edge_key(int a, int b) {
+  if (b < a) { const int t = a; a = b; b = t; }
+  return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+__global__ void k_edge_insert(const int* __restrict__ P, long long nslots, unsigned long long* keys, int* vals,
+                              uint32_t mask) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nslots) return;
+  const long long f = i / 3;
+  const int j = (int)(i - f * 3);
+  const uint64_t key = edge_key(P[i], P[f * 3 + (j + 1) % 3]);
+  uint32_t h = hash64(key) & mask;
+  for (;;) {
+    const unsigned long long prev = atomicCAS(keys + h, kEmptyKey, (unsigned long long)key);
+    if (prev == kEmptyKey || prev == key) { atomicMin(vals + h, (int)i); return; }
+    h = (h + 1) & mask;
+  }
+}
+
+__device__ __forceinline__ int edge_owner(const unsigned long long* keys, const int* vals, uint32_t mask, uint64_t key) {
+  uint32_t h = hash64(key) & mask;
+  for (;;) {
+    const unsigned long long k = keys[h];
+    if (k == key) return vals[h];
+    if (k == kEmptyKey) return -1;  // unreachable: every slot edge was inserted
+    h = (h + 1) & mask;
+  }
+}
+
+// flags: bits 0..2 = v[j], bits 3..5 = e[j]
+__global__ void k_reptri_flags(const int* __restrict__ P, int np, int pdim, const int* __restrict__ owner,
+                               const unsigned long long* __restrict__ keys, const int* __restrict__ vals, uint32_t mask,
+                               uint8_t* __restrict__ flags) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= np) return;
+  uint32_t fl = 0;
+  for (int j = 0; j < pdim && j < 3; ++j) {
+    const long long slot = (long long)f * pdim + j;
+    if (owner[P[slot]] == (int)slot) fl |= 1u << j;
+    if (pdim == 3) {
+      const uint64_t key = edge_key(P[slot], P[(long long)f * 3 + (j + 1) % 3]);
+      if (edge_owner(keys, vals, mask, key) == (int)slot) fl |= 1u << (3 + j);
+    }
+  }
+  flags[f] = (uint8_t)fl;
+}
+
+void launch_reptri(const int* P, int np, int pdim, int nv, int* owner, unsigned long long* keys, int* vals,
+                   uint32_t table_size, uint8_t* flags, hipStream_t s) {
+  const long long nslots = (long long)np * pdim;
+  fill<int>(owner, nv, 0x7fffffff, s);
+  const int g = (int)((nslots + 255) / 256);
+  k_vertex_owner<<<g, 256, 0, s>>>(P, nslots, owner);
+  if (pdim == 3) {
+    fill<unsigned long long>(keys, table_size, kEmptyKey, s);
+    fill<int>(vals, table_size, 0x7fffffff, s);
+    k_edge_insert<<<g, 256, 0, s>>>(P, nslots, keys, vals, table_size - 1);
+  }
+  k_reptri_flags<<<(np + 255) / 256, 256, 0, s>>>(P, np, pdim, owner, keys, vals, table_size - 1, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Leaf boxes (BVHTree.cpp:82-121): t0 box, t1 box and swept box of the prim's
+// vertices, each padded by exactly box_eta; active if no mask or any vertex
+// active.  Also reduces the bounds of the swept-box centres for Morton codes.
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ rec, const int* __restrict__ P, int np,
+                                                    int pdim, const int* __restrict__ active, double eta,
+                                                    LeafBox<D>* __restrict__ boxes, uint8_t* __restrict__ act,
+                                                    unsigned long long* cbounds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double clo[D], chi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) { clo[k] = __builtin_huge_val(); chi[k] = -__builtin_huge_val(); }
+  if (i < np) {
+    LeafBox<D> b;
+    const double big = 1.7976931348623157e308;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      b.sw_lo[k] = b.t0_lo[k] = b.t1_lo[k] = big;
+      b.sw_hi[k] = b.t0_hi[k] = b.t1_hi[k] = -big;
+    }
+    bool a = active == nullptr;
+    for (int j = 0; j < pdim; ++j) {
+      const int v = P[(long long)i * pdim + j];
+      const double* r = rec + (size_t)v * VRec<D>::stride;
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const double x0 = r[k], x1 = r[VRec<D>::t1 + k];
+        b.t0_lo[k] = (x0 < b.t0_lo[k]) ? x0 : b.t0_lo[k];
+        b.t0_hi[k] = (b.t0_hi[k] < x0) ? x0 : b.t0_hi[k];
+        b.t1_lo[k] = (x1 < b.t1_lo[k]) ? x1 : b.t1_lo[k];
+        b.t1_hi[k] = (b.t1_hi[k] < x1) ? x1 : b.t1_hi[k];
+        b.sw_lo[k] = (x0 < b.sw_lo[k]) ? x0 : b.sw_lo[k];
+        b.sw_hi[k] = (b.sw_hi[k] < x0) ? x0 : b.sw_hi[k];
+        b.sw_lo[k] = (x1 < b.sw_lo[k]) ? x1 : b.sw_lo[k];
+        b.sw_hi[k] = (b.sw_hi[k] < x1) ? x1 : b.sw_hi[k];
+      }
+      if (active != nullptr && active[v]) a = true;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      b.t0_lo[k] -= eta; b.t1_lo[k] -= eta; b.sw_lo[k] -= eta;
+      b.t0_hi[k] += eta; b.t1_hi[k] += eta; b.sw_hi[k] += eta;
+      const double c = (b.sw_lo[k] + b.sw_hi[k]) / 2.0;
+      clo[k] = c; chi[k] = c;
+    }
+    boxes[i] = b;
+    act[i] = a ? 1 : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const double l2 = __shfl_xor(clo[k], o), h2 = __shfl_xor(chi[k], o);
+      clo[k] = l2 < clo[k] ? l2 : clo[k];
+      chi[k] = h2 > chi[k] ? h2 : chi[k];
+    }
+  }
+  if (lane_id() == 0 && clo[0] <= chi[0]) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      atomicMin(cbounds + k, (unsigned long long)dbl_to_ord(clo[k]));
+      atomicMax(cbounds + D + k, (unsigned long long)dbl_to_ord(chi[k]));
+    }
+  }
+}
+
+void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
+                       void* boxes, uint8_t* act, unsigned long long* cbounds, hipStream_t s) {
+  // bounds start empty: lo = ord(+inf), hi = ord(-inf)
+  const unsigned long long lo0 = dbl_to_ord(__builtin_huge_val()), hi0 = dbl_to_ord(-__builtin_huge_val());
+  fill<unsigned long long>(cbounds, dim, lo0, s);
+  fill<unsigned long long>(cbounds + dim, dim, hi0, s);
+  const int g = (np + 255) / 256;
+  if (dim == 3) k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, cbounds);
+  else k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, cbounds);
+}
+
+// ---------------------------------------------------------------------------
+// Morton codes of swept-box centres: 10 bits per axis (3D), 16 (2D).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 10 bits -> every 3rd bit
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t spread2(uint32_t v) {  // 16 bits -> every 2nd bit
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+
+template <int D>
+__global__ void k_morton(const LeafBox<D>* __restrict__ boxes, int np, const unsigned long long* __restrict__ cbounds,
+                         uint32_t key_base, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const LeafBox<D>& b = boxes[i];
+  constexpr double cells = D == 3 ? 1024.0 : 65536.0;
+  uint32_t q[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double lo = ord_to_dbl(cbounds[k]), hi = ord_to_dbl(cbounds[D + k]);
+    const double c = (b.sw_lo[k] + b.sw_hi[k]) / 2.0;
+    const double ext = hi - lo;
+    double u = ext > 0 ? (c - lo) / ext * cells : 0.0;
+    u = u < 0 ? 0.0 : (u > cells - 1 ? cells - 1 : u);
+    q[k] = (uint32_t)u;
+  }
+  uint32_t code;
+  if constexpr (D == 3) code = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+  else code = (spread2(q[0]) << 1) | spread2(q[1]);
+  keys[i] = code + key_base;
+  vals[i] = (uint32_t)i;
+}
+
+void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, uint32_t* keys,
+                   uint32_t* vals, hipStream_t s) {
+  const int g = (np + 255) / 256;
+  if (dim == 3) k_morton<3><<<g, 256, 0, s>>>((const LeafBox<3>*)boxes, np, cbounds, 0, keys, vals);
+  else k_morton<2><<<g, 256, 0, s>>>((const LeafBox<2>*)boxes, np, cbounds, 0, keys, vals);
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (uint32 key, uint32 value), 8-bit digits.
+//   hist:    per-tile digit counts (LDS atomics)      -> hist[digit*ntiles+tile]
+//   scan:    exclusive scan of hist in digit-major order (one block)
+//   scatter: stable per-tile ranking by wave match (8 ballots per key)
+// ---------------------------------------------------------------------------
+constexpr int RS_T = 256, RS_ROUNDS = 16, RS_TILE = RS_T * RS_ROUNDS;
+
+__global__ __launch_bounds__(RS_T) void k_radix_hist(const uint32_t* __restrict__ keys, int n, int shift,
+                                                     uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const long long i = base + r * RS_T + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int total) {
+  __shared__ uint32_t part[1024];
+  const int per = (total + 1023) / 1024;
+  const int b = threadIdx.x * per;
+  const int e = b + per < total ? b + per : total;
+  uint32_t s = 0;
+  for (int i = b; i < e; ++i) s += hist[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[threadIdx.x] - s;  // exclusive
+  for (int i = b; i < e; ++i) {
+    const uint32_t v = hist[i];
+    hist[i] = run;
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(RS_T) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int n,
+                                                        int shift, const uint32_t* __restrict__ hist) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[RS_T / kWave][256];
+  const int t = threadIdx.x, w = t / kWave;
+  base[t] = hist[(size_t)t * gridDim.x + blockIdx.x];
+#pragma unroll
+  for (int j = 0; j < RS_T / kWave; ++j) wcnt[j][t] = 0;
+  __syncthreads();
+  const long long tile = (long long)blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const long long i = tile + r * RS_T + t;
+    const bool valid = i < n;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; ++bit) {
+      const uint64_t m = __ballot((d >> bit) & 1u);
+      peers &= ((d >> bit) & 1u) ? m : ~m;
+    }
+    const int rank = popc_below(peers);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t off = base[d] + rank;
+      for (int j = 0; j < w; ++j) off += wcnt[j][d];
+      kout[off] = key;
+      vout[off] = val;
+    }
+    __syncthreads();
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < RS_T / kWave; ++j) { tot += wcnt[j][t]; wcnt[j][t] = 0; }
+    base[t] += tot;
+    __syncthreads();
+  }
+}
+
+int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+
+// Sorts (keys, vals) in place (ping-pong through tmp buffers); nbits rounded up to 8.
+void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
+                       hipStream_t s) {
+  if (n <= 1) return;
+  const int tiles = radix_tiles(n);
+  uint32_t *ki = keys, *vi = vals, *ko = ktmp, *vo = vtmp;
+  int passes = (nbits + 7) / 8;
+  if (passes & 1) ++passes;  // even number of passes: result lands back in keys/vals
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    k_radix_hist<<<tiles, RS_T, 0, s>>>(ki, n, shift, hist);
+    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * tiles);
+    k_radix_scatter<<<tiles, RS_T, 0, s>>>(ki, vi, ko, vo, n, shift, hist);
+    uint32_t* t;
+    t = ki; ki = ko; ko = t;
+    t = vi; vi = vo; vo = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// LBVH: bottom-up agglomerative build (Apetrei 2014).  Internal node k splits
+// sorted leaves k | k+1; the split key orders splits by (Morton XOR, trailing
+// ones of k, k) so that runs of equal codes form balanced subtrees and every
+// comparison is strict.  Node ids: internal k -> k, leaf s -> (n-1)+s.
+// Boxes are FP32 rounded outward from the FP64 leaf boxes: every culling test
+// on them is conservative; the exact FP64 test runs at leaf pairs.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t split_key(const uint32_t* keys, int k) {
+  const uint32_t x = keys[k] ^ keys[k + 1];
+  const uint32_t ones = (uint32_t)__builtin_ctz(~(uint32_t)k);
+  return ((uint64_t)x << 32) | ((uint64_t)ones << 26) | (uint32_t)k;
+}
+
+template <int D> __device__ __forceinline__ void merge_node(NodeF<D>& o, const NodeF<D>& a, const NodeF<D>& b) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    o.sw_lo[k] = fminf(a.sw_lo[k], b.sw_lo[k]); o.sw_hi[k] = fmaxf(a.sw_hi[k], b.sw_hi[k]);
+    o.t0_lo[k] = fminf(a.t0_lo[k], b.t0_lo[k]); o.t0_hi[k] = fmaxf(a.t0_hi[k], b.t0_hi[k]);
+    o.t1_lo[k] = fminf(a.t1_lo[k], b.t1_lo[k]); o.t1_hi[k] = fmaxf(a.t1_hi[k], b.t1_hi[k]);
+  }
+  o.meta = (a.meta | b.meta) & 0x80000000u;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
+                                                    int n, const LeafBox<D>* __restrict__ boxes,
+                                                    const uint8_t* __restrict__ act, NodeF<D>* nodes, int* lchild,
+                                                    int* rchild, int* rng_l, int* rng_r, int* visits) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  // leaf node
+  {
+    const uint32_t prim = order[s];
+    const LeafBox<D> b = boxes[prim];
+    NodeF<D> L;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      L.sw_lo[k] = f32_down(b.sw_lo[k]); L.sw_hi[k] = f32_up(b.sw_hi[k]);
+      L.t0_lo[k] = f32_down(b.t0_lo[k]); L.t0_hi[k] = f32_up(b.t0_hi[k]);
+      L.t1_lo[k] = f32_down(b.t1_lo[k]); L.t1_hi[k] = f32_up(b.t1_hi[k]);
+    }
+    L.child = (int)prim;
+    L.meta = act[prim] ? 0x80000000u : 0u;
+    nodes[(n - 1) + s] = L;
+  }
+  if (n == 1) return;
+  int l = s, r = s, node = (n - 1) + s;
+  for (;;) {
+    int parent;
+    if (l == 0 || (r != n - 1 && split_key(keys, r) < split_key(keys, l - 1))) {
+      parent = r;
+      lchild[parent] = node;
+      rng_l[parent] = l;
+    } else {
+      parent = l - 1;
+      rchild[parent] = node;
+      rng_r[parent] = r;
+    }
+    __threadfence();
+    const int prev = __hip_atomic_fetch_add(visits + parent, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev == 0) return;  // sibling not finished: it will carry on upwards
+    __threadfence();
+    l = __hip_atomic_load(rng_l + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r = __hip_atomic_load(rng_r + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lc = __hip_atomic_load(lchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int rc = __hip_atomic_load(rchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    NodeF<D> a, b, o;
+    // the children were written by other threads before their release: read
+    // them through the coherent path
+    {
+      const uint32_t* pa = (const uint32_t*)(nodes + lc);
+      const uint32_t* pb = (const uint32_t*)(nodes + rc);
+      uint32_t* qa = (uint32_t*)&a;
+      uint32_t* qb = (uint32_t*)&b;
+#pragma unroll
+      for (int k = 0; k < (int)(sizeof(NodeF<D>) / 4); ++k) {
+        qa[k] = __hip_atomic_load(pa + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        qb[k] = __hip_atomic_load(pb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    merge_node<D>(o, a, b);
+    o.child = lc;
+    nodes[parent] = o;
+    node = parent;
+    if (l == 0 && r == n - 1) return;  // root
+  }
+}
+
+// escape(node with range [l,r]) = right child of internal node r, or end.
+template <int D>
+__global__ void k_lbvh_ropes(int n, NodeF<D>* nodes, const int* __restrict__ rchild, const int* __restrict__ rng_r) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= 2 * n - 1) return;
+  const int r = id < n - 1 ? rng_r[id] : id - (n - 1);
+  const int esc = r < n - 1 ? rchild[r] : -1;
+  nodes[id].meta = (nodes[id].meta & 0x80000000u) | (uint32_t)(esc + 1);
+}
+
+void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* order, int n, const void* boxes, const uint8_t* act,
+                 void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits, hipStream_t s) {
+  if (n < 1) return;
+  if (n > 1) fill<int>(visits, n - 1, 0, s);
+  const int g = (n + 255) / 256;
+  const int g2 = (2 * n - 1 + 255) / 256;
+  if (dim == 3) {
+    k_lbvh_build<3><<<g, 256, 0, s>>>(keys, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild, rchild,
+                                      rng_l, rng_r, visits);
+    k_lbvh_ropes<3><<<g2, 256, 0, s>>>(n, (NodeF<3>*)nodes, rchild, rng_r);
+  } else {
+    k_lbvh_build<2><<<g, 256, 0, s>>>(keys, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild, rchild,
+                                      rng_l, rng_r, visits);
+    k_lbvh_ropes<2><<<g2, 256, 0, s>>>(n, (NodeF<2>*)nodes, rchild, rng_r);
+  }
+}
+
+}  // namespace mclg

@@ -1,0 +1,121 @@
+// Shared device/host definitions for the gfx950 collision kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace mclg {
+
+constexpr int kWave = 64;  // CDNA wavefront width
+
+// Packed vertex record: one record per vertex holding both time steps, so a
+// stencil gathers one contiguous 64 B (3D) or 32 B (2D) record per vertex.
+//   3D: x0 y0 z0 pad x1 y1 z1 pad      2D: x0 y0 x1 y1
+template <int D> struct VRec;
+template <> struct VRec<3> { static constexpr int stride = 8; static constexpr int t1 = 4; };
+template <> struct VRec<2> { static constexpr int stride = 4; static constexpr int t1 = 2; };
+
+// FP64 leaf box: swept, t0, t1 (lo then hi) — 6*D doubles, reference layout of
+// mcl::BoundingBox (BVHLeaf.hpp:12-29) flattened.
+template <int D> struct LeafBox { double sw_lo[D], sw_hi[D], t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D]; };
+
+// Traversal node: the three boxes rounded outward to FP32 (conservative for
+// the culling test), the first child (internal) or primitive index (leaf),
+// and meta = (escape+1) | active<<31 where escape 0 means "end of traversal".
+template <int D> struct alignas(16) NodeF;
+template <> struct alignas(16) NodeF<3> {
+  float sw_lo[3], sw_hi[3], t0_lo[3], t0_hi[3], t1_lo[3], t1_hi[3];
+  int child;
+  uint32_t meta;
+};
+template <> struct alignas(16) NodeF<2> {
+  float sw_lo[2], sw_hi[2], t0_lo[2], t0_hi[2], t1_lo[2], t1_hi[2];
+  int child;
+  uint32_t meta;
+  int pad[2];
+};
+
+// Device counters of one traversal (all 64-bit).
+enum Counter : int {
+  C_LEAF_PAIRS = 0,
+  C_CANDIDATES,
+  C_HITS,
+  C_DISCRETE,
+  C_DIST_EVALS,
+  C_ERRORS,     // kernels that detected an inconsistent state
+  C_WORK_HEAD,  // work-queue head of a persistent kernel
+  C_COUNT = 8
+};
+
+// ---- wave helpers ---------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0)); }
+__device__ __forceinline__ int popc_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+}
+
+// Wave-aggregated append: every active lane with pred gets a slot in a global
+// list whose length lives in *counter.  Returns the slot (or -1).
+__device__ __forceinline__ long long wave_append(bool pred, unsigned long long* counter) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return -1;
+  const int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  return pred ? (long long)(base + popc_below(m)) : -1;
+}
+
+// Order-preserving map of a double onto uint64 (for atomic min/max).
+__host__ __device__ __forceinline__ uint64_t dbl_to_ord(double x) {
+  uint64_t b;
+  __builtin_memcpy(&b, &x, 8);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double ord_to_dbl(uint64_t o) {
+  uint64_t b = (o >> 63) ? (o & 0x7fffffffffffffffull) : ~o;
+  double x;
+  __builtin_memcpy(&x, &b, 8);
+  return x;
+}
+
+// One-ulp steps of a float toward -inf / +inf (finite or infinite input).
+__device__ __forceinline__ float step_down(float f) {
+  if (f == 0.0f) return -__int_as_float(1);
+  const int b = __float_as_int(f);
+  return __int_as_float(f > 0.0f ? b - 1 : b + 1);
+}
+__device__ __forceinline__ float step_up(float f) {
+  if (f == 0.0f) return __int_as_float(1);
+  const int b = __float_as_int(f);
+  return __int_as_float(f > 0.0f ? b + 1 : b - 1);
+}
+// FP32 rounding toward -inf / +inf of a double (outward box rounding).
+__device__ __forceinline__ float f32_down(double x) {
+  float f = (float)x;
+  if ((double)f > x) f = step_down(f);
+  return f;
+}
+__device__ __forceinline__ float f32_up(double x) {
+  float f = (float)x;
+  if ((double)f < x) f = step_up(f);
+  return f;
+}
+
+// blockIdx remapped so that each XCD (blocks are dealt round-robin to the 8
+// XCDs) receives a contiguous range of logical blocks: neighbouring work (in
+// Morton order) then shares one L2.  The grid must be a multiple of 8 blocks
+// (launchers round up with xcd_grid); the map is then a bijection on
+// [0, gridDim.x) and logical blocks past the work simply find no items.
+__device__ __forceinline__ int xcd_block() {
+  const int b = blockIdx.x;
+  const int per = gridDim.x / 8;
+  return (b % 8) * per + b / 8;
+}
+inline int xcd_grid(long long items, int per_block) {
+  long long b = (items + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  return (int)((b + 7) / 8 * 8);
+}
+
+}  // namespace mclg

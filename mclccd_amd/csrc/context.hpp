@@ -1,0 +1,88 @@
+// Host state of one collision context (one mcl::BVHTree on one GPU).
+#pragma once
+#include <cstdarg>
+#include <string>
+
+#include "../../include/mclg.h"
+#include "common.hpp"
+
+namespace mclg {
+
+constexpr int kNumEvents = 12;
+
+struct Ctx {
+  int dim = 3;
+  int device = 0;
+  int num_cus = 256;
+  int blocks_per_cu = 8;
+  mclg_options opt{};
+  hipStream_t own_stream = nullptr, stream = nullptr;
+  hipEvent_t ev[kNumEvents] = {};
+  bool timing = false;
+  double times[MCLG_NUM_TIMINGS] = {};
+  std::string err;
+
+  // tree state
+  int nv = 0, np = 0, pdim = 0;
+  bool have_tree = false;
+  bool rank_set = false;
+  bool keep_cand = false, keep_pairs = false;
+  mclg_counts counts{0, 0, 0, 0, 0};
+
+  // device buffers (grown on demand; cap in bytes)
+  double* d_rec = nullptr; size_t cap_rec = 0;
+  uint8_t* d_flags = nullptr; size_t cap_flags = 0;
+  int* d_owner = nullptr; size_t cap_owner = 0;
+  unsigned long long* d_ekeys = nullptr; size_t cap_ekeys = 0;
+  int* d_evals = nullptr; size_t cap_evals = 0;
+  void* d_boxes = nullptr; size_t cap_boxes = 0;
+  uint8_t* d_act = nullptr; size_t cap_act = 0;
+  unsigned long long* d_cbounds = nullptr; size_t cap_cbounds = 0;
+  uint32_t* d_keys = nullptr; size_t cap_keys = 0;
+  uint32_t* d_order = nullptr; size_t cap_order = 0;
+  uint32_t* d_ktmp = nullptr; size_t cap_ktmp = 0;
+  uint32_t* d_vtmp = nullptr; size_t cap_vtmp = 0;
+  uint32_t* d_hist = nullptr; size_t cap_hist = 0;
+  void* d_nodes = nullptr; size_t cap_nodes = 0;
+  int* d_lchild = nullptr; size_t cap_lchild = 0;
+  int* d_rchild = nullptr; size_t cap_rchild = 0;
+  int* d_rngl = nullptr; size_t cap_rngl = 0;
+  int* d_rngr = nullptr; size_t cap_rngr = 0;
+  int* d_visits = nullptr; size_t cap_visits = 0;
+  unsigned long long* d_counters = nullptr; size_t cap_counters = 0;
+  int2* d_pairs = nullptr; size_t cap_pairs_bytes = 0;
+  unsigned long long pair_cap = 0;
+  int4* d_hit_sten = nullptr; size_t cap_hit_sten = 0;
+  uint8_t* d_hit_vf = nullptr; size_t cap_hit_vf = 0;
+  double* d_hit_toi = nullptr; size_t cap_hit_toi = 0;
+  unsigned long long hit_cap = 0;
+  int2* d_disc = nullptr; size_t cap_disc = 0;
+  unsigned long long disc_cap = 0;
+  int4* d_cand_sten = nullptr; size_t cap_cand_sten = 0;
+  uint8_t* d_cand_vf = nullptr; size_t cap_cand_vf = 0;
+  double* d_cand_toi = nullptr; size_t cap_cand_toi = 0;
+  unsigned long long cand_cap = 0;
+  int* d_rank = nullptr; size_t cap_rank = 0;
+  int* d_minmax = nullptr; size_t cap_minmax = 0;
+  double* d_stage_v0 = nullptr; size_t cap_stage_v0 = 0;
+  double* d_stage_v1 = nullptr; size_t cap_stage_v1 = 0;
+  int* d_stage_p = nullptr; size_t cap_stage_p = 0;
+  int* d_stage_act = nullptr; size_t cap_stage_act = 0;
+
+  // pinned host mirrors
+  unsigned long long* h_counters = nullptr;
+  int* h_minmax = nullptr;
+
+  ~Ctx();
+  int fail(int code, const char* fmt, ...);
+  int hip(hipError_t e, const char* what);
+  int grow(void** p, size_t* cap, size_t need, const char* what);
+  void mark(int i);
+  int check_indices(const int* P, long long n, int nv, const char* what);
+  int read_counters();
+  int stage(const double* V0, const double* V1, int nv, const int* P, long long nidx, const int* active);
+  int update(const double* V0, const double* V1, int nv, const int* P, int np, int pdim, const int* active);
+  int traverse(const double* V0, const double* V1, const int* P, unsigned flags);
+};
+
+}  // namespace mclg

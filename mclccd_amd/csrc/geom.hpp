@@ -1,0 +1,427 @@
+// FP64 narrow-phase geometry for gfx950: VF/EE distances, ACCD, discrete
+// triangle-triangle (3D) and segment-segment (2D) tests.
+//
+// Every expression keeps the reference's evaluation order (x86-64 Release, no
+// FMA): dot(a,b) = (a0*b0 + a1*b1) + a2*b2, norm = sqrt(dot), element-wise
+// left to right.  The translation unit is compiled with -ffp-contract=off and
+// the pragma below; f64 '/' and sqrt lower to the correctly rounded sequences.
+// Restructured for SIMD execution: each projection region selects a
+// (numerator, denominator) pair so one division serves every branch; the
+// triangle-triangle canonical permutation is an index selection followed by a
+// single min-max check.
+#pragma once
+#include "common.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mclg {
+
+struct d3 { double x, y, z; };
+struct d2 { double x, y; };
+
+__device__ __forceinline__ d3 sub(const d3& a, const d3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d2 sub(const d2& a, const d2& b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ double dot(const d3& a, const d3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ double dot(const d2& a, const d2& b) { return a.x * b.x + a.y * b.y; }
+// libstdc++ std::max / std::min (NaN and signed zero as the reference sees them)
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+
+// ---------------------------------------------------------------------------
+// VF distance: |q - closest point of triangle (a,b,c)|
+// (Projection.hpp:60-145 then NarrowPhase.cpp:740-741)
+// ---------------------------------------------------------------------------
+enum : int { R_S = 0, R_T = 1, R_IN = 2, R_EDGE = 3 };
+
+__device__ __forceinline__ double dist_vf(const d3& q, const d3& a, const d3& b, const d3& c) {
+  const d3 e0 = sub(b, a), e1 = sub(c, a), w = sub(a, q);
+  const double A = dot(e0, e0), B = dot(e0, e1), C = dot(e1, e1);
+  const double Dd = dot(e0, w), E = dot(e1, w);
+  const double det = A * C - B * B;
+  double s = B * E - C * Dd;
+  double t = B * Dd - A * E;
+  double num, den;
+  int kind;
+  if (s + t < det) {
+    if (s < 0) {
+      if (t < 0 && Dd < 0) { kind = R_S; num = -Dd; den = A; }
+      else { kind = R_T; num = -E; den = C; }
+    } else if (t < 0) { kind = R_S; num = -Dd; den = A; }
+    else { kind = R_IN; num = 1.0; den = det; }
+  } else {
+    const double ee = A - 2.0 * B + C;
+    if (s < 0) {
+      const double t0 = B + Dd, t1 = C + E;
+      if (t1 > t0) { kind = R_EDGE; num = t1 - t0; den = ee; }
+      else { kind = R_T; num = -E; den = C; }
+    } else if (t < 0) {
+      if (A + Dd > B + E) { kind = R_EDGE; num = C + E - B - Dd; den = ee; }
+      else { kind = R_S; num = -E; den = C; }
+    } else { kind = R_EDGE; num = C + E - B - Dd; den = ee; }
+  }
+  const double r = num / den;
+  const double rc = r < 0 ? 0.0 : (r > 1 ? 1.0 : r);  // clamp_zero_one
+  if (kind == R_S) { s = rc; t = 0.0; }
+  else if (kind == R_T) { s = 0.0; t = rc; }
+  else if (kind == R_IN) { s *= r; t *= r; }
+  else { s = rc; t = 1.0 - s; }
+  const d3 pt = {(a.x + e0.x * s) + e1.x * t, (a.y + e0.y * s) + e1.y * t, (a.z + e0.z * s) + e1.z * t};
+  const d3 dv = sub(q, pt);
+  return __builtin_sqrt(dot(dv, dv));
+}
+
+// ---------------------------------------------------------------------------
+// EE distance: |shortest vector between segments p and q|
+// (Projection.hpp:241-302 then NarrowPhase.cpp:745-746)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double clamp01(double x) { return smin(1.0, smax(x, 0.0)); }
+
+__device__ __forceinline__ double dist_ee(const d3& p0, const d3& p1, const d3& q0, const d3& q1) {
+  const d3 d1 = sub(p1, p0), d2 = sub(q1, q0), r = sub(p0, q0);
+  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const double c = dot(d1, r), b = dot(d1, d2);
+  const double denom = a * e - b * b;
+  double s = 0.0, t;
+  if (denom != 0.0) s = clamp01((b * f - c * e) / denom);
+  const double tnom = b * s + f;
+  double num, den;
+  int kind;  // 0: t=0, s=clamp(-c/a); 1: t=1, s=clamp((b-c)/a); 2: t=tnom/e
+  if (tnom < 0 || e == 0) { kind = 0; num = -c; den = a; t = 0.0; }
+  else if (tnom > e) { kind = 1; num = b - c; den = a; t = 1.0; }
+  else { kind = 2; num = tnom; den = e; }
+  if (kind == 2 || a != 0.0) {
+    const double q = num / den;
+    if (kind == 2) t = q;
+    else s = clamp01(q);
+  } else {
+    s = 0.0;
+  }
+  const d3 v = {(q0.x + t * d2.x) - (p0.x + s * d1.x), (q0.y + t * d2.y) - (p0.y + s * d1.y),
+                (q0.z + t * d2.z) - (p0.z + s * d1.z)};
+  return __builtin_sqrt(dot(v, v));
+}
+
+// ---------------------------------------------------------------------------
+// 2D VF distance: |p - closest point of segment (e0,e1)| (Projection.hpp:193-204)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double dist_pe2(const d2& p, const d2& e0, const d2& e1) {
+  const d2 e = sub(e1, e0);
+  const double len2 = dot(e, e);
+  d2 pt = e0;
+  if (len2 > 0.0 || !(len2 <= 0.0)) {  // i.e. not (len2 <= 0)
+    const d2 pe = sub(p, e0);
+    const double t = dot(pe, e) / len2;
+    if (t < 0.0) pt = e0;
+    else if (t > 1.0) pt = e1;
+    else pt = {e0.x + t * e.x, e0.y + t * e.y};
+  }
+  const d2 dv = sub(p, pt);
+  return __builtin_sqrt(dot(dv, dv));
+}
+
+// ---------------------------------------------------------------------------
+// ACCD (NarrowPhase.cpp:634-711), s = 1, t_c = 1, max_iter = 1000.
+//
+// Work the reference does whose result provably cannot change the outcome is
+// skipped: (1) iteration 0 moves x by t_l0 = 0*(...)/(...) = +0, so when t_l0
+// is exactly zero and every displacement is finite x is unchanged up to the
+// sign of zero coordinates, which no comparison and no squared norm can see —
+// its distance equals d0 and is not recomputed; (2) while toi is not > 0 the
+// break test cannot fire, so when toi + t_l > 1 the reference's next distance
+// is unused and the miss is returned without evaluating it.  Returns 1 on hit
+// (toi set), 0 otherwise; *evals counts the distance evaluations performed.
+// ---------------------------------------------------------------------------
+template <int D, bool VF> struct Stencil;
+
+template <bool VF> struct Stencil<3, VF> {
+  d3 x[4], p[4];
+  __device__ __forceinline__ double dist() const {
+    if (VF) return dist_vf(x[0], x[1], x[2], x[3]);
+    return dist_ee(x[0], x[1], x[2], x[3]);
+  }
+  __device__ __forceinline__ void step(double tl) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      x[i].x = x[i].x + tl * p[i].x;
+      x[i].y = x[i].y + tl * p[i].y;
+      x[i].z = x[i].z + tl * p[i].z;
+    }
+  }
+  // v0[i], v1[i] are the stencil's vertices at t0 and t1; returns l_p, sets fin
+  __device__ __forceinline__ double setup(const d3* v0, const d3* v1, bool& fin) {
+    const double w = 1.0 / 4.0;
+    d3 pb = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      p[i] = sub(v1[i], v0[i]);
+      x[i] = v0[i];
+      pb.x = pb.x + w * p[i].x;
+      pb.y = pb.y + w * p[i].y;
+      pb.z = pb.z + w * p[i].z;
+    }
+    double lf = 0.0, ls = 0.0, sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      p[i] = sub(p[i], pb);
+      sum += (p[i].x + p[i].y) + p[i].z;
+      const double n = __builtin_sqrt(dot(p[i], p[i]));
+      const bool first = VF ? (i == 0) : (i <= 1);
+      if (first) lf = smax(lf, n);
+      else ls = smax(ls, n);
+    }
+    fin = __builtin_isfinite(sum);
+    return lf + ls;
+  }
+};
+
+template <> struct Stencil<2, true> {
+  d2 x[3], p[3];
+  __device__ __forceinline__ double dist() const { return dist_pe2(x[0], x[1], x[2]); }
+  __device__ __forceinline__ void step(double tl) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      x[i].x = x[i].x + tl * p[i].x;
+      x[i].y = x[i].y + tl * p[i].y;
+    }
+  }
+  __device__ __forceinline__ double setup(const d2* v0, const d2* v1, bool& fin) {
+    const double w = 1.0 / 3.0;
+    d2 pb = {0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      p[i] = sub(v1[i], v0[i]);
+      x[i] = v0[i];
+      pb.x = pb.x + w * p[i].x;
+      pb.y = pb.y + w * p[i].y;
+    }
+    double lf = 0.0, ls = 0.0, sum = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      p[i] = sub(p[i], pb);
+      sum += p[i].x + p[i].y;
+      const double n = __builtin_sqrt(dot(p[i], p[i]));
+      if (i == 0) lf = smax(lf, n);
+      else ls = smax(ls, n);
+    }
+    fin = __builtin_isfinite(sum);
+    return lf + ls;
+  }
+};
+
+template <int D, bool VF, typename V>
+__device__ __forceinline__ int accd(const V* v0, const V* v1, double xsi, double& toi, int& evals) {
+  Stencil<D, VF> st;
+  bool fin;
+  const double l_p = st.setup(v0, v1, fin);
+  if (l_p <= 0) return 0;
+  double d = st.dist();
+  evals = 1;
+  const double g = 1.0 * (d * d - xsi * xsi) / (d - xsi);
+  toi = 0.0;
+  const double tl0 = (1.0 - 1.0) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  if (d <= xsi) return 1;
+  // iteration 0
+  if (!(tl0 == 0.0 && fin)) {
+    st.step(tl0);
+    d = st.dist();
+    ++evals;
+  }
+  toi += tl0;
+  if (toi > 1.0) return 0;
+  double tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  for (int iter = 1; iter < 1000; ++iter) {
+    if (!(toi > 0.0)) {
+      const double nt = toi + tl;
+      if (nt > 1.0) return 0;
+      st.step(tl);
+      d = st.dist();
+      ++evals;
+      toi = nt;
+    } else {
+      st.step(tl);
+      d = st.dist();
+      ++evals;
+      const double eps = (d * d - xsi * xsi) / (d + xsi);
+      if (eps < g) return 1;
+      toi += tl;
+      if (toi > 1.0) return 0;
+    }
+    tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  }
+  return 1;  // max_iter reached: the reference returns true (its assert is compiled out)
+}
+
+// ---------------------------------------------------------------------------
+// Discrete triangle-triangle overlap at t1 (Guigue & Devillers 2003, as in
+// tt_isect.hpp:71-270,499-585): orientation predicates, canonical permutation,
+// min-max interval check; coplanar pairs are projected to 2D.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ d3 cross(const d3& a, const d3& b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// plain left-to-right sum (the predicate's own DOT)
+__device__ __forceinline__ double dot_lr(const d3& a, const d3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double orient2(const d2& a, const d2& b, const d2& c) {
+  return (a.x - c.x) * (b.y - c.y) - (a.y - c.y) * (b.x - c.x);
+}
+
+// tt_isect.hpp:502-534
+__device__ inline int isect_vertex2(const d2& P1, const d2& Q1, const d2& R1, const d2& P2, const d2& Q2,
+                                    const d2& R2) {
+  if (orient2(R2, P2, Q1) >= 0.0) {
+    if (orient2(R2, Q2, Q1) <= 0.0) {
+      if (orient2(P1, P2, Q1) > 0.0) return orient2(P1, Q2, Q1) <= 0.0;
+      if (orient2(P1, P2, R1) >= 0.0) return orient2(Q1, R1, P2) >= 0.0;
+      return 0;
+    }
+    if (orient2(P1, Q2, Q1) <= 0.0 && orient2(R2, Q2, R1) <= 0.0) return orient2(Q1, R1, Q2) >= 0.0;
+    return 0;
+  }
+  if (orient2(R2, P2, R1) >= 0.0) {
+    if (orient2(Q1, R1, R2) >= 0.0) return orient2(P1, P2, R1) >= 0.0;
+    if (orient2(Q1, R1, Q2) >= 0.0) return orient2(R2, R1, Q2) >= 0.0;
+  }
+  return 0;
+}
+
+// tt_isect.hpp:538-549
+__device__ inline int isect_edge2(const d2& P1, const d2& Q1, const d2& R1, const d2& P2, const d2& R2) {
+  if (orient2(R2, P2, Q1) >= 0.0) {
+    if (orient2(P1, P2, Q1) >= 0.0) return orient2(P1, Q1, R2) >= 0.0;
+    if (orient2(Q1, R1, P2) >= 0.0) return orient2(R1, P1, P2) >= 0.0;
+    return 0;
+  }
+  if (orient2(R2, P2, R1) >= 0.0 && orient2(P1, P2, R1) >= 0.0) {
+    if (orient2(P1, R1, R2) >= 0.0) return 1;
+    return orient2(Q1, R1, R2) >= 0.0;
+  }
+  return 0;
+}
+
+// tt_isect.hpp:553-569 (counter-clockwise T1)
+__device__ inline int ccw_tri_tri2(const d2& p1, const d2& q1, const d2& r1, const d2& p2, const d2& q2,
+                                   const d2& r2) {
+  if (orient2(p2, q2, p1) >= 0.0) {
+    if (orient2(q2, r2, p1) >= 0.0) {
+      if (orient2(r2, p2, p1) >= 0.0) return 1;
+      return isect_edge2(p1, q1, r1, p2, r2);
+    }
+    if (orient2(r2, p2, p1) >= 0.0) return isect_edge2(p1, q1, r1, r2, q2);
+    return isect_vertex2(p1, q1, r1, p2, q2, r2);
+  }
+  if (orient2(q2, r2, p1) >= 0.0) {
+    if (orient2(r2, p2, p1) >= 0.0) return isect_edge2(p1, q1, r1, q2, p2);
+    return isect_vertex2(p1, q1, r1, q2, r2, p2);
+  }
+  return isect_vertex2(p1, q1, r1, r2, p2, q2);
+}
+
+// tt_isect.hpp:572-585
+__device__ inline int tri_tri2(const d2& p1, const d2& q1, const d2& r1, const d2& p2, const d2& q2,
+                               const d2& r2) {
+  const bool cw1 = orient2(p1, q1, r1) < 0.0;
+  const bool cw2 = orient2(p2, q2, r2) < 0.0;
+  return ccw_tri_tri2(p1, cw1 ? r1 : q1, cw1 ? q1 : r1, p2, cw2 ? r2 : q2, cw2 ? q2 : r2);
+}
+
+// tt_isect.hpp:215-270 — project on the plane maximising the projected area
+__device__ inline int coplanar_tri_tri3(const d3& p1, const d3& q1, const d3& r1, const d3& p2, const d3& q2,
+                                        const d3& r2, const d3& N1) {
+  const double nx = N1.x < 0 ? -N1.x : N1.x;
+  const double ny = N1.y < 0 ? -N1.y : N1.y;
+  const double nz = N1.z < 0 ? -N1.z : N1.z;
+  if (nx > nz && nx >= ny)  // YZ, with p and q exchanged
+    return tri_tri2({q1.z, q1.y}, {p1.z, p1.y}, {r1.z, r1.y}, {q2.z, q2.y}, {p2.z, p2.y}, {r2.z, r2.y});
+  if (ny > nz && ny >= nx)  // XZ, with p and q exchanged
+    return tri_tri2({q1.x, q1.z}, {p1.x, p1.z}, {r1.x, r1.z}, {q2.x, q2.z}, {p2.x, p2.z}, {r2.x, r2.z});
+  return tri_tri2({p1.x, p1.y}, {q1.x, q1.y}, {r1.x, r1.y}, {p2.x, p2.y}, {q2.x, q2.y}, {r2.x, r2.y});
+}
+
+// tt_isect.hpp:91-102
+__device__ __forceinline__ int check_min_max(const d3& p1, const d3& q1, const d3& r1, const d3& p2, const d3& q2,
+                                             const d3& r2) {
+  d3 N = cross(sub(p2, q1), sub(p1, q1));
+  if (dot_lr(sub(q2, q1), N) > 0.0) return 0;
+  N = cross(sub(p2, p1), sub(r1, p1));
+  if (dot_lr(sub(r2, p1), N) > 0.0) return 0;
+  return 1;
+}
+
+__device__ __forceinline__ d3 pick3(int i, const d3& a, const d3& b, const d3& c) { return i == 0 ? a : (i == 1 ? b : c); }
+__device__ __forceinline__ double pickd(int i, double a, double b, double c) { return i == 0 ? a : (i == 1 ? b : c); }
+
+// tt_isect.hpp:141-211 + 108-130.  T1 = (p1,q1,r1), T2 = (p2,q2,r2).
+__device__ inline int tri_tri3(const d3& p1, const d3& q1, const d3& r1, const d3& p2, const d3& q2, const d3& r2) {
+  const d3 N2 = cross(sub(p2, r2), sub(q2, r2));
+  const double dp1 = dot_lr(sub(p1, r2), N2), dq1 = dot_lr(sub(q1, r2), N2), dr1 = dot_lr(sub(r1, r2), N2);
+  if (((dp1 * dq1) > 0.0) && ((dp1 * dr1) > 0.0)) return 0;
+  const d3 N1 = cross(sub(q1, p1), sub(r1, p1));
+  const double dp2 = dot_lr(sub(p2, r1), N1), dq2 = dot_lr(sub(q2, r1), N1), dr2 = dot_lr(sub(r2, r1), N1);
+  if (((dp2 * dq2) > 0.0) && ((dp2 * dr2) > 0.0)) return 0;
+
+  // T1 rotation: 0 = (p1,q1,r1), 1 = (q1,r1,p1), 2 = (r1,p1,q1); sw = T2 becomes (p2,r2,q2)
+  int rot1, sw;
+  if (dp1 > 0.0) {
+    if (dq1 > 0.0) { rot1 = 2; sw = 1; }
+    else if (dr1 > 0.0) { rot1 = 1; sw = 1; }
+    else { rot1 = 0; sw = 0; }
+  } else if (dp1 < 0.0) {
+    if (dq1 < 0.0) { rot1 = 2; sw = 0; }
+    else if (dr1 < 0.0) { rot1 = 1; sw = 0; }
+    else { rot1 = 0; sw = 1; }
+  } else if (dq1 < 0.0) {
+    if (dr1 >= 0.0) { rot1 = 1; sw = 1; }
+    else { rot1 = 0; sw = 0; }
+  } else if (dq1 > 0.0) {
+    if (dr1 > 0.0) { rot1 = 0; sw = 1; }
+    else { rot1 = 1; sw = 0; }
+  } else {
+    if (dr1 > 0.0) { rot1 = 2; sw = 0; }
+    else if (dr1 < 0.0) { rot1 = 2; sw = 1; }
+    else return coplanar_tri_tri3(p1, q1, r1, p2, q2, r2, N1);
+  }
+  const d3 a1 = pick3(rot1, p1, q1, r1), b1 = pick3(rot1, q1, r1, p1), c1 = pick3(rot1, r1, p1, q1);
+  const d3 a2 = p2, b2 = sw ? r2 : q2, c2 = sw ? q2 : r2;
+  const double da = dp2, db = sw ? dr2 : dq2, dc = sw ? dq2 : dr2;
+
+  // TRI_TRI_3D: swap T1's last two (s1) and rotate T2 (rot2: 0 = (a,b,c), 1 = (c,a,b), 2 = (b,c,a))
+  int s1, rot2;
+  if (da > 0.0) {
+    if (db > 0.0) { s1 = 1; rot2 = 1; }
+    else if (dc > 0.0) { s1 = 1; rot2 = 2; }
+    else { s1 = 0; rot2 = 0; }
+  } else if (da < 0.0) {
+    if (db < 0.0) { s1 = 0; rot2 = 1; }
+    else if (dc < 0.0) { s1 = 0; rot2 = 2; }
+    else { s1 = 1; rot2 = 0; }
+  } else if (db < 0.0) {
+    if (dc >= 0.0) { s1 = 1; rot2 = 2; }
+    else { s1 = 0; rot2 = 0; }
+  } else if (db > 0.0) {
+    if (dc > 0.0) { s1 = 1; rot2 = 0; }
+    else { s1 = 0; rot2 = 2; }
+  } else {
+    if (dc > 0.0) { s1 = 0; rot2 = 1; }
+    else if (dc < 0.0) { s1 = 1; rot2 = 1; }
+    else return coplanar_tri_tri3(a1, b1, c1, a2, b2, c2, N1);
+  }
+  const d3 u2 = pick3(rot2, a2, c2, b2), v2 = pick3(rot2, b2, a2, c2), w2 = pick3(rot2, c2, b2, a2);
+  return check_min_max(a1, s1 ? c1 : b1, s1 ? b1 : c1, u2, v2, w2);
+}
+
+// NarrowPhase.cpp:229-257 — 2D segment-segment test
+__device__ __forceinline__ int edge_edge2(const d2& p0, const d2& p1, const d2& q0, const d2& q1) {
+  const double eps = 2.220446049250313080847263336181640625e-16;  // DBL_EPSILON
+  const d2 n = sub(q0, p0), r = sub(p1, p0), s = sub(q1, q0);
+  const double rxs = r.x * s.y - r.y * s.x;
+  if (__builtin_fabs(rxs) < eps) return 0;
+  const double nxr = n.x * r.y - n.y * r.x;
+  if (__builtin_fabs(nxr) < eps)
+    return ((q0.x - p0.x < 0) != (q0.x - p1.x < 0)) || ((q0.y - p0.y < 0) != (q0.y - p1.y < 0));
+  const double nxs = n.x * s.y - n.y * s.x;
+  const double inv = 1.0 / rxs;
+  const double t = nxs * inv, u = nxr * inv;
+  return (t >= 0) && (t <= 1) && (u >= 0) && (u <= 1);
+}
+
+}  // namespace mclg

@@ -1,0 +1,246 @@
+"""Python mirror of ``mcl::BVHTree<double,DIM>`` over the gfx950 library.
+
+Same names and meaning as the reference (include/MCL/BVHTree.hpp:30-148):
+``options`` (box_eta, vf_ccd_eta, ee_ccd_eta, reptri, threaded, discrete,
+continuous, vf_one_sided, ee_robust), ``update(V0, V1, P, active)``,
+``traverse(V0, V1, P)`` and the four callbacks ``append_pair(sten, is_vf, toi)``,
+``append_discrete(p0, p1) -> bool``, ``filter_pair(sten, is_vf) -> bool`` and
+``narrow_phase(sten, is_vf) -> toi``.
+
+The GPU computes every result of a traverse; the callbacks are then replayed
+on the calling thread, serially (the reference calls them concurrently from TBB
+workers, so any serial order is one it can produce):
+  1. continuous: for each candidate stencil, filter_pair (if set) may skip it;
+     the toi is narrow_phase(sten, is_vf) when that callback is set, else the
+     GPU's ACCD result; toi >= 0 calls append_pair.  Without filter_pair and
+     narrow_phase only the hits are read back.
+  2. discrete: append_discrete(p0, p1) per discrete hit; a True return stops
+     the replay (the reference's stop flag, BVHTree.cpp:236-239,280-281).
+
+Inputs may be numpy arrays (copied to the GPU) or CUDA torch tensors (used in
+place: float64 vertices, int32 indices, contiguous).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+
+
+class Options:
+    """BVHTree::Options (BVHTree.hpp:42-64) with the reference defaults."""
+
+    __slots__ = ("box_eta", "vf_ccd_eta", "ee_ccd_eta", "reptri", "threaded", "discrete", "continuous",
+                 "vf_one_sided", "ee_robust")
+
+    def __init__(self):
+        self.box_eta = 1e-6
+        self.vf_ccd_eta = 1e-6
+        self.ee_ccd_eta = 1e-6
+        self.reptri = True
+        self.threaded = True
+        self.discrete = True
+        self.continuous = True
+        self.vf_one_sided = False
+        self.ee_robust = True
+
+    def to_c(self) -> _lib.Options:
+        return _lib.Options(self.box_eta, self.vf_ccd_eta, self.ee_ccd_eta, int(self.reptri), int(self.threaded),
+                            int(self.discrete), int(self.continuous), int(self.vf_one_sided), int(self.ee_robust))
+
+    def from_c(self, o: _lib.Options):
+        for f in self.__slots__:
+            v = getattr(o, f)
+            setattr(self, f, bool(v) if isinstance(getattr(self, f), bool) else v)
+
+
+def _is_device(x) -> bool:
+    return hasattr(x, "data_ptr") and getattr(x, "is_cuda", False)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if _is_device(a):
+        return C.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _host_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _host_i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _dev_check(t, dtype_name):
+    if not t.is_contiguous():
+        raise ValueError("device inputs must be contiguous")
+    if str(t.dtype) != dtype_name:
+        raise ValueError(f"device input must be {dtype_name}, got {t.dtype}")
+    return t
+
+
+class BVHTree:
+    """``mcl::BVHTree<double, dim>`` on one GPU."""
+
+    def __init__(self, dim: int = 3):
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self.dim = dim
+        self.options = Options()
+        self.append_pair = None
+        self.append_discrete = None
+        self.filter_pair = None
+        self.narrow_phase = None
+        h = C.c_void_p()
+        rc = lib.mclg_create(dim, C.byref(h))
+        if rc != _lib.OK:
+            raise _lib.MclgError(rc, "mclg_create failed (no HIP device?)")
+        self._h = h
+        self.nv = 0
+        self.np = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib.mclg_destroy(h)
+            self._h = None
+
+    # -- plumbing ---------------------------------------------------------
+    def _check(self, rc):
+        if rc != _lib.OK:
+            raise _lib.MclgError(rc, lib.mclg_last_error(self._h).decode())
+
+    def _push_options(self):
+        o = self.options.to_c()
+        self._check(lib.mclg_set_options(self._h, C.byref(o)))
+
+    def _pull_options(self):
+        o = _lib.Options()
+        self._check(lib.mclg_get_options(self._h, C.byref(o)))
+        self.options.from_c(o)
+
+    # -- reference API ------------------------------------------------------
+    def update(self, V0, V1, P, active=None):
+        """BVHTree::update (BVHTree.hpp:73-82): builds the BVH of P's primitives
+        over the swept boxes of V0 -> V1.  Mismatched row counts return silently
+        as in the reference (BVHTree.hpp:160)."""
+        if len(V0) != len(V1):
+            return
+        self._push_options()
+        dev = _is_device(V0)
+        if dev:
+            V0, V1 = _dev_check(V0, "torch.float64"), _dev_check(V1, "torch.float64")
+            P = _dev_check(P, "torch.int32")
+            if active is not None:
+                active = _dev_check(active, "torch.int32")
+                if len(active) != len(V0):
+                    active = None
+            nv, npr, pdim = int(V0.shape[0]), int(P.shape[0]), int(P.shape[1]) if P.ndim == 2 else 1
+            rc = lib.mclg_update_device(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
+        else:
+            V0, V1, P = _host_f64(V0), _host_f64(V1), _host_i32(P)
+            if P.ndim == 1:
+                P = P.reshape(-1, 1)
+            if active is not None:
+                active = _host_i32(active)
+                if len(active) != len(V0):
+                    active = None
+            nv, npr, pdim = V0.shape[0], P.shape[0], P.shape[1]
+            rc = lib.mclg_update(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
+        self._check(rc)
+        self._pull_options()
+        self.nv, self.np = nv, npr
+
+    def traverse(self, V0, V1, P):
+        """BVHTree::traverse (BVHTree.hpp:86-93) with serial callback replay."""
+        if len(V0) != len(V1):
+            return
+        need_cand = self.narrow_phase is not None or self.filter_pair is not None
+        self.run(V0, V1, P, keep_candidates=need_cand)
+        if self.options.continuous and (self.append_pair is not None or need_cand):
+            if need_cand:
+                sten, is_vf, toi = self.candidates()
+                for i in range(len(sten)):
+                    s, v = tuple(int(x) for x in sten[i]), bool(is_vf[i])
+                    if self.filter_pair is not None and self.filter_pair(s, v):
+                        continue
+                    t = self.narrow_phase(s, v) if self.narrow_phase is not None else float(toi[i])
+                    if t >= 0 and self.append_pair is not None:
+                        self.append_pair(s, v, t)
+            else:
+                sten, is_vf, toi = self.hits()
+                for i in range(len(sten)):
+                    self.append_pair(tuple(int(x) for x in sten[i]), bool(is_vf[i]), float(toi[i]))
+        if self.options.discrete and self.append_discrete is not None:
+            for p0, p1 in self.discrete():
+                if self.append_discrete(int(p0), int(p1)):
+                    break
+
+    # -- bulk access (no callbacks) --------------------------------------------
+    def run(self, V0, V1, P, keep_candidates=False, keep_leaf_pairs=False):
+        """GPU traverse only; results stay on the device (see hits(), discrete())."""
+        self._push_options()
+        flags = (_lib.KEEP_CANDIDATES if keep_candidates else 0) | (_lib.KEEP_LEAF_PAIRS if keep_leaf_pairs else 0)
+        if _is_device(V0):
+            V0, V1 = _dev_check(V0, "torch.float64"), _dev_check(V1, "torch.float64")
+            P = _dev_check(P, "torch.int32")
+            rc = lib.mclg_traverse_device(self._h, _ptr(V0), _ptr(V1), _ptr(P), flags)
+        else:
+            V0, V1, P = _host_f64(V0), _host_f64(V1), _host_i32(P)
+            rc = lib.mclg_traverse(self._h, _ptr(V0), _ptr(V1), _ptr(P), flags)
+        self._check(rc)
+
+    def counts(self) -> dict:
+        c = _lib.Counts()
+        self._check(lib.mclg_get_counts(self._h, C.byref(c)))
+        return {f: int(getattr(c, f)) for f, _ in c._fields_}
+
+    def hits(self):
+        n = self.counts()["hits"]
+        s = np.empty((n, 4), np.int32)
+        v = np.empty(n, np.uint8)
+        t = np.empty(n, np.float64)
+        self._check(lib.mclg_copy_hits(self._h, _ptr(s), _ptr(v), _ptr(t), n))
+        return s, v, t
+
+    def discrete(self):
+        n = self.counts()["discrete"]
+        out = np.empty((n, 2), np.int32)
+        self._check(lib.mclg_copy_discrete(self._h, _ptr(out), n))
+        return out
+
+    def candidates(self):
+        n = self.counts()["candidates"]
+        s = np.empty((n, 4), np.int32)
+        v = np.empty(n, np.uint8)
+        t = np.empty(n, np.float64)
+        self._check(lib.mclg_copy_candidates(self._h, _ptr(s), _ptr(v), _ptr(t), n))
+        return s, v, t
+
+    def leaf_pairs(self):
+        n = self.counts()["leaf_pairs"]
+        out = np.empty((n, 2), np.int32)
+        self._check(lib.mclg_copy_leaf_pairs(self._h, _ptr(out), n))
+        return out
+
+    def set_rank(self, rank):
+        """Orient leaf pairs by rank (the reference's KdBVH in-order position)."""
+        if rank is None:
+            self._check(lib.mclg_set_rank(self._h, None, 0))
+            return
+        r = _host_i32(rank)
+        self._check(lib.mclg_set_rank(self._h, _ptr(r), len(r)))
+
+    def timings(self) -> dict:
+        ms = np.zeros(_lib.NUM_TIMINGS)
+        self._check(lib.mclg_get_timings(self._h, _ptr(ms), _lib.NUM_TIMINGS))
+        return dict(zip(_lib.TIMING_NAMES, ms.tolist()))
+
+    def set_stream(self, stream_ptr: int | None):
+        self._check(lib.mclg_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))

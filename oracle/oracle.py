@@ -1,0 +1,226 @@
+"""ctypes wrapper over the CPU oracle (liboracle.so) — TEST INFRASTRUCTURE ONLY.
+
+Used by tests/, ``__graft_entry__.smoke()`` (as the checker) and the CPU
+baseline leg of bench.py.  The product package never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+class OracleOptions(C.Structure):
+    _fields_ = [("box_eta", C.c_double), ("vf_ccd_eta", C.c_double), ("ee_ccd_eta", C.c_double),
+                ("reptri", C.c_int), ("threaded", C.c_int), ("discrete", C.c_int), ("continuous", C.c_int)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++ -O3, no -march)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = C.CDLL(_LIB)
+        P, I, D, LL = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+        L.oracle_create.restype = P
+        L.oracle_create.argtypes = [I]
+        L.oracle_destroy.argtypes = [P]
+        L.oracle_set_options.argtypes = [P, C.POINTER(OracleOptions), I]
+        L.oracle_get_etas.argtypes = [P, P]
+        L.oracle_update.argtypes = [P, P, P, P, I, I, P]
+        L.oracle_traverse.argtypes = [P, P, P, P, I, I]
+        L.oracle_last_ms.restype = D
+        L.oracle_last_ms.argtypes = [P, I]
+        L.oracle_count.restype = LL
+        L.oracle_count.argtypes = [P, I]
+        for f in ("oracle_copy_leaf_pairs", "oracle_copy_discrete", "oracle_copy_rank", "oracle_copy_flags"):
+            getattr(L, f).argtypes = [P, P]
+        L.oracle_copy_candidates.argtypes = [P, P, P]
+        L.oracle_copy_hits.argtypes = [P, P, P, P]
+        L.oracle_copy_boxes.argtypes = [P, P, P]
+        L.oracle_brute_force_pairs.restype = LL
+        L.oracle_brute_force_pairs.argtypes = [P, P, LL]
+        L.oracle_accd.argtypes = [I, P, P, D, I, P, P]
+        L.oracle_pair_distance.restype = D
+        L.oracle_pair_distance.argtypes = [I, P, I]
+        L.oracle_point_on_triangle.argtypes = [P, P, P, P, P]
+        L.oracle_edge_to_edge.argtypes = [P, P, P, P, P, P]
+        L.oracle_point_on_edge2.argtypes = [P, P, P, P]
+        L.oracle_tri_tri.argtypes = [P, P]
+        L.oracle_edge_edge2.argtypes = [P, P]
+        L.oracle_max_threads.restype = I
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class Oracle:
+    """CPU restatement of ``mcl::BVHTree<double,dim>`` (update + traverse)."""
+
+    def __init__(self, dim: int):
+        self.dim = dim
+        self._h = lib().oracle_create(dim)
+        if not self._h:
+            raise ValueError("dim must be 2 or 3")
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.oracle_destroy(h)
+            self._h = None
+
+    def set_options(self, box_eta=1e-6, vf_ccd_eta=1e-6, ee_ccd_eta=1e-6, reptri=True, threaded=True,
+                    discrete=True, continuous=True, nthreads=0):
+        o = OracleOptions(box_eta, vf_ccd_eta, ee_ccd_eta, int(reptri), int(threaded), int(discrete),
+                          int(continuous))
+        lib().oracle_set_options(self._h, C.byref(o), int(nthreads))
+
+    def etas(self):
+        out = np.zeros(3)
+        lib().oracle_get_etas(self._h, _p(out))
+        return out
+
+    def update(self, V0, V1, P, active=None):
+        self._V0, self._V1, self._P = _f64(V0), _f64(V1), _i32(P)
+        self._act = _i32(active) if active is not None else None
+        npr, pdim = self._P.shape
+        lib().oracle_update(self._h, _p(self._V0), _p(self._V1), _p(self._P), npr, pdim, _p(self._act))
+
+    def traverse(self, V0=None, V1=None, P=None, leaf_pairs=True, candidates=True):
+        V0 = self._V0 if V0 is None else _f64(V0)
+        V1 = self._V1 if V1 is None else _f64(V1)
+        P = self._P if P is None else _i32(P)
+        lib().oracle_traverse(self._h, _p(V0), _p(V1), _p(P), int(leaf_pairs), int(candidates))
+
+    def last_ms(self):
+        return lib().oracle_last_ms(self._h, 0), lib().oracle_last_ms(self._h, 1)
+
+    def count(self, which: int) -> int:
+        return int(lib().oracle_count(self._h, which))
+
+    def leaf_pairs(self):
+        out = np.zeros((self.count(0), 2), np.int32)
+        lib().oracle_copy_leaf_pairs(self._h, _p(out))
+        return out
+
+    def candidates(self):
+        n = self.count(1)
+        s = np.zeros((n, 4), np.int32)
+        v = np.zeros(n, np.uint8)
+        lib().oracle_copy_candidates(self._h, _p(s), _p(v))
+        return s, v
+
+    def hits(self):
+        n = self.count(2)
+        s = np.zeros((n, 4), np.int32)
+        v = np.zeros(n, np.uint8)
+        t = np.zeros(n, np.float64)
+        lib().oracle_copy_hits(self._h, _p(s), _p(v), _p(t))
+        return s, v, t
+
+    def discrete(self):
+        out = np.zeros((self.count(3), 2), np.int32)
+        lib().oracle_copy_discrete(self._h, _p(out))
+        return out
+
+    def rank(self):
+        out = np.zeros(self._P.shape[0], np.int32)
+        lib().oracle_copy_rank(self._h, _p(out))
+        return out
+
+    def flags(self):
+        out = np.zeros(self._P.shape[0], np.uint8)
+        lib().oracle_copy_flags(self._h, _p(out))
+        return out
+
+    def boxes(self):
+        n = self._P.shape[0]
+        out = np.zeros((n, 6 * self.dim), np.float64)
+        act = np.zeros(n, np.uint8)
+        lib().oracle_copy_boxes(self._h, _p(out), _p(act))
+        return out, act
+
+    def brute_force_pairs(self):
+        n = int(lib().oracle_brute_force_pairs(self._h, None, 0))
+        out = np.zeros((n, 2), np.int32)
+        lib().oracle_brute_force_pairs(self._h, _p(out), n)
+        return out
+
+
+def accd(dim, verts0, verts1, eta, is_vf=True):
+    """Returns (hit, toi, distance evaluations) of the literal ACCD restatement."""
+    v0, v1 = _f64(verts0), _f64(verts1)
+    toi = C.c_double(-1.0)
+    ev = C.c_longlong(0)
+    h = lib().oracle_accd(dim, _p(v0), _p(v1), float(eta), int(is_vf), C.byref(toi), C.byref(ev))
+    return bool(h), toi.value, ev.value
+
+
+def pair_distance(dim, verts, is_vf=True):
+    v = _f64(verts)
+    return lib().oracle_pair_distance(dim, _p(v), int(is_vf))
+
+
+def point_on_triangle(q, a, b, c):
+    q, a, b, c = (_f64(x) for x in (q, a, b, c))
+    out = np.zeros(3)
+    lib().oracle_point_on_triangle(_p(q), _p(a), _p(b), _p(c), _p(out))
+    return out
+
+
+def edge_to_edge(p0, p1, q0, q1):
+    p0, p1, q0, q1 = (_f64(x) for x in (p0, p1, q0, q1))
+    out = np.zeros(3)
+    bary = np.zeros(4)
+    lib().oracle_edge_to_edge(_p(p0), _p(p1), _p(q0), _p(q1), _p(out), _p(bary))
+    return out, bary
+
+
+def point_on_edge2(p, e0, e1):
+    p, e0, e1 = (_f64(x) for x in (p, e0, e1))
+    out = np.zeros(2)
+    lib().oracle_point_on_edge2(_p(p), _p(e0), _p(e1), _p(out))
+    return out
+
+
+def tri_tri(f0, f1):
+    a, b = _f64(f0).reshape(9), _f64(f1).reshape(9)
+    return bool(lib().oracle_tri_tri(_p(a), _p(b)))
+
+
+def edge_edge2(e0, e1):
+    a, b = _f64(e0).reshape(4), _f64(e1).reshape(4)
+    return bool(lib().oracle_edge_edge2(_p(a), _p(b)))
+
+
+def run_scene(V0, V1, P, theta, dim=None, leaf_pairs=True, candidates=True, nthreads=0, **opts):
+    """update + traverse with box_eta = vf = ee = theta; returns the Oracle."""
+    dim = dim or V0.shape[1]
+    o = Oracle(dim)
+    o.set_options(box_eta=theta, vf_ccd_eta=theta, ee_ccd_eta=theta, nthreads=nthreads, **opts)
+    o.update(V0, V1, P)
+    o.traverse(leaf_pairs=leaf_pairs, candidates=candidates)
+    return o

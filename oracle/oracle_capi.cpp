@@ -1,0 +1,257 @@
+// C entry points over the CPU oracle (mclccd_oracle.hpp) for ctypes.
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the
+// CPU-baseline leg of bench.py, never by the product library.
+#include "mclccd_oracle.hpp"
+
+#include <chrono>
+#include <memory>
+
+namespace {
+
+struct OracleCtx {
+  int dim;
+  std::unique_ptr<mclo::Tree<2>> t2;
+  std::unique_ptr<mclo::Tree<3>> t3;
+  double last_update_ms = 0, last_traverse_ms = 0;
+};
+
+template <int D> mclo::Tree<D>& tree(OracleCtx* c);
+template <> mclo::Tree<2>& tree<2>(OracleCtx* c) { return *c->t2; }
+template <> mclo::Tree<3>& tree<3>(OracleCtx* c) { return *c->t3; }
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+}  // namespace
+
+struct oracle_options {
+  double box_eta, vf_ccd_eta, ee_ccd_eta;
+  int reptri, threaded, discrete, continuous;
+};
+
+template <int D> static void set_opts(OracleCtx* c, const oracle_options* o, int nthreads) {
+  auto& t = tree<D>(c);
+  t.options.box_eta = o->box_eta;
+  t.options.vf_ccd_eta = o->vf_ccd_eta;
+  t.options.ee_ccd_eta = o->ee_ccd_eta;
+  t.options.reptri = o->reptri != 0;
+  t.options.threaded = o->threaded != 0;
+  t.options.discrete = o->discrete != 0;
+  t.options.continuous = o->continuous != 0;
+  t.nthreads = nthreads;
+}
+
+extern "C" {
+
+void* oracle_create(int dim) {
+  if (dim != 2 && dim != 3) return nullptr;
+  auto* c = new OracleCtx();
+  c->dim = dim;
+  if (dim == 2) c->t2.reset(new mclo::Tree<2>());
+  else c->t3.reset(new mclo::Tree<3>());
+  return c;
+}
+
+void oracle_destroy(void* h) { delete static_cast<OracleCtx*>(h); }
+
+int oracle_set_options(void* h, const oracle_options* o, int nthreads) {
+  auto* c = static_cast<OracleCtx*>(h);
+  if (c->dim == 2) set_opts<2>(c, o, nthreads);
+  else set_opts<3>(c, o, nthreads);
+  return 0;
+}
+
+int oracle_get_etas(void* h, double* out3) {
+  auto* c = static_cast<OracleCtx*>(h);
+  const mclo::Options& o = c->dim == 2 ? c->t2->options : c->t3->options;
+  out3[0] = o.box_eta; out3[1] = o.vf_ccd_eta; out3[2] = o.ee_ccd_eta;
+  return 0;
+}
+
+int oracle_update(void* h, const double* V0, const double* V1, const int* P, int np, int pdim, const int* active) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto t0 = clk::now();
+  if (c->dim == 2) c->t2->update(V0, V1, P, np, pdim, active);
+  else c->t3->update(V0, V1, P, np, pdim, active);
+  c->last_update_ms = ms_since(t0);
+  return 0;
+}
+
+int oracle_traverse(void* h, const double* V0, const double* V1, const int* P, int want_leaf_pairs,
+                    int want_candidates) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto t0 = clk::now();
+  if (c->dim == 2) {
+    c->t2->want_leaf_pairs = want_leaf_pairs; c->t2->want_candidates = want_candidates;
+    c->t2->traverse(V0, V1, P);
+  } else {
+    c->t3->want_leaf_pairs = want_leaf_pairs; c->t3->want_candidates = want_candidates;
+    c->t3->traverse(V0, V1, P);
+  }
+  c->last_traverse_ms = ms_since(t0);
+  return 0;
+}
+
+double oracle_last_ms(void* h, int which) {
+  auto* c = static_cast<OracleCtx*>(h);
+  return which == 0 ? c->last_update_ms : c->last_traverse_ms;
+}
+
+// which: 0 leaf pairs, 1 candidates, 2 hits, 3 discrete, 4 distance evals, 5 loop iterations
+long long oracle_count(void* h, int which) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto get = [&](auto& t) -> long long {
+    switch (which) {
+      case 0: return (long long)t.leaf_pairs.size();
+      case 1: return (long long)t.candidates.size();
+      case 2: return (long long)t.hits.size();
+      case 3: return (long long)t.discrete.size();
+      case 4: return t.stats.dist_evals;
+      case 5: return t.stats.iters;
+    }
+    return -1;
+  };
+  return c->dim == 2 ? get(*c->t2) : get(*c->t3);
+}
+
+// pairs: int[2n]
+int oracle_copy_leaf_pairs(void* h, int* out) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) { for (size_t i = 0; i < t.leaf_pairs.size(); ++i) { out[2 * i] = t.leaf_pairs[i].first; out[2 * i + 1] = t.leaf_pairs[i].second; } };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+int oracle_copy_discrete(void* h, int* out) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) { for (size_t i = 0; i < t.discrete.size(); ++i) { out[2 * i] = t.discrete[i].first; out[2 * i + 1] = t.discrete[i].second; } };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+// sten: int[4n], is_vf: uint8[n]
+int oracle_copy_candidates(void* h, int* sten, unsigned char* is_vf) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) {
+    for (size_t i = 0; i < t.candidates.size(); ++i) {
+      for (int k = 0; k < 4; ++k) sten[4 * i + k] = t.candidates[i].v[k];
+      is_vf[i] = t.candidates[i].is_vf;
+    }
+  };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+int oracle_copy_hits(void* h, int* sten, unsigned char* is_vf, double* toi) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) {
+    for (size_t i = 0; i < t.hits.size(); ++i) {
+      for (int k = 0; k < 4; ++k) sten[4 * i + k] = t.hits[i].s.v[k];
+      is_vf[i] = t.hits[i].s.is_vf;
+      toi[i] = t.hits[i].toi;
+    }
+  };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+// rank[leaf] = in-order position in the KdBVH (left subtree first)
+int oracle_copy_rank(void* h, int* rank) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) { for (size_t i = 0; i < t.rank.size(); ++i) rank[i] = t.rank[i]; };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+// reptri flags: bits 0..2 = v[0..2], bits 3..5 = e[0..2]
+int oracle_copy_flags(void* h, unsigned char* out) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) {
+    for (size_t i = 0; i < t.leaves.size(); ++i) {
+      unsigned f = 0;
+      for (int j = 0; j < 3; ++j) { f |= (t.leaves[i].v[j] ? 1u : 0u) << j; f |= (t.leaves[i].e[j] ? 1u : 0u) << (3 + j); }
+      out[i] = (unsigned char)f;
+    }
+  };
+  if (c->dim == 2) cp(*c->t2); else cp(*c->t3);
+  return 0;
+}
+
+// leaf boxes: per leaf [sw.lo, sw.hi, t0.lo, t0.hi, t1.lo, t1.hi] (6*D doubles) and active
+int oracle_copy_boxes(void* h, double* out, unsigned char* active) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t, auto dimtag) {
+    constexpr int D = decltype(dimtag)::value;
+    for (size_t i = 0; i < t.leaves.size(); ++i) {
+      const auto& b = t.leaves[i].box;
+      double* o = out + i * 6 * D;
+      for (int k = 0; k < D; ++k) {
+        o[k] = b.sw.lo[k]; o[D + k] = b.sw.hi[k];
+        o[2 * D + k] = b.t0.lo[k]; o[3 * D + k] = b.t0.hi[k];
+        o[4 * D + k] = b.t1.lo[k]; o[5 * D + k] = b.t1.hi[k];
+      }
+      active[i] = b.active;
+    }
+  };
+  if (c->dim == 2) cp(*c->t2, std::integral_constant<int, 2>());
+  else cp(*c->t3, std::integral_constant<int, 3>());
+  return 0;
+}
+
+// Brute-force leaf pairs (i<j); returns count, fills out if cap suffices.
+long long oracle_brute_force_pairs(void* h, int* out, long long cap) {
+  auto* c = static_cast<OracleCtx*>(h);
+  std::vector<std::pair<int, int>> v = c->dim == 2 ? mclo::brute_force_pairs<2>(*c->t2) : mclo::brute_force_pairs<3>(*c->t3);
+  if ((long long)v.size() <= cap)
+    for (size_t i = 0; i < v.size(); ++i) { out[2 * i] = v[i].first; out[2 * i + 1] = v[i].second; }
+  return (long long)v.size();
+}
+
+// ---- per-primitive kernels -------------------------------------------------
+// verts0/verts1: (dim+1)*dim doubles each. Returns hit (0/1); toi written on hit.
+int oracle_accd(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* toi,
+                long long* dist_evals) {
+  mclo::AccdStats st;
+  double t = -1;
+  bool hit;
+  if (dim == 3) hit = mclo::additive_ccd<3>(reinterpret_cast<const double(*)[3]>(verts0), reinterpret_cast<const double(*)[3]>(verts1), eta, is_vf != 0, t, &st);
+  else hit = mclo::additive_ccd<2>(reinterpret_cast<const double(*)[2]>(verts0), reinterpret_cast<const double(*)[2]>(verts1), eta, true, t, &st);
+  if (toi) *toi = hit ? t : -1.0;
+  if (dist_evals) *dist_evals = st.dist_evals;
+  return hit ? 1 : 0;
+}
+
+double oracle_pair_distance(int dim, const double* verts, int is_vf) {
+  if (dim == 3) return mclo::pair_distance<3>(reinterpret_cast<const double(*)[3]>(verts), is_vf != 0);
+  return mclo::pair_distance<2>(reinterpret_cast<const double(*)[2]>(verts), true);
+}
+
+void oracle_point_on_triangle(const double* q, const double* a, const double* b, const double* c, double* out) {
+  mclo::point_on_triangle(q, a, b, c, out);
+}
+
+void oracle_edge_to_edge(const double* p0, const double* p1, const double* q0, const double* q1, double* out, double* bary) {
+  mclo::edge_to_edge(p0, p1, q0, q1, out, bary);
+}
+
+void oracle_point_on_edge2(const double* p, const double* e0, const double* e1, double* out) {
+  mclo::point_on_edge2(p, e0, e1, out);
+}
+
+int oracle_tri_tri(const double* f0, const double* f1) {
+  return mclo::tt::tri_tri_overlap_3d(f0, f0 + 3, f0 + 6, f1, f1 + 3, f1 + 6);
+}
+
+int oracle_edge_edge2(const double* e0, const double* e1) {
+  return mclo::discrete_edge_edge2(e0, e0 + 2, e1, e1 + 2) ? 1 : 0;
+}
+
+int oracle_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,70 @@
+"""Comparison helpers for GPU-vs-oracle parity (test infrastructure).
+
+Orientation: the reference orients each leaf pair by its KdBVH in-order
+position (include/MCL/BVHTree.cpp:288-301), which fixes the order of the two
+edges of an EE stencil and the argument order of a discrete hit.  Canonical
+forms compare results independently of orientation; in strict mode (the GPU
+given the oracle's rank) results are compared as oriented records.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def canon_pairs(pairs) -> set:
+    p = np.sort(np.asarray(pairs, dtype=np.int64).reshape(-1, 2), axis=1)
+    return set(map(tuple, p.tolist()))
+
+
+def oriented_pairs(pairs) -> list:
+    return sorted(map(tuple, np.asarray(pairs).reshape(-1, 2).tolist()))
+
+
+def _canon_key(s, vf):
+    s = tuple(int(x) for x in s)
+    if vf:
+        return (1,) + s
+    e0, e1 = s[:2], s[2:]
+    return (0,) + (e0 + e1 if e0 <= e1 else e1 + e0)
+
+
+def canon_stencils(sten, is_vf) -> list:
+    return sorted(_canon_key(s, v) for s, v in zip(np.asarray(sten).tolist(), np.asarray(is_vf).tolist()))
+
+
+def oriented_stencils(sten, is_vf) -> list:
+    return sorted((int(v),) + tuple(s) for s, v in zip(np.asarray(sten).tolist(), np.asarray(is_vf).tolist()))
+
+
+def hit_map(sten, is_vf, toi, canonical=True) -> dict:
+    """key -> sorted list of toi (lists, because without reptri a stencil may repeat)."""
+    out: dict = {}
+    for s, v, t in zip(np.asarray(sten).tolist(), np.asarray(is_vf).tolist(), np.asarray(toi).tolist()):
+        k = _canon_key(s, v) if canonical else (int(v),) + tuple(s)
+        out.setdefault(k, []).append(t)
+    for k in out:
+        out[k].sort()
+    return out
+
+
+def compare_hits(gpu, ref, canonical: bool, rtol: float = 0.0):
+    """Returns (n_mismatch_keys, max_rel_diff, n_over) between two hit maps.
+    n_over counts GPU TOIs above the reference's by more than rtol (relative)."""
+    keys = set(gpu) | set(ref)
+    bad = 0
+    maxrel = 0.0
+    over = 0
+    for k in keys:
+        a, b = gpu.get(k), ref.get(k)
+        if a is None or b is None or len(a) != len(b):
+            bad += 1
+            continue
+        for x, y in zip(a, b):
+            d = abs(x - y)
+            rel = d / max(abs(y), 1e-300) if d > 0 else 0.0
+            maxrel = max(maxrel, rel)
+            if x > y * (1 + rtol) + (0 if y != 0 else rtol):
+                over += 1
+            if rel > rtol:
+                bad += 1
+    return bad, maxrel, over
