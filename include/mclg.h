@@ -119,6 +119,13 @@ int mclg_copy_discrete(mclg_ctx* ctx, int* pairs2, int64_t cap);
 int mclg_copy_candidates(mclg_ctx* ctx, int* sten4, unsigned char* is_vf, double* toi, int64_t cap);
 int mclg_copy_leaf_pairs(mclg_ctx* ctx, int* pairs2, int64_t cap);
 
+/* The leaves of the last update (the reference's public `leaves` member,
+ * BVHTree.hpp:39, BVHLeaf.hpp:12-52): per primitive the swept, t0 and t1 boxes
+ * as [sw_min, sw_max, t0_min, t0_max, t1_min, t1_max] (6*dim doubles, padded
+ * by box_eta), the active flag and the representative-triangle flags
+ * (bits 0..2 = v[0..2], bits 3..5 = e[0..2]).  Any output may be NULL. */
+int mclg_copy_leaves(mclg_ctx* ctx, double* boxes, unsigned char* active, unsigned char* flags, int64_t cap);
+
 /* Device pointers to the last traverse's results (valid until the next traverse). */
 typedef struct mclg_device_results {
   const int* hit_sten4;

@@ -73,6 +73,7 @@ def _load():
         "mclg_copy_candidates": (I, [P, P, P, P, I64]),
         "mclg_copy_leaf_pairs": (I, [P, P, I64]),
         "mclg_get_device_results": (I, [P, P]),
+        "mclg_copy_leaves": (I, [P, P, P, P, I64]),
         "mclg_set_rank": (I, [P, P, I]),
         "mclg_get_timings": (I, [P, P, I]),
     }

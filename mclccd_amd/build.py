@@ -64,7 +64,28 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_facade_check(verbose)
     return OUT
+
+
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_check")
+
+
+def build_facade_check(verbose: bool = False) -> str:
+    """Compiles the C++ program that drives the drop-in facade (host code only:
+    g++ against include/ and libmclg.so)."""
+    if not os.path.exists(FACADE_SRC):
+        return ""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{os.path.join(ROOT, 'include')}", FACADE_SRC, "-o", FACADE_BIN,
+           f"-L{HERE}", "-lmclg", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../mclccd_amd",
+           "-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade check build failed:\n{r.stderr}")
+    if verbose:
+        print("built", FACADE_BIN)
+    return FACADE_BIN
 
 
 if __name__ == "__main__":

@@ -475,6 +475,17 @@ int mclg_copy_leaf_pairs(mclg_ctx* h, int* pairs2, int64_t cap) {
   return MCLG_OK;
 }
 
+int mclg_copy_leaves(mclg_ctx* h, double* boxes, unsigned char* active, unsigned char* flags, int64_t cap) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  const int64_t n = c->have_tree ? c->np : 0;
+  if (cap < n) return c->fail(MCLG_ERR_CAPACITY, "copy_leaves: cap < np");
+  if (boxes) CHECK(copy_out(c, boxes, c->d_boxes, sizeof(double) * 6 * c->dim * (size_t)n));
+  if (active) CHECK(copy_out(c, active, c->d_act, (size_t)n));
+  if (flags) CHECK(copy_out(c, flags, c->d_flags, (size_t)n));
+  return MCLG_OK;
+}
+
 int mclg_get_device_results(const mclg_ctx* h, mclg_device_results* out) {
   const Ctx* c = reinterpret_cast<const Ctx*>(h);
   if (!c || !out) return MCLG_ERR_ARG;

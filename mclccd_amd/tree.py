@@ -229,6 +229,15 @@ class BVHTree:
         self._check(lib.mclg_copy_leaf_pairs(self._h, _ptr(out), n))
         return out
 
+    def leaves(self):
+        """(boxes [np, 6*dim], active [np], reptri flags [np]) of the last update."""
+        n = self.np
+        b = np.empty((n, 6 * self.dim), np.float64)
+        a = np.empty(n, np.uint8)
+        f = np.empty(n, np.uint8)
+        self._check(lib.mclg_copy_leaves(self._h, _ptr(b), _ptr(a), _ptr(f), n))
+        return b, a, f
+
     def set_rank(self, rank):
         """Orient leaf pairs by rank (the reference's KdBVH in-order position)."""
         if rank is None:

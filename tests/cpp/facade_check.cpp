@@ -1,0 +1,96 @@
+// Exercises the drop-in facade include/MCL/BVHTree.hpp the way the reference's
+// README uses mcl::BVHTree (README.md:19-50): update, callbacks, traverse.
+// Reads a mesh written by tests/test_gpu_facade.py, writes what the callbacks
+// received.  Usage: facade_check <in.bin> <out.txt>
+//   in.bin: int32 nv, int32 nf, double eta, V0[nv*3], V1[nv*3], int32 F[nf*3]
+#include <MCL/BVHTree.hpp>
+
+#include <cstdio>
+#include <vector>
+
+int main(int argc, char** argv) {
+  if (argc < 3) return 2;
+  FILE* f = std::fopen(argv[1], "rb");
+  if (!f) return 2;
+  int nv = 0, nf = 0;
+  double eta = 0;
+  if (std::fread(&nv, 4, 1, f) != 1 || std::fread(&nf, 4, 1, f) != 1 || std::fread(&eta, 8, 1, f) != 1) return 2;
+  std::vector<double> V0((size_t)nv * 3), V1((size_t)nv * 3);
+  std::vector<int> F((size_t)nf * 3);
+  if (std::fread(V0.data(), 8, V0.size(), f) != V0.size() || std::fread(V1.data(), 8, V1.size(), f) != V1.size() ||
+      std::fread(F.data(), 4, F.size(), f) != F.size())
+    return 2;
+  std::fclose(f);
+  FILE* out = std::fopen(argv[2], "w");
+  if (!out) return 2;
+
+  mcl::BVHTree<double, 3> tree;
+  tree.options.box_eta = eta;
+  tree.options.vf_ccd_eta = eta;
+  tree.options.ee_ccd_eta = eta;
+  tree.update(V0.data(), V1.data(), F.data(), nf, 3);
+
+  // 1. plain: append_pair / append_discrete collect everything
+  long n_pair = 0, n_vf = 0, n_disc = 0;
+  tree.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const double& toi) {
+    ++n_pair;
+    n_vf += is_vf;
+    std::fprintf(out, "hit %d %d %d %d %d %.17g\n", s[0], s[1], s[2], s[3], (int)is_vf, toi);
+  };
+  tree.append_discrete = [&](int p0, int p1) {
+    ++n_disc;
+    std::fprintf(out, "disc %d %d\n", p0, p1);
+    return false;
+  };
+  tree.traverse(V0.data(), V1.data(), F.data());
+  const mclg_counts c = tree.counts();
+  std::fprintf(out, "plain %ld %ld %ld %lld %lld %lld\n", n_pair, n_vf, n_disc, (long long)c.hits,
+               (long long)c.discrete, (long long)c.candidates);
+
+  // 2. filter_pair drops every EE candidate: only VF hits reach append_pair
+  long n_filter_calls = 0, n_after = 0, n_after_vf = 0;
+  tree.append_pair = [&](const mcl::Vector4i&, bool is_vf, const double&) {
+    ++n_after;
+    n_after_vf += is_vf;
+  };
+  tree.append_discrete = nullptr;
+  tree.filter_pair = [&](const mcl::Vector4i&, bool is_vf) {
+    ++n_filter_calls;
+    return !is_vf;
+  };
+  tree.traverse(V0.data(), V1.data(), F.data());
+  std::fprintf(out, "filter %ld %ld %ld\n", n_filter_calls, n_after, n_after_vf);
+
+  // 3. custom narrow_phase: called for every unfiltered candidate, its toi is used
+  long n_np = 0, n_np_pairs = 0, bad_toi = 0;
+  tree.filter_pair = nullptr;
+  tree.narrow_phase = [&](const mcl::Vector4i&, bool) {
+    ++n_np;
+    return 0.25;
+  };
+  tree.append_pair = [&](const mcl::Vector4i&, bool, const double& toi) {
+    ++n_np_pairs;
+    bad_toi += toi != 0.25;
+  };
+  tree.traverse(V0.data(), V1.data(), F.data());
+  std::fprintf(out, "narrow %ld %ld %ld\n", n_np, n_np_pairs, bad_toi);
+
+  // 4. stop: append_discrete returning true ends the discrete replay
+  long n_stop = 0;
+  tree.narrow_phase = nullptr;
+  tree.append_pair = nullptr;
+  tree.append_discrete = [&](int, int) {
+    ++n_stop;
+    return true;
+  };
+  tree.traverse(V0.data(), V1.data(), F.data());
+  std::fprintf(out, "stop %ld\n", n_stop);
+
+  // 5. leaves mirror
+  tree.sync_leaves();
+  long owned_v = 0;
+  for (const auto& L : tree.leaves) owned_v += L.v[0] + L.v[1] + L.v[2];
+  std::fprintf(out, "leaves %zu %ld\n", tree.leaves.size(), owned_v);
+  std::fclose(out);
+  return 0;
+}

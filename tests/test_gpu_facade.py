@@ -1,0 +1,65 @@
+"""The C++ drop-in facade (include/MCL/BVHTree.hpp) driven by a compiled C++
+program (tests/cpp/facade_check.cpp): callback replay, filter_pair,
+narrow_phase and the discrete stop, checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mclccd_amd import scenes
+from oracle.oracle import run_scene
+from tests.parity import canon_pairs, hit_map
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "facade_check")
+
+
+def write_mesh(path, V0, V1, F, eta):
+    with open(path, "wb") as f:
+        np.array([len(V0), len(F)], np.int32).tofile(f)
+        np.array([eta], np.float64).tofile(f)
+        np.ascontiguousarray(V0, np.float64).tofile(f)
+        np.ascontiguousarray(V1, np.float64).tofile(f)
+        np.ascontiguousarray(F, np.int32).tofile(f)
+
+
+def test_facade_callbacks(gpu, tmp_path):
+    assert os.path.exists(BIN), "facade_check is built by __graft_entry__.build()"
+    V0, V1, F, _ = scenes.random_soup(400, seed=2, share=True, motion=0.2)
+    eta = 1e-2
+    write_mesh(tmp_path / "in.bin", V0, V1, F, eta)
+    r = subprocess.run([BIN, str(tmp_path / "in.bin"), str(tmp_path / "out.txt")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = open(tmp_path / "out.txt").read().splitlines()
+    rec = {l.split()[0]: list(map(float, l.split()[1:])) for l in lines if not l.startswith(("hit ", "disc "))}
+    hits = [l.split()[1:] for l in lines if l.startswith("hit ")]
+    disc = [tuple(map(int, l.split()[1:])) for l in lines if l.startswith("disc ")]
+
+    o = run_scene(V0, V1, F, eta)
+    rs, rv, rt = o.hits()
+    # 1. append_pair received exactly the oracle's hits (TOI within the EE orientation tolerance)
+    gs = np.array([[int(x) for x in h[:4]] for h in hits], np.int32).reshape(-1, 4)
+    gv = np.array([int(h[4]) for h in hits], np.uint8)
+    gt = np.array([float(h[5]) for h in hits])
+    gm, rm = hit_map(gs, gv, gt), hit_map(rs, rv, rt)
+    assert gm.keys() == rm.keys()
+    for k in gm:
+        assert np.allclose(gm[k], rm[k], rtol=1e-9, atol=0)
+    n_pair, n_vf, n_disc, c_hits, c_disc, c_cand = rec["plain"]
+    assert n_pair == len(rs) == c_hits and n_vf == int(rv.sum())
+    assert canon_pairs(disc) == canon_pairs(o.discrete()) and n_disc == c_disc == o.count(3)
+    # 2. filter_pair sees every candidate; EE filtered out -> only VF hits
+    n_filter_calls, n_after, n_after_vf = rec["filter"]
+    assert n_filter_calls == o.count(1) and n_after == n_after_vf == int(rv.sum())
+    # 3. narrow_phase replaces ACCD for every candidate
+    n_np, n_np_pairs, bad = rec["narrow"]
+    assert n_np == o.count(1) and n_np_pairs == o.count(1) and bad == 0
+    # 4. stop after the first discrete hit
+    assert rec["stop"][0] == (1 if o.count(3) > 0 else 0)
+    # 5. leaves mirror: every vertex owned once
+    nleaves, owned = rec["leaves"]
+    assert nleaves == len(F) and owned == len(np.unique(F))

@@ -45,6 +45,12 @@ def run_both(V0, V1, P, theta, strict, active=None, **opts):
 
 
 def check_parity(t, o, strict):
+    # leaves: FP64 boxes, active flags and representative-triangle flags, bit-identical
+    gb, ga, gf = t.leaves()
+    ob, oa = o.boxes()
+    assert np.array_equal(gb, ob)
+    assert np.array_equal(ga, oa)
+    assert np.array_equal(gf, o.flags())
     # leaf pairs
     gl, ol = t.leaf_pairs(), o.leaf_pairs()
     assert len(gl) == len(ol)
