@@ -166,7 +166,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ rec, const int* __restrict__ P, int np,
                                                     int pdim, const int* __restrict__ active, double eta,
                                                     LeafBox<D>* __restrict__ boxes, uint8_t* __restrict__ act,
-                                                    unsigned long long* cbounds) {
+                                                    double* __restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double clo[D], chi[D];
 #pragma unroll
@@ -215,24 +215,80 @@ __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ r
       chi[k] = h2 > chi[k] ? h2 : chi[k];
     }
   }
-  if (lane_id() == 0 && clo[0] <= chi[0]) {
+  __shared__ double s_lo[4][D], s_hi[4][D];
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) { s_lo[w][k] = clo[k]; s_hi[w][k] = chi[k]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      atomicMin(cbounds + k, (unsigned long long)dbl_to_ord(clo[k]));
-      atomicMax(cbounds + D + k, (unsigned long long)dbl_to_ord(chi[k]));
+      double lo = s_lo[0][k], hi = s_hi[0][k];
+      for (int j = 1; j < 4; ++j) {
+        lo = s_lo[j][k] < lo ? s_lo[j][k] : lo;
+        hi = s_hi[j][k] > hi ? s_hi[j][k] : hi;
+      }
+      partial[(size_t)blockIdx.x * 2 * D + k] = lo;
+      partial[(size_t)blockIdx.x * 2 * D + D + k] = hi;
     }
   }
 }
 
+// Reduces the per-block bounds to cbounds (ordered encoding, lo then hi).
+template <int D>
+__global__ __launch_bounds__(256) void k_reduce_bounds(const double* __restrict__ partial, int nb,
+                                                       unsigned long long* cbounds) {
+  double lo[D], hi[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) { lo[k] = __builtin_huge_val(); hi[k] = -__builtin_huge_val(); }
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      const double l = partial[(size_t)b * 2 * D + k], h = partial[(size_t)b * 2 * D + D + k];
+      lo[k] = l < lo[k] ? l : lo[k];
+      hi[k] = h > hi[k] ? h : hi[k];
+    }
+  __shared__ double s_lo[4][D], s_hi[4][D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    for (int o = 32; o > 0; o >>= 1) {
+      const double l2 = __shfl_xor(lo[k], o), h2 = __shfl_xor(hi[k], o);
+      lo[k] = l2 < lo[k] ? l2 : lo[k];
+      hi[k] = h2 > hi[k] ? h2 : hi[k];
+    }
+  }
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0)
+#pragma unroll
+    for (int k = 0; k < D; ++k) { s_lo[w][k] = lo[k]; s_hi[w][k] = hi[k]; }
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      double l = s_lo[0][k], h = s_hi[0][k];
+      for (int j = 1; j < 4; ++j) {
+        l = s_lo[j][k] < l ? s_lo[j][k] : l;
+        h = s_hi[j][k] > h ? s_hi[j][k] : h;
+      }
+      cbounds[k] = dbl_to_ord(l);
+      cbounds[D + k] = dbl_to_ord(h);
+    }
+}
+
+int leaf_box_blocks(int np) { return (np + 255) / 256; }
+
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
-                       void* boxes, uint8_t* act, unsigned long long* cbounds, hipStream_t s) {
-  // bounds start empty: lo = ord(+inf), hi = ord(-inf)
-  const unsigned long long lo0 = dbl_to_ord(__builtin_huge_val()), hi0 = dbl_to_ord(-__builtin_huge_val());
-  fill<unsigned long long>(cbounds, dim, lo0, s);
-  fill<unsigned long long>(cbounds + dim, dim, hi0, s);
-  const int g = (np + 255) / 256;
-  if (dim == 3) k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, cbounds);
-  else k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, cbounds);
+                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, hipStream_t s) {
+  const int g = leaf_box_blocks(np);
+  if (dim == 3) {
+    k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, partial);
+    k_reduce_bounds<3><<<1, 256, 0, s>>>(partial, g, cbounds);
+  } else {
+    k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, partial);
+    k_reduce_bounds<2><<<1, 256, 0, s>>>(partial, g, cbounds);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -460,20 +516,10 @@ __global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__
     r = __hip_atomic_load(rng_r + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int lc = __hip_atomic_load(lchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int rc = __hip_atomic_load(rchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    NodeF<D> a, b, o;
-    // the children were written by other threads before their release: read
-    // them through the coherent path
-    {
-      const uint32_t* pa = (const uint32_t*)(nodes + lc);
-      const uint32_t* pb = (const uint32_t*)(nodes + rc);
-      uint32_t* qa = (uint32_t*)&a;
-      uint32_t* qb = (uint32_t*)&b;
-#pragma unroll
-      for (int k = 0; k < (int)(sizeof(NodeF<D>) / 4); ++k) {
-        qa[k] = __hip_atomic_load(pa + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        qb[k] = __hip_atomic_load(pb + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-    }
+    // the children were written by other threads before their release
+    // fence + atomic; the acquire side above makes plain loads see them
+    const NodeF<D> a = nodes[lc], b = nodes[rc];
+    NodeF<D> o;
     merge_node<D>(o, a, b);
     o.child = lc;
     nodes[parent] = o;

@@ -221,7 +221,8 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_boxes, &cap_boxes, ldbox_bytes(dim) * (size_t)np, "leaf boxes"));
   CHECK(grow((void**)&d_act, &cap_act, (size_t)np, "active"));
   CHECK(grow((void**)&d_cbounds, &cap_cbounds, sizeof(unsigned long long) * 6, "bounds"));
-  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_cbounds, stream);
+  CHECK(grow((void**)&d_partial, &cap_partial, sizeof(double) * 6 * (size_t)leaf_box_blocks(np), "bounds"));
+  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, stream);
   mark(3);
   // Morton + sort
   CHECK(grow((void**)&d_keys, &cap_keys, sizeof(uint32_t) * (size_t)np, "keys"));
@@ -353,7 +354,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
-  void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_keys, d_order, d_ktmp,
+  void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
                   d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_cand_sten, d_cand_vf, d_cand_toi, d_rank, d_minmax,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};

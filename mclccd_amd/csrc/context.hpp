@@ -38,6 +38,7 @@ struct Ctx {
   void* d_boxes = nullptr; size_t cap_boxes = 0;
   uint8_t* d_act = nullptr; size_t cap_act = 0;
   unsigned long long* d_cbounds = nullptr; size_t cap_cbounds = 0;
+  double* d_partial = nullptr; size_t cap_partial = 0;
   uint32_t* d_keys = nullptr; size_t cap_keys = 0;
   uint32_t* d_order = nullptr; size_t cap_order = 0;
   uint32_t* d_ktmp = nullptr; size_t cap_ktmp = 0;

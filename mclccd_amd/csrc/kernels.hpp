@@ -12,8 +12,9 @@ void launch_fill_i32(int* p, long long n, int v, hipStream_t s);
 void launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t s);
 void launch_reptri(const int* P, int np, int pdim, int nv, int* owner, unsigned long long* keys, int* vals,
                    uint32_t table_size, uint8_t* flags, hipStream_t s);
+int leaf_box_blocks(int np);
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
-                       void* boxes, uint8_t* act, unsigned long long* cbounds, hipStream_t s);
+                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, hipStream_t s);
 void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, uint32_t* keys,
                    uint32_t* vals, hipStream_t s);
 int radix_tiles(int n);

@@ -18,6 +18,12 @@
 
 namespace mclg {
 
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------
 // Leaf-pair tests
 // ---------------------------------------------------------------------------
@@ -59,13 +65,37 @@ __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const Le
   return true;
 }
 
+// Pairs found by a wave are staged in LDS and written out in batches: one
+// global atomic per flush instead of one per emitting step.
+constexpr int TR_WAVES = 4, TR_BUF = 512;
+
+__device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,
+                                            unsigned long long* counter) {
+  // executed by the wave's active lanes (inside or after the divergent walk)
+  const uint64_t act = __ballot(1);
+  const int nact = __popcll(act), r = popc_below(act);
+  const int leader = __ffsll((long long)act) - 1;
+  unsigned long long base = 0;
+  if (r == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+  base = __shfl(base, leader);
+  for (int i = r; i < cnt; i += nact)
+    if (base + i < cap) pairs[base + i] = buf[i];
+}
+
 template <int D>
-__global__ __launch_bounds__(256) void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict__ boxes,
-                                                  const uint8_t* __restrict__ act, const uint32_t* __restrict__ order,
-                                                  int n, const int* __restrict__ rank, int2* __restrict__ pairs,
-                                                  unsigned long long cap, unsigned long long* counters) {
+__global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* __restrict__ nodes,
+                                                               const LeafBox<D>* __restrict__ boxes,
+                                                               const uint8_t* __restrict__ act,
+                                                               const uint32_t* __restrict__ order, int n,
+                                                               const int* __restrict__ rank, int2* __restrict__ pairs,
+                                                               unsigned long long cap, unsigned long long* counters) {
+  __shared__ int2 s_buf[TR_WAVES][TR_BUF];
+  __shared__ int s_cnt[TR_WAVES];
+  const int w = threadIdx.x / kWave;
   const int s = xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = s < n - 1;
+  if (lane_id() == 0) s_cnt[w] = 0;
+  wave_lds_sync();
   NodeF<D> q;
   LeafBox<D> qb;
   bool qa = false;
@@ -100,24 +130,40 @@ __global__ __launch_bounds__(256) void k_traverse(const NodeF<D>* __restrict__ n
         next = nd.child;
       }
     }
-    const long long slot = wave_append(emit, counters + C_LEAF_PAIRS);
-    if (emit && slot < (long long)cap) {
-      const int rb = rank ? rank[b] : b;
-      pairs[slot] = ra < rb ? make_int2(a, b) : make_int2(b, a);
+    const uint64_t m = __ballot(emit);
+    if (m) {
+      const int cnt = s_cnt[w];
+      if (emit) {
+        const int rb = rank ? rank[b] : b;
+        s_buf[w][cnt + popc_below(m)] = ra < rb ? make_int2(a, b) : make_int2(b, a);
+      }
+      const int nc = cnt + __popcll(m);
+      wave_lds_sync();
+      if (nc > TR_BUF - kWave) {
+        flush_pairs(s_buf[w], nc, pairs, cap, counters + C_LEAF_PAIRS);
+        wave_lds_sync();
+        if (popc_below(__ballot(1)) == 0) s_cnt[w] = 0;
+      } else if (popc_below(__ballot(1)) == 0) {
+        s_cnt[w] = nc;
+      }
+      wave_lds_sync();
     }
     node = next;
   }
+  wave_lds_sync();
+  const int cnt = s_cnt[w];
+  if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
 }
 
 void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
                      const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters, hipStream_t s) {
   if (n < 2) return;
-  const int g = xcd_grid(n - 1, 256);
+  const int g = xcd_grid(n - 1, TR_WAVES * kWave);
   if (dim == 3)
-    k_traverse<3><<<g, 256, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)boxes, act, order, n, rank, pairs, cap,
+    k_traverse<3><<<g, TR_WAVES * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)boxes, act, order, n, rank, pairs, cap,
                                     counters);
   else
-    k_traverse<2><<<g, 256, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)boxes, act, order, n, rank, pairs, cap,
+    k_traverse<2><<<g, TR_WAVES * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)boxes, act, order, n, rank, pairs, cap,
                                     counters);
 }
 
@@ -152,12 +198,6 @@ template <> struct VLoad<2> {
     return {r1.x, r1.y};
   }
 };
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // exclusive prefix over the wave of a small count (< 2^bits) and the total
 template <int BITS> __device__ __forceinline__ int wave_prefix(int c, int& total) {
