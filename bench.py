@@ -147,20 +147,18 @@ def main():
 
     # dominant kernel roofline
     flops = FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]
-    narrow_ms = avg["narrow_phase"]
+    accd_ms = avg["accd"]
     trav_ms = avg["traverse"]
-    if narrow_ms >= trav_ms:
-        achieved = flops / (narrow_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_narrow<3> (candidates + FP64 ACCD + tri-tri)", "bound": "valu",
+    if accd_ms >= trav_ms:
+        achieved = flops / (accd_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_accd<3,VF> + k_accd<3,EE> (FP64 ACCD over the stencil queues)", "bound": "valu",
                 "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
                 "note": "FP64 vector ALU bound (no MFMA on this path); algorithmic flops = "
                         f"{FLOPS_ACCD_SETUP}*candidates + {FLOPS_PER_EVAL}*distance evaluations, div/sqrt = 1 flop; "
                         "non-FMA FP64 peak is 39.3 T ops/s"}
     else:
-        # traversal: node records (80 B) + leaf boxes (144 B) read per visit is not counted by
-        # the kernel; the compulsory bytes are the leaf-pair output (8 B/pair) plus one pass over
-        # the nodes and leaf boxes
+        # traversal: compulsory bytes = one pass over nodes and leaf boxes + 8 B per emitted pair
         n = len(F)
         byts = 8 * c["leaf_pairs"] + 80 * (2 * n - 1) + 145 * n
         achieved = byts / (trav_ms * 1e-3) / 1e9

@@ -126,15 +126,20 @@ int mclg_copy_leaf_pairs(mclg_ctx* ctx, int* pairs2, int64_t cap);
  * (bits 0..2 = v[0..2], bits 3..5 = e[0..2]).  Any output may be NULL. */
 int mclg_copy_leaves(mclg_ctx* ctx, double* boxes, unsigned char* active, unsigned char* flags, int64_t cap);
 
-/* Device pointers to the last traverse's results (valid until the next traverse). */
+/* Device pointers to the last traverse's results (valid until the next traverse).
+ * Candidates live in two queues, VF then EE; their per-candidate toi arrays are
+ * set only when the traverse kept candidates. */
 typedef struct mclg_device_results {
   const int* hit_sten4;
   const unsigned char* hit_is_vf;
   const double* hit_toi;
   const int* discrete2;
-  const int* cand_sten4;
-  const unsigned char* cand_is_vf;
-  const double* cand_toi;
+  const int* vf_sten4;
+  const double* vf_toi;
+  int64_t n_vf;
+  const int* ee_sten4;
+  const double* ee_toi;
+  int64_t n_ee;
   const int* leaf_pairs2;
 } mclg_device_results;
 int mclg_get_device_results(const mclg_ctx* ctx, mclg_device_results* out);
@@ -150,10 +155,10 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
 /* Device milliseconds of the phases of the last update / traverse, measured
  * with HIP events on the context's stream:
  *   0 validate+pack  1 reptri  2 leaf boxes  3 morton  4 sort  5 lbvh (update)
- *   6 validate+pack  7 traversal kernel  8 narrow-phase kernel  (traverse;
- *   7 and 8 bracket the final launch of the kernel only)
+ *   6 validate+pack  7 traversal kernel  8 candidate + discrete kernel
+ *   9 ACCD kernels (traverse; 7-9 bracket the final launches only)
  * Writes min(n, MCLG_NUM_TIMINGS) values. */
-#define MCLG_NUM_TIMINGS 9
+#define MCLG_NUM_TIMINGS 10
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
 
 #ifdef __cplusplus

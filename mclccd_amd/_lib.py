@@ -22,9 +22,9 @@ ERR_CAPACITY = -6
 
 KEEP_CANDIDATES = 1
 KEEP_LEAF_PAIRS = 2
-NUM_TIMINGS = 9
+NUM_TIMINGS = 10
 TIMING_NAMES = ("update_pack", "update_reptri", "update_leaf_boxes", "update_morton", "update_sort",
-                "update_lbvh", "traverse_pack", "traverse", "narrow_phase")
+                "update_lbvh", "traverse_pack", "traverse", "expand", "accd")
 
 
 class Options(C.Structure):
@@ -40,8 +40,8 @@ class Counts(C.Structure):
 
 class DeviceResults(C.Structure):
     _fields_ = [("hit_sten4", C.c_void_p), ("hit_is_vf", C.c_void_p), ("hit_toi", C.c_void_p),
-                ("discrete2", C.c_void_p), ("cand_sten4", C.c_void_p), ("cand_is_vf", C.c_void_p), ("cand_toi", C.c_void_p),
-                ("leaf_pairs2", C.c_void_p)]
+                ("discrete2", C.c_void_p), ("vf_sten4", C.c_void_p), ("vf_toi", C.c_void_p), ("n_vf", C.c_int64),
+                ("ee_sten4", C.c_void_p), ("ee_toi", C.c_void_p), ("n_ee", C.c_int64), ("leaf_pairs2", C.c_void_p)]
 
 
 class MclgError(RuntimeError):

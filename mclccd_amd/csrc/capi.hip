@@ -294,49 +294,69 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   }
   const long long npairs = (long long)h_counters[C_LEAF_PAIRS];
   counts.leaf_pairs = npairs;
+  n_vf = n_ee = 0;
   if (npairs > 0 && (opt.continuous || opt.discrete)) {
-    if (hit_cap == 0) hit_cap = std::max<unsigned long long>(1024, 4ull * (unsigned long long)np);
+    // candidates + discrete (BVHTree.cpp:252-283)
+    if (vf_cap == 0) vf_cap = std::max<unsigned long long>(1024, (unsigned long long)(1.5 * npairs));
+    if (ee_cap == 0 && dim == 3) ee_cap = std::max<unsigned long long>(1024, (unsigned long long)(2.5 * npairs));
     if (disc_cap == 0) disc_cap = 1024;
-    if (keep_cand && cand_cap == 0) cand_cap = std::max<unsigned long long>(1024, 4ull * (unsigned long long)npairs);
     for (int attempt = 0;; ++attempt) {
-      CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
-      CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
-      CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
+      CHECK(grow((void**)&d_vfq, &cap_vfq, sizeof(int4) * vf_cap, "VF stencils"));
+      if (dim == 3) CHECK(grow((void**)&d_eeq, &cap_eeq, sizeof(int4) * ee_cap, "EE stencils"));
       CHECK(grow((void**)&d_disc, &cap_disc, sizeof(int2) * disc_cap, "discrete"));
-      if (keep_cand) {
-        CHECK(grow((void**)&d_cand_sten, &cap_cand_sten, sizeof(int4) * cand_cap, "candidates"));
-        CHECK(grow((void**)&d_cand_vf, &cap_cand_vf, cand_cap, "candidates"));
-        CHECK(grow((void**)&d_cand_toi, &cap_cand_toi, sizeof(double) * cand_cap, "candidates"));
-      }
-      launch_fill_u64(d_counters + C_CANDIDATES, C_COUNT - C_CANDIDATES, 0ull, stream);
-      NarrowLaunch L;
-      L.pairs = d_pairs; L.npairs = npairs; L.P = P; L.flags = d_flags; L.rec = d_rec;
-      L.vf_eta = opt.vf_ccd_eta; L.ee_eta = opt.ee_ccd_eta;
-      L.discrete = opt.discrete; L.continuous = opt.continuous; L.want_candidates = keep_cand;
-      L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
-      L.disc = d_disc; L.disc_cap = disc_cap;
-      L.cand_sten = d_cand_sten; L.cand_vf = d_cand_vf; L.cand_toi = d_cand_toi; L.cand_cap = keep_cand ? cand_cap : 0;
-      L.counters = d_counters;
-      L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu;
+      launch_fill_u64(d_counters + C_CAND_VF, C_COUNT - C_CAND_VF, 0ull, stream);
+      ExpandLaunch E;
+      E.pairs = d_pairs; E.npairs = npairs; E.P = P; E.flags = d_flags; E.rec = d_rec;
+      E.discrete = opt.discrete; E.continuous = opt.continuous;
+      E.vfq = d_vfq; E.vf_cap = vf_cap; E.eeq = d_eeq; E.ee_cap = dim == 3 ? ee_cap : 0;
+      E.disc = d_disc; E.disc_cap = disc_cap; E.counters = d_counters; E.num_cus = num_cus;
       mark(10);
-      launch_narrow(dim, L, stream);
+      launch_expand(dim, E, stream);
       mark(11);
       HIPCHECK(this, hipGetLastError());
       CHECK(read_counters());
       bool again = false;
-      if (h_counters[C_HITS] > hit_cap) { hit_cap = h_counters[C_HITS] + h_counters[C_HITS] / 4; again = true; }
+      if (h_counters[C_CAND_VF] > vf_cap) { vf_cap = h_counters[C_CAND_VF] + h_counters[C_CAND_VF] / 4; again = true; }
+      if (h_counters[C_CAND_EE] > ee_cap) { ee_cap = h_counters[C_CAND_EE] + h_counters[C_CAND_EE] / 4; again = true; }
       if (h_counters[C_DISCRETE] > disc_cap) { disc_cap = h_counters[C_DISCRETE] + h_counters[C_DISCRETE] / 4; again = true; }
-      if (keep_cand && h_counters[C_CANDIDATES] > cand_cap) {
-        cand_cap = h_counters[C_CANDIDATES] + h_counters[C_CANDIDATES] / 4;
-        again = true;
-      }
       if (!again) break;
-      if (attempt > 4) return fail(MCLG_ERR_STATE, "narrow phase: result sizes did not converge");
+      if (attempt > 4) return fail(MCLG_ERR_STATE, "candidate generation: sizes did not converge");
     }
-    counts.candidates = (int64_t)h_counters[C_CANDIDATES];
-    counts.hits = (int64_t)h_counters[C_HITS];
+    n_vf = (long long)h_counters[C_CAND_VF];
+    n_ee = (long long)h_counters[C_CAND_EE];
+    counts.candidates = n_vf + n_ee;
     counts.discrete = (int64_t)h_counters[C_DISCRETE];
-    counts.dist_evals = (int64_t)h_counters[C_DIST_EVALS];
+    // FP64 ACCD over the two queues (BVHTree.cpp:428-461)
+    if (opt.continuous && n_vf + n_ee > 0) {
+      if (hit_cap == 0) hit_cap = std::max<unsigned long long>(1024, (unsigned long long)(n_vf + n_ee) / 2);
+      if (keep_cand) {
+        CHECK(grow((void**)&d_vf_toi, &cap_vf_toi, sizeof(double) * (size_t)std::max(n_vf, 1LL), "candidate toi"));
+        CHECK(grow((void**)&d_ee_toi, &cap_ee_toi, sizeof(double) * (size_t)std::max(n_ee, 1LL), "candidate toi"));
+      }
+      for (int attempt = 0;; ++attempt) {
+        CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
+        CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
+        CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
+        launch_fill_u64(d_counters + C_HITS, 1, 0ull, stream);
+        launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
+        AccdLaunch L;
+        L.rec = d_rec; L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
+        L.counters = d_counters; L.num_cus = num_cus;
+        mark(12);
+        L.q = d_vfq; L.n = n_vf; L.eta = opt.vf_ccd_eta; L.cand_toi = keep_cand ? d_vf_toi : nullptr;
+        launch_accd(dim, true, L, stream);
+        L.q = d_eeq; L.n = n_ee; L.eta = opt.ee_ccd_eta; L.cand_toi = keep_cand ? d_ee_toi : nullptr;
+        if (dim == 3) launch_accd(dim, false, L, stream);
+        mark(13);
+        HIPCHECK(this, hipGetLastError());
+        CHECK(read_counters());
+        if (h_counters[C_HITS] <= hit_cap) break;
+        hit_cap = h_counters[C_HITS] + h_counters[C_HITS] / 4;
+        if (attempt > 4) return fail(MCLG_ERR_STATE, "narrow phase: sizes did not converge");
+      }
+      counts.hits = (int64_t)h_counters[C_HITS];
+      counts.dist_evals = (int64_t)h_counters[C_DIST_EVALS];
+    }
   }
   HIPCHECK(this, hipStreamSynchronize(stream));
   float ms = 0;
@@ -344,10 +364,14 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   times[6] = ms;
   (void)hipEventElapsedTime(&ms, ev[8], ev[9]);
   times[7] = ms;
-  times[8] = 0;
-  if ((opt.continuous || opt.discrete) && npairs > 0) {
+  times[8] = times[9] = 0;
+  if (npairs > 0 && (opt.continuous || opt.discrete)) {
     (void)hipEventElapsedTime(&ms, ev[10], ev[11]);
     times[8] = ms;
+    if (opt.continuous && n_vf + n_ee > 0) {
+      (void)hipEventElapsedTime(&ms, ev[12], ev[13]);
+      times[9] = ms;
+    }
   }
   return MCLG_OK;
 }
@@ -356,7 +380,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
                   d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_counters, d_pairs,
-                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_cand_sten, d_cand_vf, d_cand_toi, d_rank, d_minmax,
+                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -457,13 +481,22 @@ int mclg_copy_discrete(mclg_ctx* h, int* pairs2, int64_t cap) {
 int mclg_copy_candidates(mclg_ctx* h, int* sten4, unsigned char* is_vf, double* toi, int64_t cap) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return MCLG_ERR_ARG;
-  if (!c->keep_cand && c->counts.candidates > 0)
-    return c->fail(MCLG_ERR_STATE, "copy_candidates: traverse was not asked to keep candidates");
   const int64_t n = c->counts.candidates;
   if (cap < n) return c->fail(MCLG_ERR_CAPACITY, "copy_candidates: cap < count");
-  if (sten4) CHECK(copy_out(c, sten4, c->d_cand_sten, sizeof(int4) * (size_t)n));
-  if (is_vf) CHECK(copy_out(c, is_vf, c->d_cand_vf, (size_t)n));
-  if (toi) CHECK(copy_out(c, toi, c->d_cand_toi, sizeof(double) * (size_t)n));
+  if (toi && !c->keep_cand && n > 0)
+    return c->fail(MCLG_ERR_STATE, "copy_candidates: traverse was not asked to keep candidate tois");
+  if (sten4) {
+    CHECK(copy_out(c, sten4, c->d_vfq, sizeof(int4) * (size_t)c->n_vf));
+    CHECK(copy_out(c, sten4 + 4 * c->n_vf, c->d_eeq, sizeof(int4) * (size_t)c->n_ee));
+  }
+  if (is_vf) {
+    std::memset(is_vf, 1, (size_t)c->n_vf);
+    std::memset(is_vf + c->n_vf, 0, (size_t)c->n_ee);
+  }
+  if (toi) {
+    CHECK(copy_out(c, toi, c->d_vf_toi, sizeof(double) * (size_t)c->n_vf));
+    CHECK(copy_out(c, toi + c->n_vf, c->d_ee_toi, sizeof(double) * (size_t)c->n_ee));
+  }
   return MCLG_OK;
 }
 
@@ -494,9 +527,12 @@ int mclg_get_device_results(const mclg_ctx* h, mclg_device_results* out) {
   out->hit_is_vf = c->d_hit_vf;
   out->hit_toi = c->d_hit_toi;
   out->discrete2 = (const int*)c->d_disc;
-  out->cand_sten4 = (const int*)c->d_cand_sten;
-  out->cand_is_vf = c->d_cand_vf;
-  out->cand_toi = c->d_cand_toi;
+  out->vf_sten4 = (const int*)c->d_vfq;
+  out->vf_toi = c->keep_cand ? c->d_vf_toi : nullptr;
+  out->n_vf = c->n_vf;
+  out->ee_sten4 = (const int*)c->d_eeq;
+  out->ee_toi = c->keep_cand ? c->d_ee_toi : nullptr;
+  out->n_ee = c->n_ee;
   out->leaf_pairs2 = (const int*)c->d_pairs;
   return MCLG_OK;
 }

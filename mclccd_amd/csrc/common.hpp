@@ -39,7 +39,8 @@ template <> struct alignas(16) NodeF<2> {
 // Device counters of one traversal (all 64-bit).
 enum Counter : int {
   C_LEAF_PAIRS = 0,
-  C_CANDIDATES,
+  C_CAND_VF,
+  C_CAND_EE,
   C_HITS,
   C_DISCRETE,
   C_DIST_EVALS,

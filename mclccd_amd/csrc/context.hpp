@@ -8,7 +8,7 @@
 
 namespace mclg {
 
-constexpr int kNumEvents = 12;
+constexpr int kNumEvents = 16;
 
 struct Ctx {
   int dim = 3;
@@ -59,10 +59,12 @@ struct Ctx {
   unsigned long long hit_cap = 0;
   int2* d_disc = nullptr; size_t cap_disc = 0;
   unsigned long long disc_cap = 0;
-  int4* d_cand_sten = nullptr; size_t cap_cand_sten = 0;
-  uint8_t* d_cand_vf = nullptr; size_t cap_cand_vf = 0;
-  double* d_cand_toi = nullptr; size_t cap_cand_toi = 0;
-  unsigned long long cand_cap = 0;
+  int4* d_vfq = nullptr; size_t cap_vfq = 0;
+  int4* d_eeq = nullptr; size_t cap_eeq = 0;
+  unsigned long long vf_cap = 0, ee_cap = 0;
+  double* d_vf_toi = nullptr; size_t cap_vf_toi = 0;
+  double* d_ee_toi = nullptr; size_t cap_ee_toi = 0;
+  long long n_vf = 0, n_ee = 0;
   int* d_rank = nullptr; size_t cap_rank = 0;
   int* d_minmax = nullptr; size_t cap_minmax = 0;
   double* d_stage_v0 = nullptr; size_t cap_stage_v0 = 0;

@@ -128,8 +128,9 @@ __device__ __forceinline__ double dist_pe2(const d2& p, const d2& e0, const d2& 
 // sign of zero coordinates, which no comparison and no squared norm can see —
 // its distance equals d0 and is not recomputed; (2) while toi is not > 0 the
 // break test cannot fire, so when toi + t_l > 1 the reference's next distance
-// is unused and the miss is returned without evaluating it.  Returns 1 on hit
-// (toi set), 0 otherwise; *evals counts the distance evaluations performed.
+// is unused and the miss is returned without evaluating it.  g and the first
+// t_l are only needed once the pair is not already within the gap.  Returns 1
+// on hit (toi set), 0 otherwise; evals counts the distance evaluations made.
 // ---------------------------------------------------------------------------
 template <int D, bool VF> struct Stencil;
 
@@ -213,39 +214,31 @@ __device__ __forceinline__ int accd(const V* v0, const V* v1, double xsi, double
   Stencil<D, VF> st;
   bool fin;
   const double l_p = st.setup(v0, v1, fin);
+  evals = 0;
   if (l_p <= 0) return 0;
   double d = st.dist();
   evals = 1;
-  const double g = 1.0 * (d * d - xsi * xsi) / (d - xsi);
   toi = 0.0;
-  const double tl0 = (1.0 - 1.0) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   if (d <= xsi) return 1;
-  // iteration 0
-  if (!(tl0 == 0.0 && fin)) {
-    st.step(tl0);
-    d = st.dist();
-    ++evals;
-  }
-  toi += tl0;
-  if (toi > 1.0) return 0;
-  double tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
-  for (int iter = 1; iter < 1000; ++iter) {
-    if (!(toi > 0.0)) {
-      const double nt = toi + tl;
-      if (nt > 1.0) return 0;
+  const double g = 1.0 * (d * d - xsi * xsi) / (d - xsi);
+  double tl = (1.0 - 1.0) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  bool skip = tl == 0.0 && fin;  // iteration 0 leaves x unchanged: its distance is d0
+  // One distance evaluation site per iteration (reference loop NarrowPhase.cpp:684-707).
+  for (int iter = 0; iter < 1000; ++iter) {
+    const bool can_break = toi > 0.0 && iter > 0;
+    if (!can_break && toi + tl > 1.0) return 0;  // the reference's distance here is unused
+    if (!skip) {
       st.step(tl);
       d = st.dist();
       ++evals;
-      toi = nt;
-    } else {
-      st.step(tl);
-      d = st.dist();
-      ++evals;
+    }
+    skip = false;
+    if (can_break) {
       const double eps = (d * d - xsi * xsi) / (d + xsi);
       if (eps < g) return 1;
-      toi += tl;
-      if (toi > 1.0) return 0;
     }
+    toi += tl;
+    if (toi > 1.0) return 0;
     tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   }
   return 1;  // max_iter reached: the reference returns true (its assert is compiled out)

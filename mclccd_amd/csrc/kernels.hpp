@@ -25,28 +25,37 @@ void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* order, int n, co
 void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
                      const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters, hipStream_t s);
 
-struct NarrowLaunch {
+struct ExpandLaunch {
   const int2* pairs;
   long long npairs;
   const int* P;
   const uint8_t* flags;
   const double* rec;
-  double vf_eta, ee_eta;
-  int discrete, continuous, want_candidates;
+  int discrete, continuous;
+  int4* vfq;
+  unsigned long long vf_cap;
+  int4* eeq;
+  unsigned long long ee_cap;
+  int2* disc;
+  unsigned long long disc_cap;
+  unsigned long long* counters;
+  int num_cus;
+};
+void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s);
+
+struct AccdLaunch {
+  const int4* q;
+  long long n;
+  const double* rec;
+  double eta;
+  double* cand_toi;
   int4* hit_sten;
   uint8_t* hit_vf;
   double* hit_toi;
   unsigned long long hit_cap;
-  int2* disc;
-  unsigned long long disc_cap;
-  int4* cand_sten;
-  uint8_t* cand_vf;
-  double* cand_toi;
-  unsigned long long cand_cap;
   unsigned long long* counters;
   int num_cus;
-  int blocks_per_cu;
 };
-void launch_narrow(int dim, const NarrowLaunch& L, hipStream_t s);
+void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
 
 }  // namespace mclg

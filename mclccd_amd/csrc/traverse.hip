@@ -199,105 +199,71 @@ template <> struct VLoad<2> {
   }
 };
 
-// exclusive prefix over the wave of a small count (< 2^bits) and the total
-template <int BITS> __device__ __forceinline__ int wave_prefix(int c, int& total) {
-  int pre = 0, tot = 0;
-#pragma unroll
-  for (int b = 0; b < BITS; ++b) {
-    const uint64_t m = __ballot((c >> b) & 1);
-    pre += popc_below(m) << b;
-    tot += __popcll(m) << b;
-  }
-  total = tot;
-  return pre;
+// ---------------------------------------------------------------------------
+// Per-wave LDS staging of output records: lanes deposit records at
+// ballot-computed slots; a full buffer is written out with one global atomic.
+// Called with the whole wave converged (pred carries per-lane validity).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void stage_flush(T* buf, int cnt, T* __restrict__ dst, unsigned long long cap,
+                                            unsigned long long* counter) {
+  wave_lds_sync();
+  unsigned long long base = 0;
+  const int lane = lane_id();
+  if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+  base = __shfl(base, 0);
+  for (int i = lane; i < cnt; i += kWave)
+    if (base + i < cap) dst[base + i] = buf[i];
+  wave_lds_sync();
 }
 
-struct NarrowArgs {
+template <typename T, int CAP>
+__device__ __forceinline__ void stage_push(T* buf, int& cnt, bool pred, const T& v, T* __restrict__ dst,
+                                           unsigned long long cap, unsigned long long* counter) {
+  const uint64_t m = __ballot(pred);
+  if (m == 0) return;
+  if (pred) buf[cnt + popc_below(m)] = v;
+  cnt += __popcll(m);
+  if (cnt > CAP - kWave) {
+    stage_flush(buf, cnt, dst, cap, counter);
+    cnt = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_expand: per leaf pair (BVHTree.cpp:252-283) the discrete test at V1 and the
+// representative-triangle candidates, appended to a VF and an EE stencil queue.
+// ---------------------------------------------------------------------------
+struct ExpandArgs {
   const int2* pairs;
   long long npairs;
   const int* P;
   const uint8_t* flags;
   const double* rec;
-  double vf_eta, ee_eta;
-  int discrete, continuous, want_candidates;
-  int4* hit_sten;
-  uint8_t* hit_vf;
-  double* hit_toi;
-  unsigned long long hit_cap;
+  int discrete, continuous;
+  int4* vfq;
+  unsigned long long vf_cap;
+  int4* eeq;
+  unsigned long long ee_cap;
   int2* disc;
   unsigned long long disc_cap;
-  int4* cand_sten;
-  uint8_t* cand_vf;
-  double* cand_toi;
-  unsigned long long cand_cap;
   unsigned long long* counters;
 };
 
-template <int D, bool VF>
-__device__ __forceinline__ void run_stencils(const NarrowArgs& A, const uint16_t* list, int count, const int* fl,
-                                             int lane, long long& evals) {
-  using V = typename VLoad<D>::V;
-  for (int base = 0; base < count; base += kWave) {
-    const int e = base + lane;
-    const bool live = e < count;
-    int sv[4] = {-1, -1, -1, -1};
-    if (live) {
-      const uint32_t ent = list[e];
-      const int src = (int)(ent >> 4), code = (int)(ent & 15u);
-      const int* f = fl + src * 6;  // f0 = f[0..2], f1 = f[3..5]
-      if (VF) {
-        const int from = code < 3 ? 0 : 3, to = code < 3 ? 3 : 0, i = code < 3 ? code : code - 3;
-        sv[0] = f[from + i];
-#pragma unroll
-        for (int j = 0; j < D; ++j) sv[1 + j] = f[to + j];
-      } else {
-        const int c = code - 6, i = c / 3, j = c - 3 * (c / 3);
-        sv[0] = f[i]; sv[1] = f[(i + 1) % 3]; sv[2] = f[3 + j]; sv[3] = f[3 + (j + 1) % 3];
-      }
-    }
-    long long cslot = -1;
-    if (A.want_candidates) {
-      cslot = wave_append(live, A.counters + C_CANDIDATES);
-      if (live && cslot < (long long)A.cand_cap) {
-        A.cand_sten[cslot] = make_int4(sv[0], sv[1], sv[2], sv[3]);
-        A.cand_vf[cslot] = VF ? 1 : 0;
-      }
-    }
-    int hit = 0, ev = 0;
-    double toi = -1.0;
-    if (live) {
-      V v0[D + 1], v1[D + 1];
-#pragma unroll
-      for (int i = 0; i < D + 1; ++i) VLoad<D>::load(A.rec, sv[i], v0[i], v1[i]);
-      hit = accd<D, VF>(v0, v1, VF ? A.vf_eta : A.ee_eta, toi, ev);
-    }
-    evals += ev;
-    if (live && cslot >= 0 && cslot < (long long)A.cand_cap) A.cand_toi[cslot] = hit ? toi : -1.0;
-    const long long slot = wave_append(hit != 0, A.counters + C_HITS);
-    if (hit && slot < (long long)A.hit_cap) {
-      A.hit_sten[slot] = make_int4(sv[0], sv[1], sv[2], sv[3]);
-      A.hit_vf[slot] = VF ? 1 : 0;
-      A.hit_toi[slot] = toi;
-    }
-  }
-}
+constexpr int EX_WAVES = 4, EX_STAGE = 256, EX_DSTAGE = 128;
 
-constexpr int NW = 4;  // waves per block
-
-template <int D> __global__ __launch_bounds__(NW * kWave) void k_narrow(NarrowArgs A) {
-  __shared__ uint16_t s_vf[NW][kWave * 2 * D];
-  __shared__ uint16_t s_ee[NW][D == 3 ? kWave * 9 : 1];
-  __shared__ int s_f[NW][kWave * 6];
+template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
+  __shared__ int4 s_vf[EX_WAVES][EX_STAGE];
+  __shared__ int4 s_ee[EX_WAVES][D == 3 ? EX_STAGE : 1];
+  __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
   const int w = threadIdx.x / kWave, lane = lane_id();
+  int nvf = 0, nee = 0, nd = 0;  // staged records (wave-uniform)
   const long long nchunks = (A.npairs + kWave - 1) / kWave;
-  const long long gw = (long long)xcd_block() * NW + w;
-  const long long stride = (long long)gridDim.x * NW;
-  long long evals = 0;
-  unsigned long long ncand = 0;
-  for (long long ch = gw; ch < nchunks; ch += stride) {
+  const long long stride = (long long)gridDim.x * EX_WAVES;
+  for (long long ch = (long long)xcd_block() * EX_WAVES + w; ch < nchunks; ch += stride) {
     const long long i = ch * kWave + lane;
     const bool valid = i < A.npairs;
-    int f0[3] = {-1, -1, -1}, f1[3] = {-1, -1, -1};
+    int f0[3] = {-1, -1, -1}, f1[3] = {-2, -2, -2};
     int2 pr = make_int2(0, 0);
     uint32_t fa = 0, fb = 0;
     if (valid) {
@@ -307,8 +273,7 @@ template <int D> __global__ __launch_bounds__(NW * kWave) void k_narrow(NarrowAr
       fa = A.flags[pr.x];
       fb = A.flags[pr.y];
     }
-    // ---- discrete test at V1 (BVHTree.cpp:464-500) ----
-    if (A.discrete) {
+    if (A.discrete) {  // BVHTree.cpp:464-500
       bool shared = false;
 #pragma unroll
       for (int x = 0; x < D; ++x)
@@ -317,22 +282,17 @@ template <int D> __global__ __launch_bounds__(NW * kWave) void k_narrow(NarrowAr
       int dh = 0;
       if (valid && !shared) {
         if constexpr (D == 3) {
-          const d3 p1 = VLoad<3>::load_t1(A.rec, f0[0]), q1 = VLoad<3>::load_t1(A.rec, f0[1]),
-                   r1 = VLoad<3>::load_t1(A.rec, f0[2]);
-          const d3 p2 = VLoad<3>::load_t1(A.rec, f1[0]), q2 = VLoad<3>::load_t1(A.rec, f1[1]),
-                   r2 = VLoad<3>::load_t1(A.rec, f1[2]);
-          dh = tri_tri3(p1, q1, r1, p2, q2, r2);
+          dh = tri_tri3(VLoad<3>::load_t1(A.rec, f0[0]), VLoad<3>::load_t1(A.rec, f0[1]), VLoad<3>::load_t1(A.rec, f0[2]),
+                        VLoad<3>::load_t1(A.rec, f1[0]), VLoad<3>::load_t1(A.rec, f1[1]), VLoad<3>::load_t1(A.rec, f1[2]));
         } else {
-          const d2 p0 = VLoad<2>::load_t1(A.rec, f0[0]), p1 = VLoad<2>::load_t1(A.rec, f0[1]);
-          const d2 q0 = VLoad<2>::load_t1(A.rec, f1[0]), q1 = VLoad<2>::load_t1(A.rec, f1[1]);
-          dh = edge_edge2(p0, p1, q0, q1);
+          dh = edge_edge2(VLoad<2>::load_t1(A.rec, f0[0]), VLoad<2>::load_t1(A.rec, f0[1]), VLoad<2>::load_t1(A.rec, f1[0]),
+                          VLoad<2>::load_t1(A.rec, f1[1]));
         }
       }
-      const long long slot = wave_append(dh != 0, A.counters + C_DISCRETE);
-      if (dh && slot < (long long)A.disc_cap) A.disc[slot] = pr;
+      stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, pr, A.disc, A.disc_cap, A.counters + C_DISCRETE);
     }
     if (!A.continuous) continue;
-    // ---- candidate codes (BVHTree.cpp:305-398) ----
+    // BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (6 + 3i + j)
     uint32_t vfm = 0, eem = 0;
     if (valid) {
 #pragma unroll
@@ -346,57 +306,176 @@ template <int D> __global__ __launch_bounds__(NW * kWave) void k_narrow(NarrowAr
       if constexpr (D == 3) {
 #pragma unroll
         for (int x = 0; x < 3; ++x) {
-          if (!((fa >> (3 + x)) & 1u)) continue;
 #pragma unroll
           for (int y = 0; y < 3; ++y) {
-            if (!((fb >> (3 + y)) & 1u)) continue;
             const int a0 = f0[x], a1 = f0[(x + 1) % 3], b0 = f1[y], b1 = f1[(y + 1) % 3];
-            if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-            eem |= 1u << (3 * x + y);
+            const bool ok = ((fa >> (3 + x)) & 1u) && ((fb >> (3 + y)) & 1u) && a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1;
+            if (ok) eem |= 1u << (3 * x + y);
           }
         }
       }
     }
-    int nvf, nee = 0;
-    int off = wave_prefix<3>(__popc(vfm), nvf);
-    for (uint32_t m = vfm; m; m &= m - 1) s_vf[w][off++] = (uint16_t)((lane << 4) | (__ffs(m) - 1));
-    if constexpr (D == 3) {
-      off = wave_prefix<4>(__popc(eem), nee);
-      for (uint32_t m = eem; m; m &= m - 1) s_ee[w][off++] = (uint16_t)((lane << 4) | (6 + __ffs(m) - 1));
+    // push one code per lane per round; rounds = the wave's largest count
+    for (;;) {
+      const bool has = vfm != 0;
+      if (__ballot(has) == 0) break;
+      int4 st = make_int4(-1, -1, -1, -1);
+      if (has) {
+        const int code = __ffs(vfm) - 1;
+        vfm &= vfm - 1;
+        const bool fwd = code < 3;  // vertex of f0 against face f1, else the reverse
+        const int k = fwd ? code : code - 3;
+        const int a0 = fwd ? f0[0] : f1[0], a1 = fwd ? f0[1] : f1[1], a2 = fwd ? f0[2] : f1[2];
+        st.x = k == 0 ? a0 : (k == 1 ? a1 : a2);
+        st.y = fwd ? f1[0] : f0[0];
+        st.z = fwd ? f1[1] : f0[1];
+        st.w = D == 3 ? (fwd ? f1[2] : f0[2]) : -1;
+      }
+      stage_push<int4, EX_STAGE>(s_vf[w], nvf, has, st, A.vfq, A.vf_cap, A.counters + C_CAND_VF);
     }
-#pragma unroll
-    for (int j = 0; j < 3; ++j) { s_f[w][lane * 6 + j] = f0[j]; s_f[w][lane * 6 + 3 + j] = f1[j]; }
-    wave_lds_sync();
-    ncand += (unsigned long long)(nvf + nee);
-    run_stencils<D, true>(A, s_vf[w], nvf, s_f[w], lane, evals);
-    if constexpr (D == 3) run_stencils<D, false>(A, s_ee[w], nee, s_f[w], lane, evals);
-    wave_lds_sync();
+    if constexpr (D == 3) {
+      for (;;) {
+        const bool has = eem != 0;
+        if (__ballot(has) == 0) break;
+        int4 st = make_int4(-1, -1, -1, -1);
+        if (has) {
+          const int c = __ffs(eem) - 1;
+          eem &= eem - 1;
+          const int x = c / 3, y = c - 3 * (c / 3);
+          st.x = x == 0 ? f0[0] : (x == 1 ? f0[1] : f0[2]);
+          st.y = x == 0 ? f0[1] : (x == 1 ? f0[2] : f0[0]);
+          st.z = y == 0 ? f1[0] : (y == 1 ? f1[1] : f1[2]);
+          st.w = y == 0 ? f1[1] : (y == 1 ? f1[2] : f1[0]);
+        }
+        stage_push<int4, EX_STAGE>(s_ee[w], nee, has, st, A.eeq, A.ee_cap, A.counters + C_CAND_EE);
+      }
+    }
   }
-  // per-wave totals
-  for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
-  if (lane == 0) {
-    atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
-    if (!A.want_candidates) atomicAdd(A.counters + C_CANDIDATES, ncand);
-  }
+  if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
+  if (nvf) stage_flush(s_vf[w], nvf, A.vfq, A.vf_cap, A.counters + C_CAND_VF);
+  if constexpr (D == 3)
+    if (nee) stage_flush(s_ee[w], nee, A.eeq, A.ee_cap, A.counters + C_CAND_EE);
 }
 
-void launch_narrow(int dim, const NarrowLaunch& L, hipStream_t s) {
+// ---------------------------------------------------------------------------
+// k_accd: FP64 ACCD (NarrowPhase.cpp:634-711) over a packed stencil queue, one
+// stencil per lane; hits staged in LDS.  VF and EE queues run as separate
+// launches so every SIMD step executes one distance kernel.
+// ---------------------------------------------------------------------------
+struct AccdArgs {
+  const int4* q;
+  long long n;
+  const double* rec;
+  double eta;
+  double* cand_toi;  // per-stencil toi (-1 = no hit) or null
+  int4* hit_sten;
+  uint8_t* hit_vf;
+  double* hit_toi;
+  unsigned long long hit_cap;
+  unsigned long long* counters;
+};
+
+struct HitRec {
+  int4 s;
+  double t;
+};
+
+constexpr int AC_WAVES = 4, AC_STAGE = 256;
+
+template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
+  using V = typename VLoad<D>::V;
+  __shared__ HitRec s_h[AC_WAVES][AC_STAGE];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  int nh = 0;
+  long long evals = 0;
+  const long long nchunks = (A.n + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * AC_WAVES;
+  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
+    const long long i = ch * kWave + lane;
+    const bool live = i < A.n;
+    int4 sv = make_int4(0, 0, 0, 0);
+    int hit = 0, ev = 0;
+    double toi = -1.0;
+    if (live) {
+      sv = A.q[i];
+      const int id[4] = {sv.x, sv.y, sv.z, sv.w};
+      V v0[D + 1], v1[D + 1];
+#pragma unroll
+      for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
+      hit = accd<D, VF>(v0, v1, A.eta, toi, ev);
+      if (A.cand_toi) A.cand_toi[i] = hit ? toi : -1.0;
+    }
+    evals += ev;
+    // hits: staged, then written to the three SoA arrays
+    const uint64_t m = __ballot(hit != 0);
+    if (m) {
+      if (hit) s_h[w][nh + popc_below(m)] = HitRec{sv, toi};
+      nh += __popcll(m);
+      if (nh > AC_STAGE - kWave) {
+        wave_lds_sync();
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)nh);
+        base = __shfl(base, 0);
+        for (int k = lane; k < nh; k += kWave)
+          if (base + k < A.hit_cap) {
+            A.hit_sten[base + k] = s_h[w][k].s;
+            A.hit_toi[base + k] = s_h[w][k].t;
+            A.hit_vf[base + k] = VF ? 1 : 0;
+          }
+        nh = 0;
+        wave_lds_sync();
+      }
+    }
+  }
+  if (nh) {
+    wave_lds_sync();
+    unsigned long long base = 0;
+    if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)nh);
+    base = __shfl(base, 0);
+    for (int k = lane; k < nh; k += kWave)
+      if (base + k < A.hit_cap) {
+        A.hit_sten[base + k] = s_h[w][k].s;
+        A.hit_toi[base + k] = s_h[w][k].t;
+        A.hit_vf[base + k] = VF ? 1 : 0;
+      }
+  }
+  for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
+  if (lane == 0 && evals) atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
+}
+
+static int grid_for(long long chunks, int waves, int num_cus, int per_cu) {
+  long long b = (chunks + waves - 1) / waves;
+  const long long maxb = (long long)num_cus * per_cu;
+  if (b > maxb) b = maxb;
+  if (b < 1) b = 1;
+  return (int)((b + 7) / 8 * 8);
+}
+
+void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
   if (L.npairs <= 0) return;
-  NarrowArgs A;
+  ExpandArgs A;
   A.pairs = L.pairs; A.npairs = L.npairs; A.P = L.P; A.flags = L.flags; A.rec = L.rec;
-  A.vf_eta = L.vf_eta; A.ee_eta = L.ee_eta;
-  A.discrete = L.discrete; A.continuous = L.continuous; A.want_candidates = L.want_candidates;
+  A.discrete = L.discrete; A.continuous = L.continuous;
+  A.vfq = L.vfq; A.vf_cap = L.vf_cap; A.eeq = L.eeq; A.ee_cap = L.ee_cap;
+  A.disc = L.disc; A.disc_cap = L.disc_cap; A.counters = L.counters;
+  const int g = grid_for((L.npairs + kWave - 1) / kWave, EX_WAVES, L.num_cus, 8);
+  if (dim == 3) k_expand<3><<<g, EX_WAVES * kWave, 0, s>>>(A);
+  else k_expand<2><<<g, EX_WAVES * kWave, 0, s>>>(A);
+}
+
+void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
+  if (L.n <= 0) return;
+  AccdArgs A;
+  A.q = L.q; A.n = L.n; A.rec = L.rec; A.eta = L.eta; A.cand_toi = L.cand_toi;
   A.hit_sten = L.hit_sten; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
-  A.disc = L.disc; A.disc_cap = L.disc_cap;
-  A.cand_sten = L.cand_sten; A.cand_vf = L.cand_vf; A.cand_toi = L.cand_toi; A.cand_cap = L.cand_cap;
   A.counters = L.counters;
-  const long long chunks = (L.npairs + kWave - 1) / kWave;
-  long long blocks = (chunks + NW - 1) / NW;
-  const long long maxb = (long long)L.num_cus * L.blocks_per_cu;
-  if (blocks > maxb) blocks = maxb;
-  blocks = (blocks + 7) / 8 * 8;
-  if (dim == 3) k_narrow<3><<<(int)blocks, NW * kWave, 0, s>>>(A);
-  else k_narrow<2><<<(int)blocks, NW * kWave, 0, s>>>(A);
+  const int g = grid_for((L.n + kWave - 1) / kWave, AC_WAVES, L.num_cus, 8);
+  if (dim == 3) {
+    if (vf) k_accd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    else k_accd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A);
+  } else {
+    k_accd<2, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
+  }
 }
 
 }  // namespace mclg
