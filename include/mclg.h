@@ -161,6 +161,49 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
 #define MCLG_NUM_TIMINGS 10
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
 
+/* ------------------------------------------------------------------------
+ * Batched scenes.  No reference counterpart: the reference runs one
+ * mcl::BVHTree per scene, one after the other.  Here N independent scenes
+ * (each exactly the problem one BVHTree::update + traverse solves, with its
+ * own gaps) run as one pass over concatenated arrays: one LBVH keyed by
+ * (scene, Morton code), every kernel's queue length read on the device, the
+ * whole frame captured once as a hipGraph and replayed.
+ * ------------------------------------------------------------------------ */
+typedef struct mclg_batch mclg_batch;
+int mclg_batch_create(int dim, mclg_batch** out);
+void mclg_batch_destroy(mclg_batch* b);
+const char* mclg_batch_last_error(const mclg_batch* b);
+
+/* Scene layout: scene s owns vertices [voff[s], voff[s+1]) and primitives
+ * [poff[s], poff[s+1]) of the concatenated arrays; P (poff[n] x dim) holds each
+ * scene's own vertex indices.  opts: one entry for all scenes or one per scene
+ * (gaps per scene; reptri/discrete/continuous must agree).  Computes the
+ * representative-triangle flags once for this topology.  Host pointers. */
+int mclg_batch_set_scenes(mclg_batch* b, int nscenes, const int* voff, const int* poff, const int* P,
+                          const mclg_options* opts, int nopts);
+
+/* One frame (update + traverse of every scene).  V0, V1: voff[n] x dim,
+ * concatenated; host pointers, or device pointers when device_ptrs != 0. */
+int mclg_batch_run(mclg_batch* b, const double* V0, const double* V1, int device_ptrs);
+/* Host milliseconds of the last frame (graph launch to counters on the host). */
+double mclg_batch_last_ms(const mclg_batch* b);
+
+int mclg_batch_get_counts(const mclg_batch* b, mclg_counts* out);
+int mclg_batch_scene_counts(mclg_batch* b, int64_t* hits, int64_t* discrete);
+/* Results with their scene and scene-local indices. */
+int mclg_batch_copy_hits(mclg_batch* b, int* scene, int* sten4, unsigned char* is_vf, double* toi, int64_t cap);
+int mclg_batch_copy_discrete(mclg_batch* b, int* scene, int* pairs2, int64_t cap);
+
+/* Execution parameters (negative / zero: keep): hipGraph replay on/off, waves
+ * per traversal block (1-8), resident blocks per CU of the queue kernels,
+ * 64-item chunks a wave takes per step.  out4 receives the current values. */
+int mclg_batch_set_params(mclg_batch* b, int use_graph, int tr_waves, int blocks_per_cu, int chunk);
+int mclg_batch_get_params(const mclg_batch* b, int* out4);
+/* Times frames on the last inputs over a small grid of the parameters above and
+ * keeps the fastest; results are cached per (GPU architecture, dim, scene
+ * count, primitive count) in cache_path (NULL: no cache). */
+int mclg_batch_autotune(mclg_batch* b, const char* cache_path, int reps);
+
 #ifdef __cplusplus
 }
 #endif

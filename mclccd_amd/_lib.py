@@ -74,6 +74,19 @@ def _load():
         "mclg_copy_leaf_pairs": (I, [P, P, I64]),
         "mclg_get_device_results": (I, [P, P]),
         "mclg_copy_leaves": (I, [P, P, P, P, I64]),
+        "mclg_batch_create": (I, [I, P]),
+        "mclg_batch_destroy": (None, [P]),
+        "mclg_batch_last_error": (C.c_char_p, [P]),
+        "mclg_batch_set_scenes": (I, [P, I, P, P, P, P, I]),
+        "mclg_batch_run": (I, [P, P, P, I]),
+        "mclg_batch_last_ms": (C.c_double, [P]),
+        "mclg_batch_get_counts": (I, [P, P]),
+        "mclg_batch_scene_counts": (I, [P, P, P]),
+        "mclg_batch_copy_hits": (I, [P, P, P, P, P, I64]),
+        "mclg_batch_copy_discrete": (I, [P, P, P, I64]),
+        "mclg_batch_set_params": (I, [P, I, I, I, I]),
+        "mclg_batch_get_params": (I, [P, P]),
+        "mclg_batch_autotune": (I, [P, C.c_char_p, I]),
         "mclg_set_rank": (I, [P, P, I]),
         "mclg_get_timings": (I, [P, P, I]),
     }

@@ -166,8 +166,9 @@ template <int D>
 __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ rec, const int* __restrict__ P, int np,
                                                     int pdim, const int* __restrict__ active, double eta,
                                                     LeafBox<D>* __restrict__ boxes, uint8_t* __restrict__ act,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, SceneArgs S) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (S.prim_scene && i < np) eta = S.box_eta[S.prim_scene[i]];
   double clo[D], chi[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) { clo[k] = __builtin_huge_val(); chi[k] = -__builtin_huge_val(); }
@@ -206,6 +207,36 @@ __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ r
     }
     boxes[i] = b;
     act[i] = a ? 1 : 0;
+  }
+  if (S.prim_scene) {
+    // several scenes: per-scene bounds; a wave whose primitives all belong to
+    // one scene reduces first and issues one atomic per bound
+    const int sc = i < np ? S.prim_scene[i] : -1;
+    const int sc0 = __shfl(sc, 0);
+    if (__ballot(i < np && sc != sc0) == 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        for (int o = 32; o > 0; o >>= 1) {
+          const double l2 = __shfl_xor(clo[k], o), h2 = __shfl_xor(chi[k], o);
+          clo[k] = l2 < clo[k] ? l2 : clo[k];
+          chi[k] = h2 > chi[k] ? h2 : chi[k];
+        }
+      }
+      if (lane_id() == 0 && sc0 >= 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          atomicMin(S.bounds + (size_t)sc0 * 2 * D + k, (unsigned long long)dbl_to_ord(clo[k]));
+          atomicMax(S.bounds + (size_t)sc0 * 2 * D + D + k, (unsigned long long)dbl_to_ord(chi[k]));
+        }
+      }
+    } else if (i < np) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        atomicMin(S.bounds + (size_t)sc * 2 * D + k, (unsigned long long)dbl_to_ord(clo[k]));
+        atomicMax(S.bounds + (size_t)sc * 2 * D + D + k, (unsigned long long)dbl_to_ord(chi[k]));
+      }
+    }
+    return;
   }
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -280,15 +311,31 @@ __global__ __launch_bounds__(256) void k_reduce_bounds(const double* __restrict_
 int leaf_box_blocks(int np) { return (np + 255) / 256; }
 
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
-                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, hipStream_t s) {
+                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, const SceneArgs& S,
+                       hipStream_t s) {
   const int g = leaf_box_blocks(np);
-  if (dim == 3) {
-    k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, partial);
-    k_reduce_bounds<3><<<1, 256, 0, s>>>(partial, g, cbounds);
-  } else {
-    k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, partial);
-    k_reduce_bounds<2><<<1, 256, 0, s>>>(partial, g, cbounds);
+  if (S.prim_scene) {
+    const unsigned long long lo0 = dbl_to_ord(__builtin_huge_val()), hi0 = dbl_to_ord(-__builtin_huge_val());
+    launch_bounds_init(S.bounds, S.n, dim, lo0, hi0, s);
   }
+  if (dim == 3) {
+    k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, partial, S);
+    if (!S.prim_scene) k_reduce_bounds<3><<<1, 256, 0, s>>>(partial, g, cbounds);
+  } else {
+    k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, partial, S);
+    if (!S.prim_scene) k_reduce_bounds<2><<<1, 256, 0, s>>>(partial, g, cbounds);
+  }
+}
+
+__global__ void k_bounds_init(unsigned long long* b, int nscenes, int dim, unsigned long long lo0, unsigned long long hi0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nscenes * 2 * dim) return;
+  b[i] = (i % (2 * dim)) < dim ? lo0 : hi0;
+}
+void launch_bounds_init(unsigned long long* b, int nscenes, int dim, unsigned long long lo0, unsigned long long hi0,
+                        hipStream_t s) {
+  const int n = nscenes * 2 * dim;
+  k_bounds_init<<<(n + 255) / 256, 256, 0, s>>>(b, nscenes, dim, lo0, hi0);
 }
 
 // ---------------------------------------------------------------------------
@@ -313,15 +360,17 @@ __device__ __forceinline__ uint32_t spread2(uint32_t v) {  // 16 bits -> every 2
 
 template <int D>
 __global__ void k_morton(const LeafBox<D>* __restrict__ boxes, int np, const unsigned long long* __restrict__ cbounds,
-                         uint32_t key_base, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+                         SceneArgs S, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                         uint32_t* __restrict__ mort) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= np) return;
   const LeafBox<D>& b = boxes[i];
+  const unsigned long long* bd = S.prim_scene ? S.bounds + (size_t)S.prim_scene[i] * 2 * D : cbounds;
   constexpr double cells = D == 3 ? 1024.0 : 65536.0;
   uint32_t q[D];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    const double lo = ord_to_dbl(cbounds[k]), hi = ord_to_dbl(cbounds[D + k]);
+    const double lo = ord_to_dbl(bd[k]), hi = ord_to_dbl(bd[D + k]);
     const double c = (b.sw_lo[k] + b.sw_hi[k]) / 2.0;
     const double ext = hi - lo;
     double u = ext > 0 ? (c - lo) / ext * cells : 0.0;
@@ -331,15 +380,35 @@ __global__ void k_morton(const LeafBox<D>* __restrict__ boxes, int np, const uns
   uint32_t code;
   if constexpr (D == 3) code = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
   else code = (spread2(q[0]) << 1) | spread2(q[1]);
-  keys[i] = code + key_base;
+  keys[i] = code;
   vals[i] = (uint32_t)i;
+  if (mort) mort[i] = code;
 }
 
-void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, uint32_t* keys,
-                   uint32_t* vals, hipStream_t s) {
+void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, const SceneArgs& S,
+                   uint32_t* keys, uint32_t* vals, uint32_t* mort, hipStream_t s) {
   const int g = (np + 255) / 256;
-  if (dim == 3) k_morton<3><<<g, 256, 0, s>>>((const LeafBox<3>*)boxes, np, cbounds, 0, keys, vals);
-  else k_morton<2><<<g, 256, 0, s>>>((const LeafBox<2>*)boxes, np, cbounds, 0, keys, vals);
+  if (dim == 3) k_morton<3><<<g, 256, 0, s>>>((const LeafBox<3>*)boxes, np, cbounds, S, keys, vals, mort);
+  else k_morton<2><<<g, 256, 0, s>>>((const LeafBox<2>*)boxes, np, cbounds, S, keys, vals, mort);
+}
+
+// Scene keys of the Morton-sorted order, and the Morton codes regathered after
+// the stable sort by scene: together the (scene, Morton) order.
+__global__ void k_scene_keys(const uint32_t* __restrict__ order, int n, const int* __restrict__ prim_scene,
+                             uint32_t* __restrict__ hi) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) hi[j] = (uint32_t)prim_scene[order[j]];
+}
+__global__ void k_gather_u32(const uint32_t* __restrict__ order, int n, const uint32_t* __restrict__ src,
+                             uint32_t* __restrict__ dst) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) dst[j] = src[order[j]];
+}
+void launch_scene_keys(const uint32_t* order, int n, const int* prim_scene, uint32_t* hi, hipStream_t s) {
+  k_scene_keys<<<(n + 255) / 256, 256, 0, s>>>(order, n, prim_scene, hi);
+}
+void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32_t* dst, hipStream_t s) {
+  k_gather_u32<<<(n + 255) / 256, 256, 0, s>>>(order, n, src, dst);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,10 +526,13 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t*
 // Boxes are FP32 rounded outward from the FP64 leaf boxes: every culling test
 // on them is conservative; the exact FP64 test runs at leaf pairs.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t split_key(const uint32_t* keys, int k) {
-  const uint32_t x = keys[k] ^ keys[k + 1];
+__device__ __forceinline__ uint64_t split_key(const uint32_t* lo, const uint32_t* hi, int k) {
+  const uint64_t a = ((uint64_t)(hi ? hi[k] : 0u) << 32) | lo[k];
+  const uint64_t b = ((uint64_t)(hi ? hi[k + 1] : 0u) << 32) | lo[k + 1];
+  const uint64_t x = a ^ b;
+  const uint32_t hb = x ? 64u - (uint32_t)__clzll((long long)x) : 0u;  // highest differing bit + 1
   const uint32_t ones = (uint32_t)__builtin_ctz(~(uint32_t)k);
-  return ((uint64_t)x << 32) | ((uint64_t)ones << 26) | (uint32_t)k;
+  return ((uint64_t)hb << 32) | ((uint64_t)ones << 26) | (uint32_t)k;
 }
 
 template <int D> __device__ __forceinline__ void merge_node(NodeF<D>& o, const NodeF<D>& a, const NodeF<D>& b) {
@@ -474,10 +546,11 @@ template <int D> __device__ __forceinline__ void merge_node(NodeF<D>& o, const N
 }
 
 template <int D>
-__global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ order,
-                                                    int n, const LeafBox<D>* __restrict__ boxes,
+__global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ hi,
+                                                    const uint32_t* __restrict__ order, int n,
+                                                    const LeafBox<D>* __restrict__ boxes,
                                                     const uint8_t* __restrict__ act, NodeF<D>* nodes, int* lchild,
-                                                    int* rchild, int* rng_l, int* rng_r, int* visits) {
+                                                    int* rchild, int* rng_l, int* rng_r, int* visits, SceneArgs S) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   // leaf node
@@ -494,12 +567,16 @@ __global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__
     L.child = (int)prim;
     L.meta = act[prim] ? 0x80000000u : 0u;
     nodes[(n - 1) + s] = L;
+    if (S.root) {  // a one-primitive scene is its own subtree
+      const int sc = (int)hi[s];
+      if (S.prim_off[sc + 1] - S.prim_off[sc] == 1) S.root[sc] = (n - 1) + s;
+    }
   }
   if (n == 1) return;
   int l = s, r = s, node = (n - 1) + s;
   for (;;) {
     int parent;
-    if (l == 0 || (r != n - 1 && split_key(keys, r) < split_key(keys, l - 1))) {
+    if (l == 0 || (r != n - 1 && split_key(keys, hi, r) < split_key(keys, hi, l - 1))) {
       parent = r;
       lchild[parent] = node;
       rng_l[parent] = l;
@@ -524,6 +601,10 @@ __global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__
     o.child = lc;
     nodes[parent] = o;
     node = parent;
+    if (S.root) {  // the node spanning exactly one scene's leaves is that scene's subtree
+      const int sc = (int)hi[l];
+      if (l == S.prim_off[sc] && r == S.prim_off[sc + 1] - 1) S.root[sc] = parent;
+    }
     if (l == 0 && r == n - 1) return;  // root
   }
 }
@@ -538,19 +619,20 @@ __global__ void k_lbvh_ropes(int n, NodeF<D>* nodes, const int* __restrict__ rch
   nodes[id].meta = (nodes[id].meta & 0x80000000u) | (uint32_t)(esc + 1);
 }
 
-void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* order, int n, const void* boxes, const uint8_t* act,
-                 void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits, hipStream_t s) {
+void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
+                 const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
+                 const SceneArgs& S, hipStream_t s) {
   if (n < 1) return;
   if (n > 1) fill<int>(visits, n - 1, 0, s);
   const int g = (n + 255) / 256;
   const int g2 = (2 * n - 1 + 255) / 256;
   if (dim == 3) {
-    k_lbvh_build<3><<<g, 256, 0, s>>>(keys, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild, rchild,
-                                      rng_l, rng_r, visits);
+    k_lbvh_build<3><<<g, 256, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild,
+                                      rchild, rng_l, rng_r, visits, S);
     k_lbvh_ropes<3><<<g2, 256, 0, s>>>(n, (NodeF<3>*)nodes, rchild, rng_r);
   } else {
-    k_lbvh_build<2><<<g, 256, 0, s>>>(keys, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild, rchild,
-                                      rng_l, rng_r, visits);
+    k_lbvh_build<2><<<g, 256, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild,
+                                      rchild, rng_l, rng_r, visits, S);
     k_lbvh_ropes<2><<<g2, 256, 0, s>>>(n, (NodeF<2>*)nodes, rchild, rng_r);
   }
 }

@@ -222,7 +222,7 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_act, &cap_act, (size_t)np, "active"));
   CHECK(grow((void**)&d_cbounds, &cap_cbounds, sizeof(unsigned long long) * 6, "bounds"));
   CHECK(grow((void**)&d_partial, &cap_partial, sizeof(double) * 6 * (size_t)leaf_box_blocks(np), "bounds"));
-  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, stream);
+  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, SceneArgs(), stream);
   mark(3);
   // Morton + sort
   CHECK(grow((void**)&d_keys, &cap_keys, sizeof(uint32_t) * (size_t)np, "keys"));
@@ -230,7 +230,7 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_ktmp, &cap_ktmp, sizeof(uint32_t) * (size_t)np, "keys tmp"));
   CHECK(grow((void**)&d_vtmp, &cap_vtmp, sizeof(uint32_t) * (size_t)np, "vals tmp"));
   CHECK(grow((void**)&d_hist, &cap_hist, sizeof(uint32_t) * 256 * (size_t)radix_tiles(np), "histogram"));
-  launch_morton(dim, d_boxes, np, d_cbounds, d_keys, d_order, stream);
+  launch_morton(dim, d_boxes, np, d_cbounds, SceneArgs(), d_keys, d_order, nullptr, stream);
   mark(4);
   launch_radix_sort(d_keys, d_order, d_ktmp, d_vtmp, d_hist, np, 32, stream);
   mark(5);
@@ -242,7 +242,8 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_rngl, &cap_rngl, sizeof(int) * ni, "rng_l"));
   CHECK(grow((void**)&d_rngr, &cap_rngr, sizeof(int) * ni, "rng_r"));
   CHECK(grow((void**)&d_visits, &cap_visits, sizeof(int) * ni, "visits"));
-  launch_lbvh(dim, d_keys, d_order, np, d_boxes, d_act, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, stream);
+  launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits,
+              SceneArgs(), stream);
   mark(6);
   HIPCHECK(this, hipGetLastError());
   HIPCHECK(this, hipStreamSynchronize(stream));
@@ -282,7 +283,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   for (;;) {
     mark(8);
     launch_traverse(dim, d_nodes, d_boxes, d_act, d_order, np, rank_set ? d_rank : nullptr, d_pairs, pair_cap, d_counters,
-                    stream);
+                    nullptr, nullptr, tr_waves, stream);
     mark(9);
     HIPCHECK(this, hipGetLastError());
     CHECK(read_counters());
@@ -306,7 +307,8 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       CHECK(grow((void**)&d_disc, &cap_disc, sizeof(int2) * disc_cap, "discrete"));
       launch_fill_u64(d_counters + C_CAND_VF, C_COUNT - C_CAND_VF, 0ull, stream);
       ExpandLaunch E;
-      E.pairs = d_pairs; E.npairs = npairs; E.P = P; E.flags = d_flags; E.rec = d_rec;
+      E.pairs = d_pairs; E.npairs_dev = d_counters + C_LEAF_PAIRS; E.pair_cap = pair_cap; E.npairs_hint = npairs;
+      E.P = P; E.flags = d_flags; E.rec = d_rec; E.blocks_per_cu = blocks_per_cu; E.chunk = chunk;
       E.discrete = opt.discrete; E.continuous = opt.continuous;
       E.vfq = d_vfq; E.vf_cap = vf_cap; E.eeq = d_eeq; E.ee_cap = dim == 3 ? ee_cap : 0;
       E.disc = d_disc; E.disc_cap = disc_cap; E.counters = d_counters; E.num_cus = num_cus;
@@ -341,11 +343,14 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
         launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
         AccdLaunch L;
         L.rec = d_rec; L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
-        L.counters = d_counters; L.num_cus = num_cus;
+        L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; L.chunk = chunk;
+        L.vert_scene = nullptr; L.scene_eta = nullptr;
         mark(12);
-        L.q = d_vfq; L.n = n_vf; L.eta = opt.vf_ccd_eta; L.cand_toi = keep_cand ? d_vf_toi : nullptr;
+        L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
+        L.eta = opt.vf_ccd_eta; L.cand_toi = keep_cand ? d_vf_toi : nullptr;
         launch_accd(dim, true, L, stream);
-        L.q = d_eeq; L.n = n_ee; L.eta = opt.ee_ccd_eta; L.cand_toi = keep_cand ? d_ee_toi : nullptr;
+        L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
+        L.eta = opt.ee_ccd_eta; L.cand_toi = keep_cand ? d_ee_toi : nullptr;
         if (dim == 3) launch_accd(dim, false, L, stream);
         mark(13);
         HIPCHECK(this, hipGetLastError());

@@ -10,11 +10,16 @@ namespace mclg {
 
 constexpr int kNumEvents = 16;
 
+struct BatchState;
+
 struct Ctx {
   int dim = 3;
   int device = 0;
   int num_cus = 256;
-  int blocks_per_cu = 8;
+  // tunables (the batch autotuner searches them)
+  int blocks_per_cu = 8;   // resident blocks per CU for the queue kernels
+  int chunk = 1;           // consecutive 64-item chunks per wave step
+  int tr_waves = 4;        // waves per traversal block
   mclg_options opt{};
   hipStream_t own_stream = nullptr, stream = nullptr;
   hipEvent_t ev[kNumEvents] = {};
@@ -71,6 +76,8 @@ struct Ctx {
   double* d_stage_v1 = nullptr; size_t cap_stage_v1 = 0;
   int* d_stage_p = nullptr; size_t cap_stage_p = 0;
   int* d_stage_act = nullptr; size_t cap_stage_act = 0;
+
+  BatchState* batch = nullptr;  // set for batched-scene contexts
 
   // pinned host mirrors
   unsigned long long* h_counters = nullptr;

@@ -5,6 +5,21 @@
 
 namespace mclg {
 
+// Several independent scenes solved together (all pointers null for one scene).
+// Primitives and vertices of a scene are contiguous; the tree is keyed by
+// (scene, Morton) so each scene is one subtree.
+struct SceneArgs {
+  int n = 0;                           // number of scenes
+  const int* prim_scene = nullptr;     // scene of each primitive
+  const int* vert_scene = nullptr;     // scene of each vertex
+  const int* prim_off = nullptr;       // n+1 primitive offsets
+  const double* box_eta = nullptr;     // per scene
+  const double* vf_eta = nullptr;      // per scene
+  const double* ee_eta = nullptr;      // per scene
+  unsigned long long* bounds = nullptr;  // per scene centre bounds, ordered encoding [lo(dim), hi(dim)]
+  int* root = nullptr;                 // per scene subtree root node
+};
+
 void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s);
 void launch_minmax_i32(const int* a, long long n, int* out2, hipStream_t s);
 void launch_fill_u8(uint8_t* p, long long n, uint8_t v, hipStream_t s);
@@ -14,20 +29,32 @@ void launch_reptri(const int* P, int np, int pdim, int nv, int* owner, unsigned 
                    uint32_t table_size, uint8_t* flags, hipStream_t s);
 int leaf_box_blocks(int np);
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
-                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, hipStream_t s);
-void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, uint32_t* keys,
-                   uint32_t* vals, hipStream_t s);
+                       void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, const SceneArgs& S,
+                       hipStream_t s);
+void launch_bounds_init(unsigned long long* b, int nscenes, int dim, unsigned long long lo0, unsigned long long hi0,
+                        hipStream_t s);
+void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, const SceneArgs& S,
+                   uint32_t* keys, uint32_t* vals, uint32_t* mort, hipStream_t s);
+void launch_scene_keys(const uint32_t* order, int n, const int* prim_scene, uint32_t* hi, hipStream_t s);
+void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32_t* dst, hipStream_t s);
 int radix_tiles(int n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
-void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* order, int n, const void* boxes, const uint8_t* act,
-                 void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits, hipStream_t s);
+void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
+                 const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
+                 const SceneArgs& S, hipStream_t s);
 void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
-                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters, hipStream_t s);
+                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters,
+                     const uint32_t* scene_of_sorted, const int* scene_root, int waves, hipStream_t s);
 
+// Kernels that follow a variable-size queue read its length from the device
+// counter (so a captured graph needs no host round trip); the *_hint sizes
+// only shape the grid (-1: the capacity).
 struct ExpandLaunch {
   const int2* pairs;
-  long long npairs;
+  const unsigned long long* npairs_dev;
+  unsigned long long pair_cap;
+  long long npairs_hint;
   const int* P;
   const uint8_t* flags;
   const double* rec;
@@ -40,14 +67,20 @@ struct ExpandLaunch {
   unsigned long long disc_cap;
   unsigned long long* counters;
   int num_cus;
+  int blocks_per_cu;
+  int chunk;
 };
 void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s);
 
 struct AccdLaunch {
   const int4* q;
-  long long n;
+  const unsigned long long* n_dev;
+  unsigned long long cap;
+  long long n_hint;
   const double* rec;
   double eta;
+  const int* vert_scene;
+  const double* scene_eta;
   double* cand_toi;
   int4* hit_sten;
   uint8_t* hit_vf;
@@ -55,6 +88,8 @@ struct AccdLaunch {
   unsigned long long hit_cap;
   unsigned long long* counters;
   int num_cus;
+  int blocks_per_cu;
+  int chunk;
 };
 void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
 

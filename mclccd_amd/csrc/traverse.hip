@@ -67,7 +67,7 @@ __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const Le
 
 // Pairs found by a wave are staged in LDS and written out in batches: one
 // global atomic per flush instead of one per emitting step.
-constexpr int TR_WAVES = 4, TR_BUF = 512;
+constexpr int TR_MAX_WAVES = 8, TR_BUF = 512;
 
 __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,
                                             unsigned long long* counter) {
@@ -83,14 +83,16 @@ __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __re
 }
 
 template <int D>
-__global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* __restrict__ nodes,
+__global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D>* __restrict__ nodes,
                                                                const LeafBox<D>* __restrict__ boxes,
                                                                const uint8_t* __restrict__ act,
                                                                const uint32_t* __restrict__ order, int n,
                                                                const int* __restrict__ rank, int2* __restrict__ pairs,
-                                                               unsigned long long cap, unsigned long long* counters) {
-  __shared__ int2 s_buf[TR_WAVES][TR_BUF];
-  __shared__ int s_cnt[TR_WAVES];
+                                                               unsigned long long cap, unsigned long long* counters,
+                                                               const uint32_t* __restrict__ scene_of_sorted,
+                                                               const int* __restrict__ scene_root) {
+  __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
+  __shared__ int s_cnt[TR_MAX_WAVES];
   const int w = threadIdx.x / kWave;
   const int s = xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = s < n - 1;
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* _
   LeafBox<D> qb;
   bool qa = false;
   int a = 0, ra = 0;
-  int node = -1;
+  int node = -1, end = -1;
   if (valid) {
     q = nodes[(n - 1) + s];
     a = (int)order[s];
@@ -108,6 +110,9 @@ __global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* _
     qa = act[a] != 0;
     ra = rank ? rank[a] : a;
     node = (int)(q.meta & 0x7fffffffu) - 1;
+    // several scenes: the walk ends where the pre-order leaves this leaf's scene subtree
+    if (scene_root) end = (int)(nodes[scene_root[scene_of_sorted[s]]].meta & 0x7fffffffu) - 1;
+    if (node == end) node = -1;
   }
   // a pre-order walk visits each of the 2n-1 nodes at most once: more steps
   // than that means a malformed tree, which is reported instead of looping
@@ -130,6 +135,7 @@ __global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* _
         next = nd.child;
       }
     }
+    if (next == end) next = -1;
     const uint64_t m = __ballot(emit);
     if (m) {
       const int cnt = s_cnt[w];
@@ -156,15 +162,17 @@ __global__ __launch_bounds__(TR_WAVES * kWave) void k_traverse(const NodeF<D>* _
 }
 
 void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
-                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters, hipStream_t s) {
+                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters,
+                     const uint32_t* scene_of_sorted, const int* scene_root, int waves, hipStream_t s) {
   if (n < 2) return;
-  const int g = xcd_grid(n - 1, TR_WAVES * kWave);
+  waves = waves < 1 ? 4 : (waves > TR_MAX_WAVES ? TR_MAX_WAVES : waves);
+  const int g = xcd_grid(n - 1, waves * kWave);
   if (dim == 3)
-    k_traverse<3><<<g, TR_WAVES * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)boxes, act, order, n, rank, pairs, cap,
-                                    counters);
+    k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)boxes, act, order, n, rank,
+                                                 pairs, cap, counters, scene_of_sorted, scene_root);
   else
-    k_traverse<2><<<g, TR_WAVES * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)boxes, act, order, n, rank, pairs, cap,
-                                    counters);
+    k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)boxes, act, order, n, rank,
+                                                 pairs, cap, counters, scene_of_sorted, scene_root);
 }
 
 // ---------------------------------------------------------------------------
@@ -236,7 +244,8 @@ __device__ __forceinline__ void stage_push(T* buf, int& cnt, bool pred, const T&
 // ---------------------------------------------------------------------------
 struct ExpandArgs {
   const int2* pairs;
-  long long npairs;
+  const unsigned long long* npairs_dev;  // pairs found by the traversal (device counter)
+  unsigned long long pair_cap;
   const int* P;
   const uint8_t* flags;
   const double* rec;
@@ -248,6 +257,7 @@ struct ExpandArgs {
   int2* disc;
   unsigned long long disc_cap;
   unsigned long long* counters;
+  int chunk;  // consecutive 64-item chunks a wave takes per grid-stride step
 };
 
 constexpr int EX_WAVES = 4, EX_STAGE = 256, EX_DSTAGE = 128;
@@ -258,11 +268,13 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
   const int w = threadIdx.x / kWave, lane = lane_id();
   int nvf = 0, nee = 0, nd = 0;  // staged records (wave-uniform)
-  const long long nchunks = (A.npairs + kWave - 1) / kWave;
-  const long long stride = (long long)gridDim.x * EX_WAVES;
-  for (long long ch = (long long)xcd_block() * EX_WAVES + w; ch < nchunks; ch += stride) {
+  const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
+  const long long nchunks = (npairs + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * EX_WAVES * A.chunk;
+  for (long long c0 = ((long long)xcd_block() * EX_WAVES + w) * A.chunk; c0 < nchunks; c0 += stride)
+  for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
     const long long i = ch * kWave + lane;
-    const bool valid = i < A.npairs;
+    const bool valid = i < npairs;
     int f0[3] = {-1, -1, -1}, f1[3] = {-2, -2, -2};
     int2 pr = make_int2(0, 0);
     uint32_t fa = 0, fb = 0;
@@ -364,15 +376,19 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
 // ---------------------------------------------------------------------------
 struct AccdArgs {
   const int4* q;
-  long long n;
+  const unsigned long long* n_dev;  // queue length (device counter)
+  unsigned long long cap;
   const double* rec;
   double eta;
+  const int* vert_scene;    // several scenes: per-scene gaps
+  const double* scene_eta;
   double* cand_toi;  // per-stencil toi (-1 = no hit) or null
   int4* hit_sten;
   uint8_t* hit_vf;
   double* hit_toi;
   unsigned long long hit_cap;
   unsigned long long* counters;
+  int chunk;
 };
 
 struct HitRec {
@@ -388,11 +404,13 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
   const int w = threadIdx.x / kWave, lane = lane_id();
   int nh = 0;
   long long evals = 0;
-  const long long nchunks = (A.n + kWave - 1) / kWave;
-  const long long stride = (long long)gridDim.x * AC_WAVES;
-  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
+  const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
+  const long long nchunks = (n + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * AC_WAVES * A.chunk;
+  for (long long c0 = ((long long)xcd_block() * AC_WAVES + w) * A.chunk; c0 < nchunks; c0 += stride)
+  for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
     const long long i = ch * kWave + lane;
-    const bool live = i < A.n;
+    const bool live = i < n;
     int4 sv = make_int4(0, 0, 0, 0);
     int hit = 0, ev = 0;
     double toi = -1.0;
@@ -402,7 +420,8 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
       V v0[D + 1], v1[D + 1];
 #pragma unroll
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
-      hit = accd<D, VF>(v0, v1, A.eta, toi, ev);
+      const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
+      hit = accd<D, VF>(v0, v1, eta, toi, ev);
       if (A.cand_toi) A.cand_toi[i] = hit ? toi : -1.0;
     }
     evals += ev;
@@ -452,24 +471,29 @@ static int grid_for(long long chunks, int waves, int num_cus, int per_cu) {
 }
 
 void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
-  if (L.npairs <= 0) return;
   ExpandArgs A;
-  A.pairs = L.pairs; A.npairs = L.npairs; A.P = L.P; A.flags = L.flags; A.rec = L.rec;
+  A.pairs = L.pairs; A.npairs_dev = L.npairs_dev; A.pair_cap = L.pair_cap; A.P = L.P; A.flags = L.flags; A.rec = L.rec;
   A.discrete = L.discrete; A.continuous = L.continuous;
   A.vfq = L.vfq; A.vf_cap = L.vf_cap; A.eeq = L.eeq; A.ee_cap = L.ee_cap;
   A.disc = L.disc; A.disc_cap = L.disc_cap; A.counters = L.counters;
-  const int g = grid_for((L.npairs + kWave - 1) / kWave, EX_WAVES, L.num_cus, 8);
+  A.chunk = L.chunk < 1 ? 1 : L.chunk;
+  const long long items = L.npairs_hint >= 0 ? L.npairs_hint : (long long)L.pair_cap;
+  if (items <= 0) return;
+  const int g = grid_for((items + kWave - 1) / kWave / A.chunk + 1, EX_WAVES, L.num_cus, L.blocks_per_cu);
   if (dim == 3) k_expand<3><<<g, EX_WAVES * kWave, 0, s>>>(A);
   else k_expand<2><<<g, EX_WAVES * kWave, 0, s>>>(A);
 }
 
 void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
-  if (L.n <= 0) return;
   AccdArgs A;
-  A.q = L.q; A.n = L.n; A.rec = L.rec; A.eta = L.eta; A.cand_toi = L.cand_toi;
+  A.q = L.q; A.n_dev = L.n_dev; A.cap = L.cap; A.rec = L.rec; A.eta = L.eta;
+  A.vert_scene = L.vert_scene; A.scene_eta = L.scene_eta; A.cand_toi = L.cand_toi;
   A.hit_sten = L.hit_sten; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
   A.counters = L.counters;
-  const int g = grid_for((L.n + kWave - 1) / kWave, AC_WAVES, L.num_cus, 8);
+  A.chunk = L.chunk < 1 ? 1 : L.chunk;
+  const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
+  if (items <= 0) return;
+  const int g = grid_for((items + kWave - 1) / kWave / A.chunk + 1, AC_WAVES, L.num_cus, L.blocks_per_cu);
   if (dim == 3) {
     if (vf) k_accd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
     else k_accd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A);
