@@ -105,6 +105,8 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&c->d_rngl, &c->cap_rngl, sizeof(int) * ni, "rng_l"));
   CHECK(c->grow((void**)&c->d_rngr, &c->cap_rngr, sizeof(int) * ni, "rng_r"));
   CHECK(c->grow((void**)&c->d_visits, &c->cap_visits, sizeof(int) * ni, "visits"));
+  CHECK(c->grow((void**)&c->d_frontier, &c->cap_frontier, sizeof(int4) * (size_t)lbvh_frontier_cap(np), "frontier"));
+  CHECK(c->grow((void**)&c->d_nfrontier, &c->cap_nfrontier, sizeof(unsigned long long), "frontier"));
   CHECK(c->grow((void**)&c->d_counters, &c->cap_counters, sizeof(unsigned long long) * C_COUNT, "counters"));
   if (c->pair_cap == 0) c->pair_cap = std::max<unsigned long long>(1024, 16ull * (unsigned long long)np);
   if (c->vf_cap == 0) c->vf_cap = std::max<unsigned long long>(1024, 16ull * (unsigned long long)np);
@@ -138,7 +140,7 @@ static void batch_enqueue(Ctx* c) {
   launch_radix_sort(b->d_khi, c->d_order, c->d_ktmp, c->d_vtmp, c->d_hist, np, b->scene_bits, s);
   launch_gather_u32(c->d_order, np, b->d_mort, c->d_keys, s);
   launch_lbvh(dim, c->d_keys, b->d_khi, c->d_order, np, c->d_boxes, c->d_act, c->d_nodes, c->d_lchild, c->d_rchild,
-              c->d_rngl, c->d_rngr, c->d_visits, S, s);
+              c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
   launch_traverse(dim, c->d_nodes, c->d_boxes, c->d_act, c->d_order, np, nullptr, c->d_pairs, c->pair_cap, c->d_counters,
                   b->d_khi, b->d_root, c->tr_waves, s);
   if (!(o.continuous || o.discrete)) return;

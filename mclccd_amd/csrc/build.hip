@@ -545,67 +545,133 @@ template <int D> __device__ __forceinline__ void merge_node(NodeF<D>& o, const N
   o.meta = (a.meta | b.meta) & 0x80000000u;
 }
 
-template <int D>
-__global__ __launch_bounds__(256) void k_lbvh_build(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ hi,
-                                                    const uint32_t* __restrict__ order, int n,
-                                                    const LeafBox<D>* __restrict__ boxes,
-                                                    const uint8_t* __restrict__ act, NodeF<D>* nodes, int* lchild,
-                                                    int* rchild, int* rng_l, int* rng_r, int* visits, SceneArgs S) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  // leaf node
-  {
-    const uint32_t prim = order[s];
-    const LeafBox<D> b = boxes[prim];
-    NodeF<D> L;
+// Parent of the node spanning sorted leaves [l, r] (Apetrei 2014): internal
+// node r (this node is its left child) or l-1 (right child), whichever split
+// orders lower.
+__device__ __forceinline__ bool parent_is_right(const uint32_t* keys, const uint32_t* hi, int l, int r, int n) {
+  return l == 0 || (r != n - 1 && split_key(keys, hi, r) < split_key(keys, hi, l - 1));
+}
+
+template <int D> __device__ __forceinline__ NodeF<D> leaf_node(const LeafBox<D>& b, int prim, bool active) {
+  NodeF<D> L;
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-      L.sw_lo[k] = f32_down(b.sw_lo[k]); L.sw_hi[k] = f32_up(b.sw_hi[k]);
-      L.t0_lo[k] = f32_down(b.t0_lo[k]); L.t0_hi[k] = f32_up(b.t0_hi[k]);
-      L.t1_lo[k] = f32_down(b.t1_lo[k]); L.t1_hi[k] = f32_up(b.t1_hi[k]);
-    }
-    L.child = (int)prim;
-    L.meta = act[prim] ? 0x80000000u : 0u;
+  for (int k = 0; k < D; ++k) {
+    L.sw_lo[k] = f32_down(b.sw_lo[k]); L.sw_hi[k] = f32_up(b.sw_hi[k]);
+    L.t0_lo[k] = f32_down(b.t0_lo[k]); L.t0_hi[k] = f32_up(b.t0_hi[k]);
+    L.t1_lo[k] = f32_down(b.t1_lo[k]); L.t1_hi[k] = f32_up(b.t1_hi[k]);
+  }
+  L.child = prim;
+  L.meta = active ? 0x80000000u : 0u;
+  return L;
+}
+
+__device__ __forceinline__ void note_scene_root(const SceneArgs& S, const uint32_t* hi, int l, int r, int node) {
+  if (!S.root) return;
+  const int sc = (int)hi[l];
+  if (l == S.prim_off[sc] && r == S.prim_off[sc + 1] - 1) S.root[sc] = node;
+}
+
+// Phase A: each block of LB_T threads owns sorted leaves [B, E) and builds every
+// node whose split lies in [B, E-2] and whose children it completes, with LDS
+// visit counters (workgroup scope only).  A completed node whose parent split
+// is a block boundary (B-1 or E-1) is handed to phase B.
+constexpr int LB_T = 256;
+
+template <int D>
+__global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ hi,
+                                                     const uint32_t* __restrict__ order, int n,
+                                                     const LeafBox<D>* __restrict__ boxes,
+                                                     const uint8_t* __restrict__ act, NodeF<D>* nodes, int* lchild,
+                                                     int* rchild, int* rng_l, int* rng_r, int* visits, int4* frontier,
+                                                     unsigned long long* nfrontier, SceneArgs S) {
+  __shared__ NodeF<D> s_node[2 * LB_T];  // [0, LB_T): leaves, [LB_T, 2 LB_T): internal nodes
+  __shared__ int s_vis[LB_T], s_lc[LB_T], s_rc[LB_T], s_rl[LB_T], s_rr[LB_T];
+  const int B = blockIdx.x * LB_T, E = min(B + LB_T, n);
+  const int t = threadIdx.x, s = B + t;
+  s_vis[t] = 0;
+  if (t == LB_T - 1 && s < n - 1) visits[s] = 0;  // boundary split: phase B's counter
+  if (s < n) {
+    const uint32_t prim = order[s];
+    const NodeF<D> L = leaf_node<D>(boxes[prim], (int)prim, act[prim] != 0);
     nodes[(n - 1) + s] = L;
-    if (S.root) {  // a one-primitive scene is its own subtree
+    s_node[t] = L;
+    if (S.root) {
       const int sc = (int)hi[s];
       if (S.prim_off[sc + 1] - S.prim_off[sc] == 1) S.root[sc] = (n - 1) + s;
     }
   }
-  if (n == 1) return;
-  int l = s, r = s, node = (n - 1) + s;
-  for (;;) {
-    int parent;
-    if (l == 0 || (r != n - 1 && split_key(keys, hi, r) < split_key(keys, hi, l - 1))) {
-      parent = r;
-      lchild[parent] = node;
-      rng_l[parent] = l;
-    } else {
-      parent = l - 1;
-      rchild[parent] = node;
-      rng_r[parent] = r;
+  __syncthreads();
+  if (s < n && n > 1) {
+    int l = s, r = s, node = (n - 1) + s;
+    NodeF<D> cur = s_node[t];
+    for (;;) {
+      const bool right = parent_is_right(keys, hi, l, r, n);
+      const int p = right ? r : l - 1;
+      if (p < B || p > E - 2) {  // parent split on a block boundary: phase B
+        const unsigned long long k = atomicAdd(nfrontier, 1ull);
+        frontier[k] = make_int4(node, l, r, 0);
+        break;
+      }
+      const int q = p - B;
+      if (right) { s_lc[q] = node; s_rl[q] = l; }
+      else { s_rc[q] = node; s_rr[q] = r; }
+      __threadfence_block();
+      if (atomicAdd(&s_vis[q], 1) == 0) break;  // the sibling carries on
+      __threadfence_block();
+      l = s_rl[q];
+      r = s_rr[q];
+      const int lc = s_lc[q], rc = s_rc[q];
+      const NodeF<D>& a = lc < n - 1 ? s_node[LB_T + (lc - B)] : s_node[lc - (n - 1) - B];
+      const NodeF<D>& b = rc < n - 1 ? s_node[LB_T + (rc - B)] : s_node[rc - (n - 1) - B];
+      merge_node<D>(cur, a, b);
+      cur.child = lc;
+      s_node[LB_T + q] = cur;
+      nodes[p] = cur;
+      node = p;
+      note_scene_root(S, hi, l, r, node);
+      if (l == 0 && r == n - 1) break;  // root of the whole tree
     }
+  }
+  __syncthreads();
+  // publish this block's partial nodes for phase B
+  if (s < E - 1) {
+    visits[s] = s_vis[t];
+    lchild[s] = s_lc[t]; rng_l[s] = s_rl[t];
+    rchild[s] = s_rc[t]; rng_r[s] = s_rr[t];
+  }
+}
+
+// Phase B: the frontier nodes climb the block-crossing part of the tree with
+// device-scope release/acquire (one pair per crossing node arrival).
+template <int D>
+__global__ __launch_bounds__(256) void k_lbvh_cross(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ hi,
+                                                    int n, NodeF<D>* nodes, int* lchild, int* rchild, int* rng_l,
+                                                    int* rng_r, int* visits, const int4* __restrict__ frontier,
+                                                    const unsigned long long* nfrontier, SceneArgs S) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= (long long)*nfrontier) return;
+  const int4 f = frontier[k];
+  int node = f.x, l = f.y, r = f.z;
+  for (;;) {
+    const bool right = parent_is_right(keys, hi, l, r, n);
+    const int p = right ? r : l - 1;
+    if (right) { lchild[p] = node; rng_l[p] = l; }
+    else { rchild[p] = node; rng_r[p] = r; }
     __threadfence();
-    const int prev = __hip_atomic_fetch_add(visits + parent, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (prev == 0) return;  // sibling not finished: it will carry on upwards
+    if (__hip_atomic_fetch_add(visits + p, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
     __threadfence();
-    l = __hip_atomic_load(rng_l + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    r = __hip_atomic_load(rng_r + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int lc = __hip_atomic_load(lchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int rc = __hip_atomic_load(rchild + parent, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    // the children were written by other threads before their release
-    // fence + atomic; the acquire side above makes plain loads see them
+    l = __hip_atomic_load(rng_l + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    r = __hip_atomic_load(rng_r + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int lc = __hip_atomic_load(lchild + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int rc = __hip_atomic_load(rchild + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const NodeF<D> a = nodes[lc], b = nodes[rc];
     NodeF<D> o;
     merge_node<D>(o, a, b);
     o.child = lc;
-    nodes[parent] = o;
-    node = parent;
-    if (S.root) {  // the node spanning exactly one scene's leaves is that scene's subtree
-      const int sc = (int)hi[l];
-      if (l == S.prim_off[sc] && r == S.prim_off[sc + 1] - 1) S.root[sc] = parent;
-    }
-    if (l == 0 && r == n - 1) return;  // root
+    nodes[p] = o;
+    node = p;
+    note_scene_root(S, hi, l, r, node);
+    if (l == 0 && r == n - 1) return;
   }
 }
 
@@ -619,20 +685,27 @@ __global__ void k_lbvh_ropes(int n, NodeF<D>* nodes, const int* __restrict__ rch
   nodes[id].meta = (nodes[id].meta & 0x80000000u) | (uint32_t)(esc + 1);
 }
 
+int lbvh_frontier_cap(int n) { return 2 * ((n + LB_T - 1) / LB_T) + 2; }
+
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
                  const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
-                 const SceneArgs& S, hipStream_t s) {
+                 int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s) {
   if (n < 1) return;
-  if (n > 1) fill<int>(visits, n - 1, 0, s);
-  const int g = (n + 255) / 256;
+  fill<unsigned long long>(nfrontier, 1, 0ull, s);
+  const int g = (n + LB_T - 1) / LB_T;
+  const int gf = (lbvh_frontier_cap(n) + 255) / 256;
   const int g2 = (2 * n - 1 + 255) / 256;
   if (dim == 3) {
-    k_lbvh_build<3><<<g, 256, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild,
-                                      rchild, rng_l, rng_r, visits, S);
+    k_lbvh_local<3><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild,
+                                       rchild, rng_l, rng_r, visits, frontier, nfrontier, S);
+    if (n > 1) k_lbvh_cross<3><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<3>*)nodes, lchild, rchild, rng_l, rng_r, visits,
+                                                  frontier, nfrontier, S);
     k_lbvh_ropes<3><<<g2, 256, 0, s>>>(n, (NodeF<3>*)nodes, rchild, rng_r);
   } else {
-    k_lbvh_build<2><<<g, 256, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild,
-                                      rchild, rng_l, rng_r, visits, S);
+    k_lbvh_local<2><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild,
+                                       rchild, rng_l, rng_r, visits, frontier, nfrontier, S);
+    if (n > 1) k_lbvh_cross<2><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<2>*)nodes, lchild, rchild, rng_l, rng_r, visits,
+                                                  frontier, nfrontier, S);
     k_lbvh_ropes<2><<<g2, 256, 0, s>>>(n, (NodeF<2>*)nodes, rchild, rng_r);
   }
 }

@@ -242,8 +242,10 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_rngl, &cap_rngl, sizeof(int) * ni, "rng_l"));
   CHECK(grow((void**)&d_rngr, &cap_rngr, sizeof(int) * ni, "rng_r"));
   CHECK(grow((void**)&d_visits, &cap_visits, sizeof(int) * ni, "visits"));
+  CHECK(grow((void**)&d_frontier, &cap_frontier, sizeof(int4) * (size_t)lbvh_frontier_cap(np), "frontier"));
+  CHECK(grow((void**)&d_nfrontier, &cap_nfrontier, sizeof(unsigned long long), "frontier"));
   launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits,
-              SceneArgs(), stream);
+              d_frontier, d_nfrontier, SceneArgs(), stream);
   mark(6);
   HIPCHECK(this, hipGetLastError());
   HIPCHECK(this, hipStreamSynchronize(stream));
@@ -384,7 +386,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
-                  d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_counters, d_pairs,
+                  d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)

@@ -55,6 +55,8 @@ struct Ctx {
   int* d_rngl = nullptr; size_t cap_rngl = 0;
   int* d_rngr = nullptr; size_t cap_rngr = 0;
   int* d_visits = nullptr; size_t cap_visits = 0;
+  int4* d_frontier = nullptr; size_t cap_frontier = 0;
+  unsigned long long* d_nfrontier = nullptr; size_t cap_nfrontier = 0;
   unsigned long long* d_counters = nullptr; size_t cap_counters = 0;
   int2* d_pairs = nullptr; size_t cap_pairs_bytes = 0;
   unsigned long long pair_cap = 0;

@@ -40,9 +40,10 @@ void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32
 int radix_tiles(int n);
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
+int lbvh_frontier_cap(int n);
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
                  const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
-                 const SceneArgs& S, hipStream_t s);
+                 int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s);
 void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
                      const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters,
                      const uint32_t* scene_of_sorted, const int* scene_root, int waves, hipStream_t s);
