@@ -161,6 +161,14 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
 #define MCLG_NUM_TIMINGS 10
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
 
+/* TOI adjoint.  No reference counterpart.  For the CCD hits of the last
+ * traverse (in the order mclg_copy_hits returns them) and per-hit weights w,
+ * writes dL/dV0 and dL/dV1 (nv x dim each) of L = sum_i w_i toi_i, where toi_i
+ * is the ACCD time of impact as computed by NarrowPhase.cpp:634-711 (its
+ * derivative wherever it is smooth; hits already within the gap have toi 0 and
+ * contribute nothing).  Host pointers, or device pointers when device_ptrs. */
+int mclg_toi_grad(mclg_ctx* ctx, const double* w, double* dV0, double* dV1, int device_ptrs);
+
 /* ------------------------------------------------------------------------
  * Batched scenes.  No reference counterpart: the reference runs one
  * mcl::BVHTree per scene, one after the other.  Here N independent scenes

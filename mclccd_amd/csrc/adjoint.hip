@@ -1,0 +1,344 @@
+// TOI adjoint (no reference counterpart): gradient of L = sum_i w_i toi_i over
+// the CCD hits of the last traverse with respect to V0 and V1.
+//
+// Forward model (NarrowPhase.cpp:634-711 with s = 1): with centred
+// displacements p, c = 0.9 / l_p and phi(d) = (d^2 - xi^2)/(d + xi) (phi' = 1),
+//   toi = sum_{k=1..K} tau_k,  tau_k = c phi(d_{k-1}),
+//   d_j = D(x0 + S_j p),  S_j = tau_1 + ... + tau_j  (S_0 = 0),
+// K being the number of steps added before the loop stopped (K = 1 for the
+// common hit that breaks at iteration 2).  Reverse sweep: the adjoint of
+// tau_K is w and G_j = G_{j+1} (1 + c grad D(x_j) . p); each d_j receives
+// G_{j+1} c, which flows to x0 (directly) and to p (through S_j); l_p receives
+// -sum_k G_k tau_k / l_p and passes it to the two maximising displacements.
+// D's gradient is taken at the closest points the forward distance computes
+// (the reference's branches, including its point_on_triangle quirk region),
+// so the result is the derivative of the TOI wherever it is smooth.
+#include <algorithm>
+
+#include "../../include/mclg.h"
+#include "context.hpp"
+#include "geom.hpp"
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace mclg {
+
+__device__ __forceinline__ d3 add(const d3& a, const d3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 scl(const d3& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ d2 add(const d2& a, const d2& b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ d2 scl(const d2& a, double s) { return {a.x * s, a.y * s}; }
+
+// Distance gradients.  Every closest-point branch of the reference yields the
+// minimiser (s,t) of the distance over the primitive (or over the edge it
+// clamps to), so the derivative is the partial at fixed (s,t) (envelope
+// theorem): no chain through the projection's ratios, whose rounding noise
+// would be amplified near degenerate (parallel, coplanar) configurations.
+// The exception is the reference's point_on_triangle quirk branch
+// (Projection.hpp:131: s = clamp(-e/c), t = 0 for s >= 0, t < 0,
+// s + t >= det, a + d <= b + e), whose s is not a minimiser and is
+// differentiated through.
+
+// VF distance and its gradient (scaled by seed) w.r.t. q, a, b, c.
+__device__ double dist_vf_grad(const d3& q, const d3& a, const d3& b, const d3& c, double seed, d3& gq, d3& ga,
+                               d3& gb, d3& gc) {
+  const d3 e0 = sub(b, a), e1 = sub(c, a), w = sub(a, q);
+  const double A = dot(e0, e0), B = dot(e0, e1), C = dot(e1, e1);
+  const double Dd = dot(e0, w), E = dot(e1, w);
+  const double det = A * C - B * B;
+  const double s0 = B * E - C * Dd, t0 = B * Dd - A * E;
+  // st: 0 s=clamp(r),t=0   1 s=0,t=clamp(r)   2 s=s0 r,t=t0 r   3 s=clamp(r),t=1-s
+  int st;
+  bool quirk = false;
+  double num, den;
+  if (s0 + t0 < det) {
+    if (s0 < 0) {
+      if (t0 < 0 && Dd < 0) { st = 0; num = -Dd; den = A; }
+      else { st = 1; num = -E; den = C; }
+    } else if (t0 < 0) { st = 0; num = -Dd; den = A; }
+    else { st = 2; num = 1.0; den = det; }
+  } else {
+    const double ee = A - 2.0 * B + C;
+    if (s0 < 0) {
+      if (C + E > B + Dd) { st = 3; num = C + E - B - Dd; den = ee; }
+      else { st = 1; num = -E; den = C; }
+    } else if (t0 < 0) {
+      if (A + Dd > B + E) { st = 3; num = C + E - B - Dd; den = ee; }
+      else { st = 0; num = -E; den = C; quirk = true; }
+    } else { st = 3; num = C + E - B - Dd; den = ee; }
+  }
+  const double r = num / den;
+  const double rc = r < 0 ? 0.0 : (r > 1 ? 1.0 : r);
+  double s, t;
+  if (st == 0) { s = rc; t = 0.0; }
+  else if (st == 1) { s = 0.0; t = rc; }
+  else if (st == 2) { s = s0 * r; t = t0 * r; }
+  else { s = rc; t = 1.0 - s; }
+  const d3 pt = add(add(a, scl(e0, s)), scl(e1, t));
+  const d3 v = sub(q, pt);
+  const double d = __builtin_sqrt(dot(v, v));
+  if (!(d > 0.0)) return d;
+  const d3 vb = scl(v, seed / d);  // d|v|/dv
+  // pt = (1-s-t) a + s b + t c at fixed (s,t)
+  gq = add(gq, vb);
+  ga = sub(ga, scl(vb, 1.0 - s - t));
+  gb = sub(gb, scl(vb, s));
+  gc = sub(gc, scl(vb, t));
+  if (quirk && r >= 0.0 && r <= 1.0) {
+    // s = -E/C with E = e1.(a-q), C = e1.e1: add (dd/ds) ds/dx
+    const double sb = -dot(e0, vb);  // dd/ds
+    const double Eb = -sb / C, Cb = sb * E / (C * C);
+    const d3 e1b = add(scl(e1, 2.0 * Cb), scl(w, Eb));
+    const d3 wb = scl(e1, Eb);
+    gc = add(gc, e1b);
+    ga = add(ga, sub(wb, e1b));
+    gq = sub(gq, wb);
+  }
+  return d;
+}
+
+// EE distance and its gradient w.r.t. p0, p1, q0, q1.
+__device__ double dist_ee_grad(const d3& p0, const d3& p1, const d3& q0, const d3& q1, double seed, d3& gp0, d3& gp1,
+                               d3& gq0, d3& gq1) {
+  const d3 d1 = sub(p1, p0), d2 = sub(q1, q0), r = sub(p0, q0);
+  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const double c = dot(d1, r), b = dot(d1, d2);
+  const double denom = a * e - b * b;
+  double s = 0.0, t;
+  if (denom != 0.0) s = clamp01((b * f - c * e) / denom);
+  const double tnom = b * s + f;
+  if (tnom < 0 || e == 0) { t = 0.0; s = a == 0 ? 0.0 : clamp01(-c / a); }
+  else if (tnom > e) { t = 1.0; s = a == 0 ? 0.0 : clamp01((b - c) / a); }
+  else { t = tnom / e; }
+  const d3 v = sub(add(q0, scl(d2, t)), add(p0, scl(d1, s)));
+  const double dist = __builtin_sqrt(dot(v, v));
+  if (!(dist > 0.0)) return dist;
+  const d3 vb = scl(v, seed / dist);
+  gq0 = add(gq0, scl(vb, 1.0 - t));
+  gq1 = add(gq1, scl(vb, t));
+  gp0 = sub(gp0, scl(vb, 1.0 - s));
+  gp1 = sub(gp1, scl(vb, s));
+  return dist;
+}
+
+// 2D point-edge distance and its gradient w.r.t. p, e0, e1.
+__device__ double dist_pe2_grad(const d2& p, const d2& e0, const d2& e1, double seed, d2& gp, d2& ge0, d2& ge1) {
+  const d2 e = sub(e1, e0);
+  const double len2 = dot(e, e);
+  double t = 0.0;  // closest point e0 + t e
+  d2 pt = e0;
+  if (!(len2 <= 0.0)) {
+    const d2 pe = sub(p, e0);
+    const double u = dot(pe, e) / len2;
+    if (u > 1.0) { t = 1.0; pt = e1; }
+    else if (!(u < 0.0)) { t = u; pt = {e0.x + u * e.x, e0.y + u * e.y}; }
+  }
+  const d2 v = sub(p, pt);
+  const double d = __builtin_sqrt(dot(v, v));
+  if (!(d > 0.0)) return d;
+  const d2 vb = scl(v, seed / d);
+  gp = add(gp, vb);
+  ge0 = sub(ge0, scl(vb, 1.0 - t));
+  ge1 = sub(ge1, scl(vb, t));
+  return d;
+}
+
+template <int D> struct AdjV;
+template <> struct AdjV<3> { using V = d3; };
+template <> struct AdjV<2> { using V = d2; };
+
+template <int D, bool VF>
+__device__ __forceinline__ double dist_g(const typename AdjV<D>::V* x, double seed, typename AdjV<D>::V* g) {
+  if constexpr (D == 3) {
+    if (VF) return dist_vf_grad(x[0], x[1], x[2], x[3], seed, g[0], g[1], g[2], g[3]);
+    return dist_ee_grad(x[0], x[1], x[2], x[3], seed, g[0], g[1], g[2], g[3]);
+  } else {
+    return dist_pe2_grad(x[0], x[1], x[2], seed, g[0], g[1], g[2]);
+  }
+}
+
+constexpr int ADJ_STORE = 8;
+
+// One hit: accumulates w * d toi / d(V0, V1) into gV0 / gV1 (atomics).
+template <int D, bool VF>
+__device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double w, double* gV0, double* gV1) {
+  using V = typename AdjV<D>::V;
+  constexpr int ns = D + 1;
+  V v0[ns], v1[ns];
+#pragma unroll
+  for (int i = 0; i < ns; ++i) VLoad<D>::load(rec, sv[i], v0[i], v1[i]);
+  // setup (centred displacements, l_p and its maximisers)
+  Stencil<D, VF> st;
+  bool fin;
+  const double l_p = st.setup(v0, v1, fin);
+  if (!(l_p > 0)) return;
+  int i_first = -1, i_second = -1;
+  {
+    double lf = 0.0, ls = 0.0;
+#pragma unroll
+    for (int i = 0; i < ns; ++i) {
+      const double n = __builtin_sqrt(dot(st.p[i], st.p[i]));
+      const bool first = VF ? (i == 0) : (i <= 1);
+      if (first) { if (lf < n) { lf = n; i_first = i; } }
+      else if (ls < n) { ls = n; i_second = i; }
+    }
+  }
+  const double d0 = st.dist();
+  if (d0 <= xsi) return;  // toi = 0 whatever the inputs nearby
+  const double c = 0.9 / l_p;
+  // forward: count the steps K summed into toi, keep S_j (j < ADJ_STORE)
+  double S[ADJ_STORE];
+  S[0] = 0.0;
+  const double g = (d0 * d0 - xsi * xsi) / (d0 - xsi);
+  double toi = 0.0, tl = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+  int K = 0;  // a loop that reaches max_iter sums steps 1..999 (K = 999)
+  for (int iter = 1; iter < 1000; ++iter) {
+    const bool can_break = toi > 0.0;
+    if (!can_break && toi + tl > 1.0) return;
+    st.step(tl);
+    const double d = st.dist();
+    if (can_break && (d * d - xsi * xsi) / (d + xsi) < g) break;
+    toi += tl;
+    ++K;
+    if (K < ADJ_STORE) S[K] = S[K - 1] + tl;
+    if (toi > 1.0) return;
+    tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  }
+  // reverse sweep
+  V gx0[ns], gp[ns];
+#pragma unroll
+  for (int i = 0; i < ns; ++i) { gx0[i] = {}; gp[i] = {}; }
+  double G = w, glp = 0.0;
+  for (int j = K - 1; j >= 0; --j) {
+    double Sj;
+    if (j < ADJ_STORE) Sj = S[j];
+    else {  // long tails: replay the steps up to j
+      Stencil<D, VF> f2;
+      bool fin2;
+      (void)f2.setup(v0, v1, fin2);
+      double tt = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+      Sj = 0.0;
+      for (int m = 1; m <= j; ++m) {
+        Sj += tt;
+        f2.step(tt);
+        const double dm = f2.dist();
+        tt = 0.9 * (dm * dm - xsi * xsi) / ((dm + xsi) * l_p);
+      }
+    }
+    V xj[ns], gd[ns];
+#pragma unroll
+    for (int i = 0; i < ns; ++i) {
+      xj[i] = add(v0[i], scl(st.p[i], Sj));
+      gd[i] = {};
+    }
+    const double dj = dist_g<D, VF>(xj, 1.0, gd);
+    const double tau = c * ((dj * dj - xsi * xsi) / (dj + xsi));
+    const double dbar = G * c;
+    double gdotp = 0.0;
+#pragma unroll
+    for (int i = 0; i < ns; ++i) {
+      gx0[i] = add(gx0[i], scl(gd[i], dbar));
+      gp[i] = add(gp[i], scl(gd[i], dbar * Sj));
+      gdotp += dot(gd[i], st.p[i]);
+    }
+    glp -= G * tau / l_p;
+    if (j >= 1) G = G * (1.0 + c * gdotp);
+  }
+  // l_p = |p_first| + |p_second| at the maximisers
+  if (i_first >= 0) {
+    const double n = __builtin_sqrt(dot(st.p[i_first], st.p[i_first]));
+    if (n > 0) gp[i_first] = add(gp[i_first], scl(st.p[i_first], glp / n));
+  }
+  if (i_second >= 0) {
+    const double n = __builtin_sqrt(dot(st.p[i_second], st.p[i_second]));
+    if (n > 0) gp[i_second] = add(gp[i_second], scl(st.p[i_second], glp / n));
+  }
+  // centring p_i = P_i - mean(P): gP_i = gp_i - mean(gp); V1 = V0 + P
+  V mean = {};
+#pragma unroll
+  for (int i = 0; i < ns; ++i) mean = add(mean, gp[i]);
+  mean = scl(mean, 1.0 / ns);
+#pragma unroll
+  for (int i = 0; i < ns; ++i) {
+    const V gP = sub(gp[i], mean);
+    const V g0 = sub(gx0[i], gP);
+    double* o0 = gV0 + (size_t)sv[i] * D;
+    double* o1 = gV1 + (size_t)sv[i] * D;
+    atomicAdd(o0 + 0, g0.x);
+    atomicAdd(o0 + 1, g0.y);
+    atomicAdd(o1 + 0, gP.x);
+    atomicAdd(o1 + 1, gP.y);
+    if constexpr (D == 3) {
+      atomicAdd(o0 + 2, g0.z);
+      atomicAdd(o1 + 2, gP.z);
+    }
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_toi_adjoint(const int4* __restrict__ sten, const uint8_t* __restrict__ is_vf,
+                                                     const double* __restrict__ w, long long nh,
+                                                     const double* __restrict__ rec, double vf_eta, double ee_eta,
+                                                     double* gV0, double* gV1) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= nh) return;
+  const int4 s4 = sten[i];
+  const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+  const double wi = w[i];
+  if (wi == 0.0) return;
+  if constexpr (D == 3) {
+    if (is_vf[i]) toi_adjoint<3, true>(sv, rec, vf_eta, wi, gV0, gV1);
+    else toi_adjoint<3, false>(sv, rec, ee_eta, wi, gV0, gV1);
+  } else {
+    toi_adjoint<2, true>(sv, rec, vf_eta, wi, gV0, gV1);
+  }
+}
+
+}  // namespace mclg
+
+using namespace mclg;
+
+#define CHECK(x)                     \
+  do {                               \
+    int _rc = (x);                   \
+    if (_rc != MCLG_OK) return _rc;  \
+  } while (0)
+#define HIPCHECK(ctx, call) CHECK((ctx)->hip((call), #call))
+
+extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* dV1, int device_ptrs) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !w || !dV0 || !dV1) return MCLG_ERR_ARG;
+  const long long nh = c->counts.hits;
+  const size_t gbytes = sizeof(double) * c->dim * (size_t)c->nv;
+  double *gw = nullptr, *g0 = nullptr, *g1 = nullptr;
+  if (device_ptrs) {
+    gw = const_cast<double*>(w);
+    g0 = dV0;
+    g1 = dV1;
+  } else {
+    CHECK(c->grow((void**)&c->d_adj_w, &c->cap_adj_w, sizeof(double) * (size_t)std::max(nh, 1LL), "adjoint"));
+    CHECK(c->grow((void**)&c->d_adj_g, &c->cap_adj_g, 2 * gbytes, "adjoint"));
+    gw = c->d_adj_w;
+    g0 = c->d_adj_g;
+    g1 = c->d_adj_g + (size_t)c->dim * c->nv;
+    if (nh > 0) HIPCHECK(c, hipMemcpyAsync(gw, w, sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
+  }
+  launch_fill_u64(reinterpret_cast<unsigned long long*>(g0), (long long)c->dim * c->nv, 0ull, c->stream);
+  launch_fill_u64(reinterpret_cast<unsigned long long*>(g1), (long long)c->dim * c->nv, 0ull, c->stream);
+  if (nh > 0) {
+    const int g = (int)((nh + 255) / 256);
+    if (c->dim == 3)
+      k_toi_adjoint<3><<<g, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, nh, c->d_rec, c->opt.vf_ccd_eta,
+                                                 c->opt.ee_ccd_eta, g0, g1);
+    else
+      k_toi_adjoint<2><<<g, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, nh, c->d_rec, c->opt.vf_ccd_eta,
+                                                 c->opt.ee_ccd_eta, g0, g1);
+    HIPCHECK(c, hipGetLastError());
+  }
+  if (!device_ptrs) {
+    HIPCHECK(c, hipMemcpyAsync(dV0, g0, gbytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipMemcpyAsync(dV1, g1, gbytes, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  return MCLG_OK;
+}

@@ -74,6 +74,8 @@ struct Ctx {
   long long n_vf = 0, n_ee = 0;
   int* d_rank = nullptr; size_t cap_rank = 0;
   int* d_minmax = nullptr; size_t cap_minmax = 0;
+  double* d_adj_w = nullptr; size_t cap_adj_w = 0;
+  double* d_adj_g = nullptr; size_t cap_adj_g = 0;
   double* d_stage_v0 = nullptr; size_t cap_stage_v0 = 0;
   double* d_stage_v1 = nullptr; size_t cap_stage_v1 = 0;
   int* d_stage_p = nullptr; size_t cap_stage_p = 0;

@@ -178,35 +178,6 @@ void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_
 // ---------------------------------------------------------------------------
 // Narrow phase
 // ---------------------------------------------------------------------------
-template <int D> struct VLoad;
-template <> struct VLoad<3> {
-  using V = d3;
-  __device__ __forceinline__ static void load(const double* rec, int v, d3& a, d3& b) {
-    const double2* r = reinterpret_cast<const double2*>(rec + (size_t)v * 8);
-    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-    a = {r0.x, r0.y, r1.x};
-    b = {r2.x, r2.y, r3.x};
-  }
-  __device__ __forceinline__ static d3 load_t1(const double* rec, int v) {
-    const double2* r = reinterpret_cast<const double2*>(rec + (size_t)v * 8);
-    const double2 r2 = r[2], r3 = r[3];
-    return {r2.x, r2.y, r3.x};
-  }
-};
-template <> struct VLoad<2> {
-  using V = d2;
-  __device__ __forceinline__ static void load(const double* rec, int v, d2& a, d2& b) {
-    const double2* r = reinterpret_cast<const double2*>(rec + (size_t)v * 4);
-    const double2 r0 = r[0], r1 = r[1];
-    a = {r0.x, r0.y};
-    b = {r1.x, r1.y};
-  }
-  __device__ __forceinline__ static d2 load_t1(const double* rec, int v) {
-    const double2 r1 = reinterpret_cast<const double2*>(rec + (size_t)v * 4)[1];
-    return {r1.x, r1.y};
-  }
-};
-
 // ---------------------------------------------------------------------------
 // Per-wave LDS staging of output records: lanes deposit records at
 // ballot-computed slots; a full buffer is written out with one global atomic.

@@ -104,6 +104,7 @@ class BVHTree:
         self._h = h
         self.nv = 0
         self.np = 0
+        self.serial = 0  # counts traversals (the adjoint refers to the last one)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -186,6 +187,7 @@ class BVHTree:
     def run(self, V0, V1, P, keep_candidates=False, keep_leaf_pairs=False):
         """GPU traverse only; results stay on the device (see hits(), discrete())."""
         self._push_options()
+        self.serial += 1
         flags = (_lib.KEEP_CANDIDATES if keep_candidates else 0) | (_lib.KEEP_LEAF_PAIRS if keep_leaf_pairs else 0)
         if _is_device(V0):
             V0, V1 = _dev_check(V0, "torch.float64"), _dev_check(V1, "torch.float64")

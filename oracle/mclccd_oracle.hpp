@@ -61,15 +61,26 @@ namespace mclo {
 // ---------------------------------------------------------------------------
 // Small vector helpers in the Eigen 3.4 fixed-size evaluation order.
 // ---------------------------------------------------------------------------
-template <int D> inline double vdot(const double* a, const double* b);
-template <> inline double vdot<3>(const double* a, const double* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
-template <> inline double vdot<2>(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1]; }
-template <int D> inline double vnorm(const double* a) { return std::sqrt(vdot<D>(a, a)); }
+// The ACCD path below is written over a scalar type T: double for the checker
+// itself, a forward-mode dual number (oracle/dual.hpp) for the reference TOI
+// gradient.  With T = double every expression is the plain FP64 one.
+inline double osqrt(double x) { return std::sqrt(x); }
+
+template <int D, typename T = double> inline T vdot(const T* a, const T* b) {
+  if constexpr (D == 3) return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+  else return a[0] * b[0] + a[1] * b[1];
+}
+template <int D, typename T = double> inline T vnorm(const T* a) { return osqrt(vdot<D, T>(a, a)); }
 
 // std::max / std::min as libstdc++ defines them: max(a,b) = a<b ? b : a,
 // min(a,b) = b<a ? b : a.  The NaN and signed-zero behaviour matters.
-inline double smax(double a, double b) { return (a < b) ? b : a; }
-inline double smin(double a, double b) { return (b < a) ? b : a; }
+template <typename T> inline T smax(const T& a, const T& b) { return (a < b) ? b : a; }
+// Held<T>::of(x) is x for plain arithmetic; a derivative-carrying scalar type
+// specialises it to drop x's derivatives (x treated as a constant).
+template <typename T> struct Held {
+  static T of(const T& x) { return x; }
+};
+template <typename T> inline T smin(const T& a, const T& b) { return (b < a) ? b : a; }
 
 // ---------------------------------------------------------------------------
 // Eigen::AlignedBox semantics (Eigen 3.4): empty = [+max, lowest]; extend is
@@ -131,54 +142,62 @@ template <int D> inline bool boxes_intersect(const BBox<D>& l, const BBox<D>& r)
 // Projections (Projection.hpp).
 // ---------------------------------------------------------------------------
 // Projection.hpp:60-145 — closest point of triangle (a,b,c) to q.
-inline void point_on_triangle(const double* q, const double* a, const double* b, const double* c, double* out) {
-  auto clamp01 = [](double v) { return v < 0 ? 0.0 : (v > 1 ? 1.0 : v); };
-  double e0[3], e1[3], w[3];
+template <typename T = double>
+inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T* out) {
+  auto clamp01 = [](const T& v) { return v < 0.0 ? T(0.0) : (v > 1.0 ? T(1.0) : v); };
+  T e0[3], e1[3], w[3];
   for (int k = 0; k < 3; ++k) { e0[k] = b[k] - a[k]; e1[k] = c[k] - a[k]; w[k] = a[k] - q[k]; }
-  const double A = vdot<3>(e0, e0), B = vdot<3>(e0, e1), C = vdot<3>(e1, e1);
-  const double Dd = vdot<3>(e0, w), E = vdot<3>(e1, w);
-  const double det = A * C - B * B;
-  double s = B * E - C * Dd;
-  double t = B * Dd - A * E;
+  const T A = vdot<3, T>(e0, e0), B = vdot<3, T>(e0, e1), C = vdot<3, T>(e1, e1);
+  const T Dd = vdot<3, T>(e0, w), E = vdot<3, T>(e1, w);
+  const T det = A * C - B * B;
+  T s = B * E - C * Dd;
+  T t = B * Dd - A * E;
+  bool quirk = false;  // Projection.hpp:131 (s >= 0, t < 0, s + t >= det, a + d <= b + e)
   if (s + t < det) {
-    if (s < 0) {
-      if (t < 0) {
-        if (Dd < 0) { s = clamp01(-Dd / A); t = 0; }
-        else { s = 0; t = clamp01(-E / C); }
-      } else { s = 0; t = clamp01(-E / C); }
-    } else if (t < 0) {
-      s = clamp01(-Dd / A); t = 0;
+    if (s < 0.0) {
+      if (t < 0.0) {
+        if (Dd < 0.0) { s = clamp01(-Dd / A); t = 0.0; }
+        else { s = 0.0; t = clamp01(-E / C); }
+      } else { s = 0.0; t = clamp01(-E / C); }
+    } else if (t < 0.0) {
+      s = clamp01(-Dd / A); t = 0.0;
     } else {
-      const double inv = 1.0 / det;
-      s *= inv; t *= inv;
+      const T inv = T(1.0) / det;
+      s = s * inv; t = t * inv;
     }
   } else {
-    if (s < 0) {
-      const double t0 = B + Dd, t1 = C + E;
+    if (s < 0.0) {
+      const T t0 = B + Dd, t1 = C + E;
       if (t1 > t0) {
-        const double num = t1 - t0, den = A - 2.0 * B + C;
-        s = clamp01(num / den); t = 1.0 - s;
-      } else { t = clamp01(-E / C); s = 0; }
-    } else if (t < 0) {
+        const T num = t1 - t0, den = A - 2.0 * B + C;
+        s = clamp01(num / den); t = T(1.0) - s;
+      } else { t = clamp01(-E / C); s = 0.0; }
+    } else if (t < 0.0) {
       if (A + Dd > B + E) {
-        const double num = C + E - B - Dd, den = A - 2.0 * B + C;
-        s = clamp01(num / den); t = 1.0 - s;
-      } else { s = clamp01(-E / C); t = 0; }
+        const T num = C + E - B - Dd, den = A - 2.0 * B + C;
+        s = clamp01(num / den); t = T(1.0) - s;
+      } else { s = clamp01(-E / C); t = 0.0; quirk = true; }
     } else {
-      const double num = C + E - B - Dd, den = A - 2.0 * B + C;
-      s = clamp01(num / den); t = 1.0 - s;
+      const T num = C + E - B - Dd, den = A - 2.0 * B + C;
+      s = clamp01(num / den); t = T(1.0) - s;
     }
   }
+  // Every branch but the quirk one yields the minimiser (s,t) of |q - x(s,t)|
+  // over the triangle (or over the edge it clamps to), where the distance's
+  // derivative is the partial at fixed (s,t) (envelope theorem).  Holding
+  // them keeps rounding noise out of the derivative; the quirk branch's s is
+  // not a minimiser and keeps its derivative.
+  if (!quirk) { s = Held<T>::of(s); t = Held<T>::of(t); }
   for (int k = 0; k < 3; ++k) out[k] = (a[k] + e0[k] * s) + e1[k] * t;
 }
 
 // Projection.hpp:193-204 — closest point of segment (e0,e1) to p (2D).
-inline void point_on_edge2(const double* p, const double* e0, const double* e1, double* out) {
-  double e[2] = {e1[0] - e0[0], e1[1] - e0[1]};
-  const double len2 = vdot<2>(e, e);
+template <typename T = double> inline void point_on_edge2(const T* p, const T* e0, const T* e1, T* out) {
+  T e[2] = {e1[0] - e0[0], e1[1] - e0[1]};
+  const T len2 = vdot<2, T>(e, e);
   if (len2 <= 0.0) { out[0] = e0[0]; out[1] = e0[1]; return; }
-  double pe[2] = {p[0] - e0[0], p[1] - e0[1]};
-  const double t = vdot<2>(pe, e) / len2;
+  T pe[2] = {p[0] - e0[0], p[1] - e0[1]};
+  const T t = Held<T>::of(vdot<2, T>(pe, e) / len2);  // minimiser: derivative at fixed t
   if (t < 0.0) { out[0] = e0[0]; out[1] = e0[1]; return; }
   if (t > 1.0) { out[0] = e1[0]; out[1] = e1[1]; return; }
   out[0] = e0[0] + t * e[0];
@@ -186,52 +205,55 @@ inline void point_on_edge2(const double* p, const double* e0, const double* e1, 
 }
 
 // Projection.hpp:241-302 — shortest vector from segment p to segment q.
-inline void edge_to_edge(const double* p0, const double* p1, const double* q0, const double* q1, double* out,
-                         double* bary = nullptr) {
-  auto clamp01 = [](double x) { return smin(1.0, smax(x, 0.0)); };
-  double d1[3], d2[3], r[3];
+template <typename T = double>
+inline void edge_to_edge(const T* p0, const T* p1, const T* q0, const T* q1, T* out, T* bary = nullptr) {
+  auto clamp01 = [](const T& x) { return smin<T>(T(1.0), smax<T>(x, T(0.0))); };
+  T d1[3], d2[3], r[3];
   for (int k = 0; k < 3; ++k) { d1[k] = p1[k] - p0[k]; d2[k] = q1[k] - q0[k]; r[k] = p0[k] - q0[k]; }
-  const double a = vdot<3>(d1, d1), e = vdot<3>(d2, d2), f = vdot<3>(d2, r);
-  double s = 0, t = 0;
-  const double c = vdot<3>(d1, r), b = vdot<3>(d1, d2);
-  const double denom = a * e - b * b;
+  const T a = vdot<3, T>(d1, d1), e = vdot<3, T>(d2, d2), f = vdot<3, T>(d2, r);
+  T s = 0.0, t = 0.0;
+  const T c = vdot<3, T>(d1, r), b = vdot<3, T>(d1, d2);
+  const T denom = a * e - b * b;
   if (denom != 0.0) s = clamp01((b * f - c * e) / denom);
-  else s = 0;
-  const double tnom = b * s + f;
-  if (tnom < 0 || e == 0) {
-    t = 0;
-    s = (a == 0) ? 0.0 : clamp01(-c / a);
+  else s = 0.0;
+  const T tnom = b * s + f;
+  if (tnom < 0.0 || e == 0.0) {
+    t = 0.0;
+    s = (a == 0.0) ? T(0.0) : clamp01(-c / a);
   } else if (tnom > e) {
     t = 1.0;
-    s = (a == 0) ? 0.0 : clamp01((b - c) / a);
+    s = (a == 0.0) ? T(0.0) : clamp01((b - c) / a);
   } else {
     t = tnom / e;
   }
+  // (s,t) minimises the segment distance: derivative at fixed (s,t)
+  s = Held<T>::of(s);
+  t = Held<T>::of(t);
   for (int k = 0; k < 3; ++k) {
-    const double c1 = p0[k] + s * d1[k];
-    const double c2 = q0[k] + t * d2[k];
+    const T c1 = p0[k] + s * d1[k];
+    const T c2 = q0[k] + t * d2[k];
     out[k] = c2 - c1;
   }
-  if (bary) { bary[0] = 1 - s; bary[1] = s; bary[2] = 1 - t; bary[3] = t; }
+  if (bary) { bary[0] = T(1.0) - s; bary[1] = s; bary[2] = T(1.0) - t; bary[3] = t; }
 }
 
 // NarrowPhase.cpp:714-750
-template <int D> inline double pair_distance(const double (*x)[D], bool is_vf) {
+template <int D, typename T = double> inline T pair_distance(const T (*x)[D], bool is_vf) {
   if constexpr (D == 2) {
-    double pt[2];
-    point_on_edge2(x[0], x[1], x[2], pt);
-    double d[2] = {x[0][0] - pt[0], x[0][1] - pt[1]};
-    return vnorm<2>(d);
+    T pt[2];
+    point_on_edge2<T>(x[0], x[1], x[2], pt);
+    T d[2] = {x[0][0] - pt[0], x[0][1] - pt[1]};
+    return vnorm<2, T>(d);
   } else {
     if (is_vf) {
-      double pt[3];
-      point_on_triangle(x[0], x[1], x[2], x[3], pt);
-      double d[3] = {x[0][0] - pt[0], x[0][1] - pt[1], x[0][2] - pt[2]};
-      return vnorm<3>(d);
+      T pt[3];
+      point_on_triangle<T>(x[0], x[1], x[2], x[3], pt);
+      T d[3] = {x[0][0] - pt[0], x[0][1] - pt[1], x[0][2] - pt[2]};
+      return vnorm<3, T>(d);
     }
-    double v[3];
-    edge_to_edge(x[0], x[1], x[2], x[3], v);
-    return vnorm<3>(v);
+    T v[3];
+    edge_to_edge<T>(x[0], x[1], x[2], x[3], v);
+    return vnorm<3, T>(v);
   }
 }
 
@@ -242,12 +264,12 @@ struct AccdStats {
 
 // NarrowPhase.cpp:634-711 — additive CCD, evaluated literally (including the
 // redundant first-iteration distance evaluation) so it is the checker.
-template <int D>
-inline bool additive_ccd(const double (*v0)[D], const double (*v1)[D], double eta, bool is_vf, double& toi,
-                         AccdStats* st = nullptr) {
+template <int D, typename T = double>
+inline bool additive_ccd(const T (*v0)[D], const T (*v1)[D], double eta, bool is_vf, T& toi, AccdStats* st = nullptr) {
   constexpr int ns = D + 1;
-  const double s = 1, xsi = eta, t_c = 1;
-  double p[ns][D], x[ns][D], pbar[D];
+  const double s = 1, t_c = 1;
+  const T xsi = eta;
+  T p[ns][D], x[ns][D], pbar[D];
   for (int k = 0; k < D; ++k) pbar[k] = 0.0;
   const double inv_ns = 1.0 / double(ns);
   for (int i = 0; i < ns; ++i)
@@ -258,31 +280,31 @@ inline bool additive_ccd(const double (*v0)[D], const double (*v1)[D], double et
     }
   for (int i = 0; i < ns; ++i)
     for (int k = 0; k < D; ++k) p[i][k] = p[i][k] - pbar[k];
-  double lf = 0, ls = 0;
+  T lf = 0.0, ls = 0.0;
   for (int i = 0; i < ns; ++i) {
     const bool first = is_vf ? i == 0 : i <= 1;
-    const double n = vnorm<D>(p[i]);
-    if (first) lf = smax(lf, n);
-    else ls = smax(ls, n);
+    const T n = vnorm<D, T>(p[i]);
+    if (first) lf = smax<T>(lf, n);
+    else ls = smax<T>(ls, n);
   }
-  const double l_p = lf + ls;
-  if (l_p <= 0) return false;
-  double d = pair_distance<D>(x, is_vf);
+  const T l_p = lf + ls;
+  if (l_p <= 0.0) return false;
+  T d = pair_distance<D, T>(x, is_vf);
   if (st) st->dist_evals++;
-  const double g = s * (d * d - xsi * xsi) / (d - xsi);
-  toi = 0;
-  double t_l = (1 - s) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+  const T g = s * (d * d - xsi * xsi) / (d - xsi);
+  toi = 0.0;
+  T t_l = (1 - s) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   if (d <= xsi) return true;
   const int max_iter = 1000;
   for (int iter = 0; iter < max_iter; ++iter) {
     if (st) st->iters++;
     for (int i = 0; i < ns; ++i)
       for (int k = 0; k < D; ++k) x[i][k] = x[i][k] + t_l * p[i][k];
-    d = pair_distance<D>(x, is_vf);
+    d = pair_distance<D, T>(x, is_vf);
     if (st) st->dist_evals++;
-    const double eps = (d * d - xsi * xsi) / (d + xsi);
-    if (toi > 0 && iter > 0 && eps < g) break;
-    toi += t_l;
+    const T eps = (d * d - xsi * xsi) / (d + xsi);
+    if (toi > 0.0 && iter > 0 && eps < g) break;
+    toi = toi + t_l;
     if (toi > t_c) return false;
     t_l = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   }

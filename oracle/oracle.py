@@ -61,6 +61,9 @@ def lib():
         L.oracle_tri_tri.argtypes = [P, P]
         L.oracle_edge_edge2.argtypes = [P, P]
         L.oracle_max_threads.restype = I
+        L.oracle_accd_grad.argtypes = [I, P, P, D, I, P, P, P]
+        L.oracle_toi_list.argtypes = [I, P, P, P, P, LL, D, D, P]
+        L.oracle_toi_grad_list.argtypes = [I, P, P, I, P, P, P, LL, D, D, P, P]
         _lib = L
     return _lib
 
@@ -214,6 +217,38 @@ def tri_tri(f0, f1):
 def edge_edge2(e0, e1):
     a, b = _f64(e0).reshape(4), _f64(e1).reshape(4)
     return bool(lib().oracle_edge_edge2(_p(a), _p(b)))
+
+
+def accd_grad(dim, verts0, verts1, eta, is_vf=True):
+    """(hit, toi, d toi/d verts0, d toi/d verts1) by dual numbers."""
+    v0, v1 = _f64(verts0), _f64(verts1)
+    g0 = np.zeros_like(v0)
+    g1 = np.zeros_like(v1)
+    toi = C.c_double(-1.0)
+    h = lib().oracle_accd_grad(dim, _p(v0), _p(v1), float(eta), int(is_vf), _p(g0), _p(g1), C.byref(toi))
+    return bool(h), toi.value, g0, g1
+
+
+def toi_list(V0, V1, sten, is_vf, eta_vf, eta_ee):
+    V0, V1 = _f64(V0), _f64(V1)
+    s = _i32(sten).reshape(-1, 4)
+    v = np.ascontiguousarray(is_vf, np.uint8)
+    out = np.zeros(len(s))
+    lib().oracle_toi_list(V0.shape[1], _p(V0), _p(V1), _p(s), _p(v), len(s), float(eta_vf), float(eta_ee), _p(out))
+    return out
+
+
+def toi_grad_list(V0, V1, sten, is_vf, w, eta_vf, eta_ee):
+    """sum_i w_i d toi_i / d(V0, V1) over a stencil list (dual numbers)."""
+    V0, V1 = _f64(V0), _f64(V1)
+    s = _i32(sten).reshape(-1, 4)
+    v = np.ascontiguousarray(is_vf, np.uint8)
+    w = _f64(w)
+    g0 = np.zeros_like(V0)
+    g1 = np.zeros_like(V1)
+    lib().oracle_toi_grad_list(V0.shape[1], _p(V0), _p(V1), len(V0), _p(s), _p(v), _p(w), len(s), float(eta_vf),
+                               float(eta_ee), _p(g0), _p(g1))
+    return g0, g1
 
 
 def run_scene(V0, V1, P, theta, dim=None, leaf_pairs=True, candidates=True, nthreads=0, **opts):

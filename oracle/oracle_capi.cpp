@@ -1,6 +1,7 @@
 // C entry points over the CPU oracle (mclccd_oracle.hpp) for ctypes.
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, the smoke check and the
 // CPU-baseline leg of bench.py, never by the product library.
+#include "dual.hpp"
 #include "mclccd_oracle.hpp"
 
 #include <chrono>
@@ -244,6 +245,92 @@ int oracle_tri_tri(const double* f0, const double* f1) {
 
 int oracle_edge_edge2(const double* e0, const double* e1) {
   return mclo::discrete_edge_edge2(e0, e0 + 2, e1, e1 + 2) ? 1 : 0;
+}
+
+}  // extern "C"
+
+// ---- TOI and its reference gradient ---------------------------------------
+namespace {
+template <int D>
+bool accd_grad_one(const double* verts0, const double* verts1, double eta, bool is_vf, double* g0, double* g1,
+                   double* toi_out) {
+  constexpr int ns = D + 1, N = 2 * ns * D;
+  using T = mclo::Dual<N>;
+  T a[ns][D], b[ns][D];
+  for (int i = 0; i < ns; ++i)
+    for (int k = 0; k < D; ++k) {
+      a[i][k] = T(verts0[i * D + k]);
+      a[i][k].g[i * D + k] = 1.0;
+      b[i][k] = T(verts1[i * D + k]);
+      b[i][k].g[ns * D + i * D + k] = 1.0;
+    }
+  T toi(-1.0);
+  const bool hit = mclo::additive_ccd<D, T>(a, b, eta, is_vf, toi);
+  if (toi_out) *toi_out = hit ? toi.v : -1.0;
+  for (int j = 0; j < ns * D; ++j) {
+    g0[j] = hit ? toi.g[j] : 0.0;
+    g1[j] = hit ? toi.g[ns * D + j] : 0.0;
+  }
+  return hit;
+}
+}  // namespace
+
+extern "C" {
+
+// d toi / d(verts0, verts1) by forward-mode dual numbers through the oracle's
+// additive_ccd; g0, g1: (dim+1)*dim each.  Returns hit.
+int oracle_accd_grad(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* g0,
+                     double* g1, double* toi) {
+  if (dim == 3) return accd_grad_one<3>(verts0, verts1, eta, is_vf != 0, g0, g1, toi) ? 1 : 0;
+  return accd_grad_one<2>(verts0, verts1, eta, true, g0, g1, toi) ? 1 : 0;
+}
+
+// toi (or -1) of every stencil of a list.
+void oracle_toi_list(int dim, const double* V0, const double* V1, const int* sten, const unsigned char* is_vf,
+                     long long n, double eta_vf, double eta_ee, double* toi) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long i = 0; i < n; ++i) {
+    double a[12], b[12], t = -1;
+    const int ns = dim + 1;
+    for (int j = 0; j < ns; ++j)
+      for (int k = 0; k < dim; ++k) { a[j * dim + k] = V0[(size_t)sten[4 * i + j] * dim + k]; b[j * dim + k] = V1[(size_t)sten[4 * i + j] * dim + k]; }
+    const bool vf = dim == 2 || is_vf[i];
+    bool hit;
+    if (dim == 3) hit = mclo::additive_ccd<3>(reinterpret_cast<const double(*)[3]>(a), reinterpret_cast<const double(*)[3]>(b), vf ? eta_vf : eta_ee, vf, t);
+    else hit = mclo::additive_ccd<2>(reinterpret_cast<const double(*)[2]>(a), reinterpret_cast<const double(*)[2]>(b), eta_vf, true, t);
+    toi[i] = hit ? t : -1.0;
+  }
+}
+
+// gV0, gV1 (nv*dim, overwritten) = sum_i w_i d toi_i / d(V0, V1) over a stencil list.
+void oracle_toi_grad_list(int dim, const double* V0, const double* V1, int nv, const int* sten,
+                          const unsigned char* is_vf, const double* w, long long n, double eta_vf, double eta_ee,
+                          double* gV0, double* gV1) {
+  const size_t m = (size_t)nv * dim;
+  std::vector<double> acc0(m, 0.0), acc1(m, 0.0);
+#pragma omp parallel
+  {
+    std::vector<double> l0(m, 0.0), l1(m, 0.0);
+#pragma omp for schedule(dynamic, 256)
+    for (long long i = 0; i < n; ++i) {
+      double a[12], b[12], g0[12], g1[12];
+      const int ns = dim + 1;
+      for (int j = 0; j < ns; ++j)
+        for (int k = 0; k < dim; ++k) { a[j * dim + k] = V0[(size_t)sten[4 * i + j] * dim + k]; b[j * dim + k] = V1[(size_t)sten[4 * i + j] * dim + k]; }
+      const bool vf = dim == 2 || is_vf[i];
+      if (dim == 3) accd_grad_one<3>(a, b, vf ? eta_vf : eta_ee, vf, g0, g1, nullptr);
+      else accd_grad_one<2>(a, b, eta_vf, true, g0, g1, nullptr);
+      for (int j = 0; j < ns; ++j)
+        for (int k = 0; k < dim; ++k) {
+          l0[(size_t)sten[4 * i + j] * dim + k] += w[i] * g0[j * dim + k];
+          l1[(size_t)sten[4 * i + j] * dim + k] += w[i] * g1[j * dim + k];
+        }
+    }
+#pragma omp critical
+    for (size_t j = 0; j < m; ++j) { acc0[j] += l0[j]; acc1[j] += l1[j]; }
+  }
+  std::memcpy(gV0, acc0.data(), sizeof(double) * m);
+  std::memcpy(gV1, acc1.data(), sizeof(double) * m);
 }
 
 int oracle_max_threads() {

@@ -68,3 +68,42 @@ def compare_hits(gpu, ref, canonical: bool, rtol: float = 0.0):
             if rel > rtol:
                 bad += 1
     return bad, maxrel, over
+
+
+def quirk_region(q, a, b, c):
+    """Projection.hpp:123-133: for s >= 0, t < 0, s + t >= det and a+d <= b+e the
+    reference sets s = clamp(-e/c), t = 0, i.e. a point of edge (p1,p2) chosen
+    with edge (p1,p3)'s parameter, which is not the closest point.  Returns
+    that point when q falls in the region, else None."""
+    e0, e1, w = b - a, c - a, a - q
+    A, B, C = e0 @ e0, e0 @ e1, e1 @ e1
+    D, E = e0 @ w, e1 @ w
+    det = A * C - B * B
+    s, t = B * E - C * D, B * D - A * E
+    if s + t >= det and s >= 0 and t < 0 and not (A + D > B + E):
+        r = -E / C
+        return a + e0 * min(max(r, 0.0), 1.0)
+    return None
+
+
+def quirk_stencils(count, seed=11):
+    """VF stencils (vertex, triangle) whose vertex starts in the reference's
+    point_on_triangle quirk region, moving toward the quirk point and
+    stopping 0.005 short of it; the triangle drifts slightly (no l_p ties)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < count:
+        a, b, c = rng.normal(size=(3, 3))
+        q = rng.normal(size=3) * 1.5
+        pt = quirk_region(q, a, b, c)
+        if pt is None:
+            continue
+        e1, w = c - a, a - q
+        if not 0.05 < -(e1 @ w) / (e1 @ e1) < 0.95:
+            continue
+        v0 = np.array([q, a, b, c])
+        v1 = v0.copy()
+        v1[0] = q + (pt - q) * (1 - 0.5e-2 / np.linalg.norm(q - pt))
+        v1[1:] += rng.normal(0, 1e-3, (3, 3))
+        out.append((v0, v1))
+    return out

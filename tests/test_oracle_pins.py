@@ -13,6 +13,7 @@ import pytest
 
 from mclccd_amd import scenes
 from oracle import oracle as orc
+from tests.parity import quirk_region, quirk_stencils
 
 # ---------------------------------------------------------------------------
 # leaf-pair set: tree traversal == O(n^2) brute force (the set is tree-independent)
@@ -251,22 +252,6 @@ def test_accd_conservative_on_cloth():
 # ---------------------------------------------------------------------------
 # projections vs dense sampling
 # ---------------------------------------------------------------------------
-def _quirk_region(q, a, b, c):
-    """Projection.hpp:123-133: for s >= 0, t < 0, s + t >= det and a+d <= b+e the
-    reference sets s = clamp(-e/c), t = 0, i.e. a point of edge (p1,p2) chosen
-    with edge (p1,p3)'s parameter, which is not the closest point.  Returns
-    that point when q falls in the region, else None."""
-    e0, e1, w = b - a, c - a, a - q
-    A, B, C = e0 @ e0, e0 @ e1, e1 @ e1
-    D, E = e0 @ w, e1 @ w
-    det = A * C - B * B
-    s, t = B * E - C * D, B * D - A * E
-    if s + t >= det and s >= 0 and t < 0 and not (A + D > B + E):
-        r = -E / C
-        return a + e0 * min(max(r, 0.0), 1.0)
-    return None
-
-
 def test_point_triangle_dense():
     rng = np.random.default_rng(1)
     g = np.linspace(0, 1, 241)
@@ -278,7 +263,7 @@ def test_point_triangle_dense():
         a, b, c = rng.normal(size=(3, 3))
         q = rng.normal(size=3) * 1.5
         pt = orc.point_on_triangle(q, a, b, c)
-        quirk = _quirk_region(q, a, b, c)
+        quirk = quirk_region(q, a, b, c)
         if quirk is not None:
             n_quirk += 1
             assert np.allclose(pt, quirk, atol=1e-12)
@@ -412,3 +397,85 @@ def test_edge_edge_2d_exact():
         assert got == exp
         seen["hit" if exp else "miss"] += 1
     assert min(seen.values()) > 100
+
+
+# ---------------------------------------------------------------------------
+# reference TOI gradient (dual numbers) vs central differences, on the CPU
+# ---------------------------------------------------------------------------
+def _general_position_cloth(n=16, seed=5):
+    # the rest pose of cloth_fold lies exactly on a grid (parallel edges, tied
+    # displacement norms), where the TOI has kinks; jitter both poses
+    V0, V1, F, th = scenes.cloth_fold(n=n)
+    rg = np.random.default_rng(seed)
+    return V0 + rg.normal(0, 1e-4, V0.shape), V1 + rg.normal(0, 1e-3, V1.shape), F, th
+
+
+def test_oracle_toi_gradient_vs_finite_differences():
+    V0, V1, F, th = _general_position_cloth()
+    o = orc.run_scene(V0, V1, F, th)
+    s, v, toi = o.hits()
+    assert v.sum() > 50 and (1 - v).sum() > 50
+    rng = np.random.default_rng(0)
+    h = 1e-7
+    checked = 0
+    for i in rng.choice(len(s), 200, replace=False):
+        st = s[i][:4]
+        a, b = V0[st], V1[st]
+        hit, t0, ga, gb = orc.accd_grad(3, a, b, th, bool(v[i]))
+        assert hit and t0 == toi[i]
+        for arr, g in ((0, ga), (1, gb)):
+            j, k = int(rng.integers(0, 4)), int(rng.integers(0, 3))
+            ap, bp = a.copy(), b.copy()
+            (ap if arr == 0 else bp)[j, k] += h
+            am, bm = a.copy(), b.copy()
+            (am if arr == 0 else bm)[j, k] -= h
+            hp, tp, _ = orc.accd(3, ap, bp, th, bool(v[i]))
+            hm, tm, _ = orc.accd(3, am, bm, th, bool(v[i]))
+            if not (hp and hm):
+                continue
+            assert abs((tp - tm) / (2 * h) - g[j, k]) <= 1e-6 * max(1.0, abs(g[j, k]))
+            checked += 1
+    assert checked > 350
+
+
+def test_oracle_toi_gradient_at_a_kink_is_one_sided():
+    # two parallel edges at t0 (distance 0.0625): moving q0 away keeps the
+    # distance (attained at q1), moving it closer shrinks it at unit rate, so
+    # the TOI has a kink; the reference gradient follows the branch the
+    # forward pass takes, i.e. equals one of the one-sided derivatives
+    a = np.array([[0.125, 0.375, 0.0], [0.125, 0.4375, 0.0], [0.1875, 0.375, 0.0], [0.1875, 0.4375, 0.0]])
+    b = np.array([[0.12736789, 0.37431296, 0.19628921], [0.12380183, 0.43635634, 0.10455922],
+                  [0.18655589, 0.37648112, 0.20963031], [0.18549369, 0.43685593, 0.11128993]])
+    h = 1e-8
+    hit, t0, ga, gb = orc.accd_grad(3, a, b, 1e-3, False)
+    assert hit
+    ap, am = a.copy(), a.copy()
+    ap[2, 0] += h
+    am[2, 0] -= h
+    fwd = (orc.accd(3, ap, b, 1e-3, False)[1] - t0) / h
+    bwd = (t0 - orc.accd(3, am, b, 1e-3, False)[1]) / h
+    assert abs(fwd - bwd) > 1.0
+    assert min(abs(ga[2, 0] - fwd), abs(ga[2, 0] - bwd)) < 1e-5
+
+
+def test_oracle_toi_gradient_in_quirk_region():
+    # there s = clamp(-e/c) is not a minimiser, so the gradient must carry
+    # ds/dx; holding s (as in the other branches) fails this check
+    h, eta, checked = 1e-7, 1e-2, 0
+    for v0, v1 in quirk_stencils(30):
+        hit, toi, g0, g1 = orc.accd_grad(3, v0, v1, eta, True)
+        if not hit:
+            continue
+        for arr, g in ((0, g0), (1, g1)):
+            for j in range(4):
+                for k in range(3):
+                    A0, A1 = v0.copy(), v1.copy()
+                    (A0 if arr == 0 else A1)[j, k] += h
+                    B0, B1 = v0.copy(), v1.copy()
+                    (B0 if arr == 0 else B1)[j, k] -= h
+                    hp, tp, _ = orc.accd(3, A0, A1, eta, True)
+                    hm, tm, _ = orc.accd(3, B0, B1, eta, True)
+                    assert hp and hm
+                    assert abs((tp - tm) / (2 * h) - g[j, k]) <= 1e-6 * max(1.0, abs(g[j, k]))
+                    checked += 1
+    assert checked >= 600
