@@ -13,16 +13,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import _lib
 from ._lib import lib
-
-
-class _DeviceArray:
-    """Borrowed device buffer exposed through __cuda_array_interface__ (copied at once)."""
-
-    def __init__(self, ptr, shape, typestr):
-        self.__cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (int(ptr or 0), False),
-                                         "version": 3}
 
 
 class TOIFunction(torch.autograd.Function):
@@ -31,22 +22,12 @@ class TOIFunction(torch.autograd.Function):
         torch.cuda.current_stream().synchronize()
         tree.update(V0, V1, F)
         tree.run(V0, V1, F)
-        n = tree.counts()["hits"]
-        r = _lib.DeviceResults()
-        tree._check(lib.mclg_get_device_results(tree._h, C.byref(r)))
-        dev = V0.device
-        if n:
-            toi = torch.as_tensor(_DeviceArray(r.hit_toi, (n,), "<f8"), device=dev).clone()
-            sten = torch.as_tensor(_DeviceArray(r.hit_sten4, (n, 4), "<i4"), device=dev).clone()
-            is_vf = torch.as_tensor(_DeviceArray(r.hit_is_vf, (n,), "|u1"), device=dev).clone()
-        else:
-            toi = torch.zeros(0, dtype=torch.float64, device=dev)
-            sten = torch.zeros((0, 4), dtype=torch.int32, device=dev)
-            is_vf = torch.zeros(0, dtype=torch.uint8, device=dev)
+        sten, is_vf, toi = (x.clone() for x in tree.device_hits())
         ctx.tree = tree
         ctx.serial = tree.serial
         ctx.shape = V0.shape
         ctx.mark_non_differentiable(sten, is_vf)
+        torch.cuda.current_stream().synchronize()
         return toi, sten, is_vf
 
     @staticmethod

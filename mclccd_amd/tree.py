@@ -211,6 +211,27 @@ class BVHTree:
         self._check(lib.mclg_copy_hits(self._h, _ptr(s), _ptr(v), _ptr(t), n))
         return s, v, t
 
+    def device_hits(self):
+        """(sten [n,4] int32, is_vf [n] uint8, toi [n] f64) as torch views of the
+        library's device buffers (no copy); valid until the next traverse."""
+        import torch
+
+        n = self.counts()["hits"]
+        r = _lib.DeviceResults()
+        self._check(lib.mclg_get_device_results(self._h, C.byref(r)))
+        dev = torch.device("cuda", torch.cuda.current_device())
+        if n == 0:
+            return (torch.zeros((0, 4), dtype=torch.int32, device=dev), torch.zeros(0, dtype=torch.uint8, device=dev),
+                    torch.zeros(0, dtype=torch.float64, device=dev))
+
+        def view(ptr, shape, typestr):
+            class _A:
+                __cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (int(ptr), False),
+                                            "version": 3}
+            return torch.as_tensor(_A(), device=dev)
+
+        return view(r.hit_sten4, (n, 4), "<i4"), view(r.hit_is_vf, (n,), "|u1"), view(r.hit_toi, (n,), "<f8")
+
     def discrete(self):
         n = self.counts()["discrete"]
         out = np.empty((n, 2), np.int32)

@@ -75,6 +75,17 @@ template <int D, typename T = double> inline T vnorm(const T* a) { return osqrt(
 // std::max / std::min as libstdc++ defines them: max(a,b) = a<b ? b : a,
 // min(a,b) = b<a ? b : a.  The NaN and signed-zero behaviour matters.
 template <typename T> inline T smax(const T& a, const T& b) { return (a < b) ? b : a; }
+// Branch signature (test instrumentation for finite-difference checks): while
+// branch_sig points to a word on this thread, the distance and ACCD routines
+// fold every branch they take (closest-point region, clamp state of each
+// ratio, l_p argmax, step count) into it.  Two inputs with equal signatures
+// evaluate the same smooth formula.
+inline thread_local uint64_t* branch_sig = nullptr;
+inline void note_branch(uint64_t code) {
+  if (branch_sig) *branch_sig = (*branch_sig ^ (code + 0x9e3779b97f4a7c15ULL)) * 0x100000001b3ULL;
+}
+template <typename T> inline int clamp_state(const T& x) { return x < 0.0 ? 0 : (x > 1.0 ? 2 : 1); }
+
 // Held<T>::of(x) is x for plain arithmetic; a derivative-carrying scalar type
 // specialises it to drop x's derivatives (x treated as a constant).
 template <typename T> struct Held {
@@ -144,7 +155,10 @@ template <int D> inline bool boxes_intersect(const BBox<D>& l, const BBox<D>& r)
 // Projection.hpp:60-145 — closest point of triangle (a,b,c) to q.
 template <typename T = double>
 inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T* out) {
-  auto clamp01 = [](const T& v) { return v < 0.0 ? T(0.0) : (v > 1.0 ? T(1.0) : v); };
+  auto clamp01 = [](const T& v) {
+    note_branch(30 + clamp_state(v));
+    return v < 0.0 ? T(0.0) : (v > 1.0 ? T(1.0) : v);
+  };
   T e0[3], e1[3], w[3];
   for (int k = 0; k < 3; ++k) { e0[k] = b[k] - a[k]; e1[k] = c[k] - a[k]; w[k] = a[k] - q[k]; }
   const T A = vdot<3, T>(e0, e0), B = vdot<3, T>(e0, e1), C = vdot<3, T>(e1, e1);
@@ -153,15 +167,17 @@ inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T*
   T s = B * E - C * Dd;
   T t = B * Dd - A * E;
   bool quirk = false;  // Projection.hpp:131 (s >= 0, t < 0, s + t >= det, a + d <= b + e)
+  int region;
   if (s + t < det) {
     if (s < 0.0) {
       if (t < 0.0) {
-        if (Dd < 0.0) { s = clamp01(-Dd / A); t = 0.0; }
-        else { s = 0.0; t = clamp01(-E / C); }
-      } else { s = 0.0; t = clamp01(-E / C); }
+        if (Dd < 0.0) { region = 1; s = clamp01(-Dd / A); t = 0.0; }
+        else { region = 2; s = 0.0; t = clamp01(-E / C); }
+      } else { region = 3; s = 0.0; t = clamp01(-E / C); }
     } else if (t < 0.0) {
-      s = clamp01(-Dd / A); t = 0.0;
+      region = 4; s = clamp01(-Dd / A); t = 0.0;
     } else {
+      region = 5;
       const T inv = T(1.0) / det;
       s = s * inv; t = t * inv;
     }
@@ -169,19 +185,23 @@ inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T*
     if (s < 0.0) {
       const T t0 = B + Dd, t1 = C + E;
       if (t1 > t0) {
+        region = 6;
         const T num = t1 - t0, den = A - 2.0 * B + C;
         s = clamp01(num / den); t = T(1.0) - s;
-      } else { t = clamp01(-E / C); s = 0.0; }
+      } else { region = 7; t = clamp01(-E / C); s = 0.0; }
     } else if (t < 0.0) {
       if (A + Dd > B + E) {
+        region = 8;
         const T num = C + E - B - Dd, den = A - 2.0 * B + C;
         s = clamp01(num / den); t = T(1.0) - s;
-      } else { s = clamp01(-E / C); t = 0.0; quirk = true; }
+      } else { region = 9; s = clamp01(-E / C); t = 0.0; quirk = true; }
     } else {
+      region = 10;
       const T num = C + E - B - Dd, den = A - 2.0 * B + C;
       s = clamp01(num / den); t = T(1.0) - s;
     }
   }
+  note_branch(10 + region);
   // Every branch but the quirk one yields the minimiser (s,t) of |q - x(s,t)|
   // over the triangle (or over the edge it clamps to), where the distance's
   // derivative is the partial at fixed (s,t) (envelope theorem).  Holding
@@ -195,9 +215,10 @@ inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T*
 template <typename T = double> inline void point_on_edge2(const T* p, const T* e0, const T* e1, T* out) {
   T e[2] = {e1[0] - e0[0], e1[1] - e0[1]};
   const T len2 = vdot<2, T>(e, e);
-  if (len2 <= 0.0) { out[0] = e0[0]; out[1] = e0[1]; return; }
+  if (len2 <= 0.0) { note_branch(40); out[0] = e0[0]; out[1] = e0[1]; return; }
   T pe[2] = {p[0] - e0[0], p[1] - e0[1]};
   const T t = Held<T>::of(vdot<2, T>(pe, e) / len2);  // minimiser: derivative at fixed t
+  note_branch(41 + clamp_state(t));
   if (t < 0.0) { out[0] = e0[0]; out[1] = e0[1]; return; }
   if (t > 1.0) { out[0] = e1[0]; out[1] = e1[1]; return; }
   out[0] = e0[0] + t * e[0];
@@ -207,7 +228,10 @@ template <typename T = double> inline void point_on_edge2(const T* p, const T* e
 // Projection.hpp:241-302 — shortest vector from segment p to segment q.
 template <typename T = double>
 inline void edge_to_edge(const T* p0, const T* p1, const T* q0, const T* q1, T* out, T* bary = nullptr) {
-  auto clamp01 = [](const T& x) { return smin<T>(T(1.0), smax<T>(x, T(0.0))); };
+  auto clamp01 = [](const T& x) {
+    note_branch(50 + clamp_state(x));
+    return smin<T>(T(1.0), smax<T>(x, T(0.0)));
+  };
   T d1[3], d2[3], r[3];
   for (int k = 0; k < 3; ++k) { d1[k] = p1[k] - p0[k]; d2[k] = q1[k] - q0[k]; r[k] = p0[k] - q0[k]; }
   const T a = vdot<3, T>(d1, d1), e = vdot<3, T>(d2, d2), f = vdot<3, T>(d2, r);
@@ -217,6 +241,7 @@ inline void edge_to_edge(const T* p0, const T* p1, const T* q0, const T* q1, T* 
   if (denom != 0.0) s = clamp01((b * f - c * e) / denom);
   else s = 0.0;
   const T tnom = b * s + f;
+  note_branch(60 + (denom != 0.0) + 2 * (tnom < 0.0 || e == 0.0 ? 0 : (tnom > e ? 1 : 2)) + 8 * (a == 0.0));
   if (tnom < 0.0 || e == 0.0) {
     t = 0.0;
     s = (a == 0.0) ? T(0.0) : clamp01(-c / a);
@@ -281,12 +306,14 @@ inline bool additive_ccd(const T (*v0)[D], const T (*v1)[D], double eta, bool is
   for (int i = 0; i < ns; ++i)
     for (int k = 0; k < D; ++k) p[i][k] = p[i][k] - pbar[k];
   T lf = 0.0, ls = 0.0;
+  int af = -1, as = -1;  // argmax (signature only)
   for (int i = 0; i < ns; ++i) {
     const bool first = is_vf ? i == 0 : i <= 1;
     const T n = vnorm<D, T>(p[i]);
-    if (first) lf = smax<T>(lf, n);
-    else ls = smax<T>(ls, n);
+    if (first) { if (lf < n) af = i; lf = smax<T>(lf, n); }
+    else { if (ls < n) as = i; ls = smax<T>(ls, n); }
   }
+  note_branch(70 + af + 8 * as);
   const T l_p = lf + ls;
   if (l_p <= 0.0) return false;
   T d = pair_distance<D, T>(x, is_vf);
@@ -303,9 +330,9 @@ inline bool additive_ccd(const T (*v0)[D], const T (*v1)[D], double eta, bool is
     d = pair_distance<D, T>(x, is_vf);
     if (st) st->dist_evals++;
     const T eps = (d * d - xsi * xsi) / (d + xsi);
-    if (toi > 0.0 && iter > 0 && eps < g) break;
+    if (toi > 0.0 && iter > 0 && eps < g) { note_branch(200 + iter); break; }
     toi = toi + t_l;
-    if (toi > t_c) return false;
+    if (toi > t_c) { note_branch(1300 + iter); return false; }
     t_l = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   }
   return true;  // also reached at max_iter (the release-build assert is compiled out)

@@ -62,7 +62,7 @@ def lib():
         L.oracle_edge_edge2.argtypes = [P, P]
         L.oracle_max_threads.restype = I
         L.oracle_accd_grad.argtypes = [I, P, P, D, I, P, P, P]
-        L.oracle_toi_list.argtypes = [I, P, P, P, P, LL, D, D, P]
+        L.oracle_toi_list.argtypes = [I, P, P, P, P, LL, D, D, P, P]
         L.oracle_toi_grad_list.argtypes = [I, P, P, I, P, P, P, LL, D, D, P, P]
         _lib = L
     return _lib
@@ -229,13 +229,17 @@ def accd_grad(dim, verts0, verts1, eta, is_vf=True):
     return bool(h), toi.value, g0, g1
 
 
-def toi_list(V0, V1, sten, is_vf, eta_vf, eta_ee):
+def toi_list(V0, V1, sten, is_vf, eta_vf, eta_ee, signature=False):
+    """ACCD TOI per stencil (-1: no hit); with ``signature`` also a uint64 per
+    stencil hashing every branch taken (equal signatures: same smooth formula)."""
     V0, V1 = _f64(V0), _f64(V1)
     s = _i32(sten).reshape(-1, 4)
     v = np.ascontiguousarray(is_vf, np.uint8)
     out = np.zeros(len(s))
-    lib().oracle_toi_list(V0.shape[1], _p(V0), _p(V1), _p(s), _p(v), len(s), float(eta_vf), float(eta_ee), _p(out))
-    return out
+    sig = np.zeros(len(s), np.uint64) if signature else None
+    lib().oracle_toi_list(V0.shape[1], _p(V0), _p(V1), _p(s), _p(v), len(s), float(eta_vf), float(eta_ee), _p(out),
+                          _p(sig))
+    return (out, sig) if signature else out
 
 
 def toi_grad_list(V0, V1, sten, is_vf, w, eta_vf, eta_ee):

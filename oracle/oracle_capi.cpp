@@ -286,10 +286,13 @@ int oracle_accd_grad(int dim, const double* verts0, const double* verts1, double
 }
 
 // toi (or -1) of every stencil of a list.
+// toi[i] = ACCD TOI of stencil i (-1: no hit); sig (optional) = its branch signature.
 void oracle_toi_list(int dim, const double* V0, const double* V1, const int* sten, const unsigned char* is_vf,
-                     long long n, double eta_vf, double eta_ee, double* toi) {
+                     long long n, double eta_vf, double eta_ee, double* toi, unsigned long long* sig) {
 #pragma omp parallel for schedule(dynamic, 256)
   for (long long i = 0; i < n; ++i) {
+    uint64_t word = 0;
+    mclo::branch_sig = sig ? &word : nullptr;
     double a[12], b[12], t = -1;
     const int ns = dim + 1;
     for (int j = 0; j < ns; ++j)
@@ -299,6 +302,8 @@ void oracle_toi_list(int dim, const double* V0, const double* V1, const int* ste
     if (dim == 3) hit = mclo::additive_ccd<3>(reinterpret_cast<const double(*)[3]>(a), reinterpret_cast<const double(*)[3]>(b), vf ? eta_vf : eta_ee, vf, t);
     else hit = mclo::additive_ccd<2>(reinterpret_cast<const double(*)[2]>(a), reinterpret_cast<const double(*)[2]>(b), eta_vf, true, t);
     toi[i] = hit ? t : -1.0;
+    if (sig) sig[i] = word;
+    mclo::branch_sig = nullptr;
   }
 }
 
