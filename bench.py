@@ -226,6 +226,27 @@ def main():
         roof = {"kernel": "k_traverse<3>", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
 
+    # TOI adjoint over every hit of the last traverse, w_i ~ U(0,1) (seed 4)
+    import ctypes as C
+
+    from mclccd_amd._lib import lib
+
+    gen = torch.Generator(device=dev).manual_seed(4)
+    w = torch.rand(c["hits"], dtype=torch.float64, device=dev, generator=gen)
+    gV0 = torch.empty_like(dV0)
+    gV1 = torch.empty_like(dV0)
+    adj = []
+    for i in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t._check(lib.mclg_toi_grad(t._h, C.c_void_p(w.data_ptr()), C.c_void_p(gV0.data_ptr()),
+                                   C.c_void_p(gV1.data_ptr()), 1))
+        if i >= args.warmup:
+            adj.append(t.timings()["adjoint"])
+    del w
+    adjoint = {"ms": round(float(np.median(adj)), 3), "hits": c["hits"], "weights": "U(0,1), seed 4",
+               "note": "dL/dV0, dL/dV1 of L = sum w_i toi_i over every CCD hit (FP64, HIP events)"}
+    print(f"[bench] adjoint {adjoint}", file=sys.stderr, flush=True)
+
     batched = None
     if not args.no_batch:
         batched = batch_bench(max(args.steps, 3))
@@ -254,6 +275,7 @@ def main():
                    "phase_ms": {k: round(v, 4) for k, v in avg.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
         "batched": batched,
     }
     print(json.dumps(line), flush=True)

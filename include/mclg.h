@@ -157,8 +157,9 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
  *   0 validate+pack  1 reptri  2 leaf boxes  3 morton  4 sort  5 lbvh (update)
  *   6 validate+pack  7 traversal kernel  8 candidate + discrete kernel
  *   9 ACCD kernels (traverse; 7-9 bracket the final launches only)
+ *  10 TOI adjoint (last mclg_toi_grad: gradient clear + adjoint kernel)
  * Writes min(n, MCLG_NUM_TIMINGS) values. */
-#define MCLG_NUM_TIMINGS 10
+#define MCLG_NUM_TIMINGS 11
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
 
 /* TOI adjoint.  No reference counterpart.  For the CCD hits of the last

@@ -22,9 +22,9 @@ ERR_CAPACITY = -6
 
 KEEP_CANDIDATES = 1
 KEEP_LEAF_PAIRS = 2
-NUM_TIMINGS = 10
+NUM_TIMINGS = 11
 TIMING_NAMES = ("update_pack", "update_reptri", "update_leaf_boxes", "update_morton", "update_sort",
-                "update_lbvh", "traverse_pack", "traverse", "expand", "accd")
+                "update_lbvh", "traverse_pack", "traverse", "expand", "accd", "adjoint")
 
 
 class Options(C.Structure):

@@ -14,6 +14,7 @@
 // (the reference's branches, including its point_on_triangle quirk region),
 // so the result is the derivative of the TOI wherever it is smooth.
 #include <algorithm>
+#include <cstdint>
 
 #include "../../include/mclg.h"
 #include "context.hpp"
@@ -157,21 +158,29 @@ __device__ __forceinline__ double dist_g(const typename AdjV<D>::V* x, double se
   }
 }
 
-constexpr int ADJ_STORE = 8;
+constexpr int ADJ_T = 256;     // threads per block
+constexpr int ADJ_STORE = 8;   // step sums S_j kept in LDS (j < ADJ_STORE); later ones are replayed
 
 // One hit: accumulates w * d toi / d(V0, V1) into gV0 / gV1 (atomics).
+// S points to this thread's LDS column (stride ADJ_T).  Every per-vertex array
+// is indexed by unrolled constants only, so nothing lives in scratch memory.
 template <int D, bool VF>
-__device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double w, double* gV0, double* gV1) {
+__device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double w, double* S, bool skip_k1,
+                            double* gV0, double* gV1) {
   using V = typename AdjV<D>::V;
   constexpr int ns = D + 1;
-  V v0[ns], v1[ns];
-#pragma unroll
-  for (int i = 0; i < ns; ++i) VLoad<D>::load(rec, sv[i], v0[i], v1[i]);
-  // setup (centred displacements, l_p and its maximisers)
+  V v0[ns];
   Stencil<D, VF> st;
-  bool fin;
-  const double l_p = st.setup(v0, v1, fin);
+  double l_p;
+  {
+    V v1[ns];
+#pragma unroll
+    for (int i = 0; i < ns; ++i) VLoad<D>::load(rec, sv[i], v0[i], v1[i]);
+    bool fin;
+    l_p = st.setup(v0, v1, fin);
+  }
   if (!(l_p > 0)) return;
+  // maximisers of l_p = max_first |p_i| + max_second |p_i| (first maximum wins)
   int i_first = -1, i_second = -1;
   {
     double lf = 0.0, ls = 0.0;
@@ -187,10 +196,10 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
   if (d0 <= xsi) return;  // toi = 0 whatever the inputs nearby
   const double c = 0.9 / l_p;
   // forward: count the steps K summed into toi, keep S_j (j < ADJ_STORE)
-  double S[ADJ_STORE];
-  S[0] = 0.0;
   const double g = (d0 * d0 - xsi * xsi) / (d0 - xsi);
-  double toi = 0.0, tl = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+  const double tl1 = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+  double toi = 0.0, tl = tl1;
+  S[0] = 0.0;
   int K = 0;  // a loop that reaches max_iter sums steps 1..999 (K = 999)
   for (int iter = 1; iter < 1000; ++iter) {
     const bool can_break = toi > 0.0;
@@ -200,10 +209,11 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
     if (can_break && (d * d - xsi * xsi) / (d + xsi) < g) break;
     toi += tl;
     ++K;
-    if (K < ADJ_STORE) S[K] = S[K - 1] + tl;
+    if (K < ADJ_STORE) S[K * ADJ_T] = S[(K - 1) * ADJ_T] + tl;
     if (toi > 1.0) return;
     tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
   }
+  if (skip_k1 && K == 1) return;  // done by the one-step kernel
   // reverse sweep
   V gx0[ns], gp[ns];
 #pragma unroll
@@ -211,17 +221,17 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
   double G = w, glp = 0.0;
   for (int j = K - 1; j >= 0; --j) {
     double Sj;
-    if (j < ADJ_STORE) Sj = S[j];
-    else {  // long tails: replay the steps up to j
-      Stencil<D, VF> f2;
-      bool fin2;
-      (void)f2.setup(v0, v1, fin2);
-      double tt = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+    if (j < ADJ_STORE) {
+      Sj = S[j * ADJ_T];
+    } else {  // long tails: replay the steps up to j from x0
+#pragma unroll
+      for (int i = 0; i < ns; ++i) st.x[i] = v0[i];
+      double tt = tl1;
       Sj = 0.0;
       for (int m = 1; m <= j; ++m) {
         Sj += tt;
-        f2.step(tt);
-        const double dm = f2.dist();
+        st.step(tt);
+        const double dm = st.dist();
         tt = 0.9 * (dm * dm - xsi * xsi) / ((dm + xsi) * l_p);
       }
     }
@@ -244,19 +254,17 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
     glp -= G * tau / l_p;
     if (j >= 1) G = G * (1.0 + c * gdotp);
   }
-  // l_p = |p_first| + |p_second| at the maximisers
-  if (i_first >= 0) {
-    const double n = __builtin_sqrt(dot(st.p[i_first], st.p[i_first]));
-    if (n > 0) gp[i_first] = add(gp[i_first], scl(st.p[i_first], glp / n));
-  }
-  if (i_second >= 0) {
-    const double n = __builtin_sqrt(dot(st.p[i_second], st.p[i_second]));
-    if (n > 0) gp[i_second] = add(gp[i_second], scl(st.p[i_second], glp / n));
-  }
-  // centring p_i = P_i - mean(P): gP_i = gp_i - mean(gp); V1 = V0 + P
+  // l_p = |p_first| + |p_second| at the maximisers; then the centring
+  // p_i = P_i - mean(P): gP_i = gp_i - mean(gp), and V1 = V0 + P
   V mean = {};
 #pragma unroll
-  for (int i = 0; i < ns; ++i) mean = add(mean, gp[i]);
+  for (int i = 0; i < ns; ++i) {
+    if (i == i_first || i == i_second) {
+      const double n = __builtin_sqrt(dot(st.p[i], st.p[i]));
+      if (n > 0) gp[i] = add(gp[i], scl(st.p[i], glp / n));
+    }
+    mean = add(mean, gp[i]);
+  }
   mean = scl(mean, 1.0 / ns);
 #pragma unroll
   for (int i = 0; i < ns; ++i) {
@@ -275,23 +283,221 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
   }
 }
 
+// Per-block LDS accumulator: open-addressing table vertex -> (dL/dV0, dL/dV1)
+// summed with LDS atomics; flushed with one global atomic per component and
+// distinct vertex.  A crowded table sends the contribution to global memory.
+constexpr int ADJ_TAB_BITS = 10, ADJ_TAB = 1 << ADJ_TAB_BITS, ADJ_PROBES = 16;
+constexpr int ADJ_R = 4;  // hits per thread per block of the one-step kernel
+
+struct AdjTable {
+  int* key;     // ADJ_TAB entries, -1 = empty
+  double* acc;  // 2*D components, component-major (acc[c*ADJ_TAB + slot])
+};
+
 template <int D>
-__global__ __launch_bounds__(256) void k_toi_adjoint(const int4* __restrict__ sten, const uint8_t* __restrict__ is_vf,
-                                                     const double* __restrict__ w, long long nh,
-                                                     const double* __restrict__ rec, double vf_eta, double ee_eta,
-                                                     double* gV0, double* gV1) {
+__device__ __forceinline__ void adj_accum(const AdjTable& tab, int v, const double* val, double* gV0, double* gV1) {
+  unsigned h = ((unsigned)v * 2654435761u) >> (32 - ADJ_TAB_BITS);
+  for (int probe = 0; probe < ADJ_PROBES; ++probe) {
+    const int k = atomicCAS(&tab.key[h], -1, v);
+    if (k == -1 || k == v) {
+#pragma unroll
+      for (int c = 0; c < 2 * D; ++c) atomicAdd(&tab.acc[c * ADJ_TAB + h], val[c]);
+      return;
+    }
+    h = (h + 1) & (ADJ_TAB - 1);
+  }
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    atomicAdd(gV0 + (size_t)v * D + c, val[c]);
+    atomicAdd(gV1 + (size_t)v * D + c, val[D + c]);
+  }
+}
+
+// One-step hits (the loop breaks at its second iteration: K = 1, nearly every
+// hit): toi = tau_1 = c phi(d0), so the gradient needs D's gradient at x0
+// only and no step sums.  Returns false, adding nothing, when the hit takes
+// more steps; hits that are not CCD hits add nothing and return true.
+template <int D, bool VF>
+__device__ bool toi_adjoint_k1(const int* sv, const double* rec, double xsi, double w, const AdjTable& tab,
+                               double* gV0, double* gV1) {
+  using V = typename AdjV<D>::V;
+  constexpr int ns = D + 1;
+  Stencil<D, VF> st;
+  double l_p;
+  {
+    V v0[ns], v1[ns];
+#pragma unroll
+    for (int i = 0; i < ns; ++i) VLoad<D>::load(rec, sv[i], v0[i], v1[i]);
+    bool fin;
+    l_p = st.setup(v0, v1, fin);
+  }
+  if (!(l_p > 0)) return true;
+  const double d0 = st.dist();
+  if (d0 <= xsi) return true;
+  const double g = (d0 * d0 - xsi * xsi) / (d0 - xsi);
+  const double tl1 = 0.9 * (d0 * d0 - xsi * xsi) / ((d0 + xsi) * l_p);
+  if (tl1 > 1.0) return true;  // iteration 1: toi + t_l > 1 with toi = 0
+  st.step(tl1);
+  const double d1 = st.dist();
+  if (!(tl1 > 0.0)) return false;  // the break test cannot fire at iteration 2
+  const double tl2 = 0.9 * (d1 * d1 - xsi * xsi) / ((d1 + xsi) * l_p);
+  st.step(tl2);
+  const double d2 = st.dist();
+  if (!((d2 * d2 - xsi * xsi) / (d2 + xsi) < g)) return false;
+  // reverse: dtoi/dx0_i = c grad_i D(x0); dtoi/dl_p = -toi / l_p to the maximisers
+  const double c = 0.9 / l_p;
+  V v0[ns], gd[ns];  // x0 is reloaded (cached) rather than kept live through the forward steps
+#pragma unroll
+  for (int i = 0; i < ns; ++i) {
+    V unused;
+    VLoad<D>::load(rec, sv[i], v0[i], unused);
+    gd[i] = {};
+  }
+  const double dd = dist_g<D, VF>(v0, w * c, gd);
+  const double glp = -w * (c * ((dd * dd - xsi * xsi) / (dd + xsi))) / l_p;
+  V gp[ns], mean = {};
+  {
+    double lf = 0.0, ls = 0.0;
+    int i_first = -1, i_second = -1;
+#pragma unroll
+    for (int i = 0; i < ns; ++i) {
+      const double n = __builtin_sqrt(dot(st.p[i], st.p[i]));
+      const bool first = VF ? (i == 0) : (i <= 1);
+      if (first) { if (lf < n) { lf = n; i_first = i; } }
+      else if (ls < n) { ls = n; i_second = i; }
+    }
+#pragma unroll
+    for (int i = 0; i < ns; ++i) {
+      gp[i] = {};
+      if (i == i_first || i == i_second) {
+        const double n = __builtin_sqrt(dot(st.p[i], st.p[i]));
+        if (n > 0) gp[i] = scl(st.p[i], glp / n);
+      }
+      mean = add(mean, gp[i]);
+    }
+  }
+  mean = scl(mean, 1.0 / ns);
+#pragma unroll
+  for (int i = 0; i < ns; ++i) {
+    const V gP = sub(gp[i], mean);
+    const V g0 = sub(gd[i], gP);
+    double val[2 * D];
+    val[0] = g0.x;
+    val[1] = g0.y;
+    val[D] = gP.x;
+    val[D + 1] = gP.y;
+    if constexpr (D == 3) {
+      val[2] = g0.z;
+      val[5] = gP.z;
+    }
+    adj_accum<D>(tab, sv[i], val, gV0, gV1);
+  }
+  return true;
+}
+
+// One-step kernel, one launch per stencil kind.  Entries k in [0, nh) are hit
+// order[k] (or hit k when order is null); skey[k] (sorted (kind, first
+// vertex) keys, kind at bit kind_shift) lets blocks of the other kind leave at
+// once.  Longer hits are appended to queue (at most qcap; *qn counts all).
+template <int D, bool VF>
+__global__ __launch_bounds__(ADJ_T) void k_toi_adjoint_k1(const int4* __restrict__ sten,
+                                                          const uint8_t* __restrict__ is_vf,
+                                                          const double* __restrict__ w,
+                                                          const uint32_t* __restrict__ order,
+                                                          const uint32_t* __restrict__ skey, int kind_shift,
+                                                          long long nh, const double* __restrict__ rec, double eta,
+                                                          double* gV0, double* gV1, long long* queue, long long qcap,
+                                                          unsigned long long* qn) {
+  __shared__ int s_key[ADJ_TAB];
+  __shared__ double s_acc[2 * D * ADJ_TAB];
+  const uint32_t my_kind = (D == 3 && !VF) ? 1u : 0u;
+  const long long base = blockIdx.x * (long long)(ADJ_T * ADJ_R);
+  if (skey) {  // block-uniform: keys are sorted by kind first
+    const long long last = (base + ADJ_T * ADJ_R < nh ? base + ADJ_T * ADJ_R : nh) - 1;
+    const uint32_t k0 = skey[base] >> kind_shift, k1 = skey[last] >> kind_shift;
+    if (k0 > my_kind || k1 < my_kind) return;
+  }
+  for (int j = threadIdx.x; j < ADJ_TAB; j += ADJ_T) {
+    s_key[j] = -1;
+#pragma unroll
+    for (int c = 0; c < 2 * D; ++c) s_acc[c * ADJ_TAB + j] = 0.0;
+  }
+  __syncthreads();
+  const AdjTable tab{s_key, s_acc};
+  for (int r = 0; r < ADJ_R; ++r) {
+    const long long k = base + r * ADJ_T + threadIdx.x;
+    bool more = false;
+    long long i = k;
+    if (k < nh) {
+      bool mine;
+      if (skey) {
+        i = order[k];
+        mine = (skey[k] >> kind_shift) == my_kind;
+      } else {
+        mine = D == 2 || (is_vf[i] != 0) == VF;
+      }
+      if (mine) {
+        const double wi = w[i];
+        if (wi != 0.0) {
+          const int4 s4 = sten[i];
+          const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+          more = !toi_adjoint_k1<D, VF>(sv, rec, eta, wi, tab, gV0, gV1);
+        }
+      }
+    }
+    const long long slot = wave_append(more, qn);
+    if (more && slot < qcap) queue[slot] = i;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < ADJ_TAB; j += ADJ_T) {
+    const int v = s_key[j];
+    if (v < 0) continue;
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      const double a0 = s_acc[c * ADJ_TAB + j], a1 = s_acc[(D + c) * ADJ_TAB + j];
+      if (a0 != 0.0) atomicAdd(gV0 + (size_t)v * D + c, a0);
+      if (a1 != 0.0) atomicAdd(gV1 + (size_t)v * D + c, a1);
+    }
+  }
+}
+
+// Sort keys of the hits: (kind, first vertex) with kind = 1 for EE, 0 else.
+__global__ void k_adj_keys(const int4* __restrict__ sten, const uint8_t* __restrict__ is_vf, long long nh, int dim,
+                           int kind_shift, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= nh) return;
-  const int4 s4 = sten[i];
-  const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+  const uint32_t kind = (dim == 3 && !is_vf[i]) ? 1u : 0u;
+  keys[i] = (kind << kind_shift) | (uint32_t)sten[i].x;
+  vals[i] = (uint32_t)i;
+}
+
+// Multi-step hits: over the queue (queue != nullptr, n entries), or over all
+// hits skipping the one-step ones (queue overflowed).
+template <int D, bool VF>
+__global__ __launch_bounds__(ADJ_T) void k_toi_adjoint(const int4* __restrict__ sten,
+                                                       const uint8_t* __restrict__ is_vf,
+                                                       const double* __restrict__ w, const long long* queue,
+                                                       long long n, const double* __restrict__ rec, double eta,
+                                                       double* gV0, double* gV1) {
+  __shared__ double s_S[ADJ_STORE * ADJ_T];
+  const long long k = blockIdx.x * (long long)ADJ_T + threadIdx.x;
+  if (k >= n) return;
+  const long long i = queue ? queue[k] : k;
+  if (D == 3 && (is_vf[i] != 0) != VF) return;
   const double wi = w[i];
   if (wi == 0.0) return;
-  if constexpr (D == 3) {
-    if (is_vf[i]) toi_adjoint<3, true>(sv, rec, vf_eta, wi, gV0, gV1);
-    else toi_adjoint<3, false>(sv, rec, ee_eta, wi, gV0, gV1);
-  } else {
-    toi_adjoint<2, true>(sv, rec, vf_eta, wi, gV0, gV1);
-  }
+  const int4 s4 = sten[i];
+  const int sv[4] = {s4.x, s4.y, s4.z, s4.w};
+  toi_adjoint<D, VF>(sv, rec, eta, wi, s_S + threadIdx.x, queue == nullptr, gV0, gV1);
+}
+
+template <int D, bool VF>
+static void launch_adjoint(Ctx* c, const double* gw, const uint32_t* order, const uint32_t* skey, int kind_shift,
+                           long long nh, double eta, double* g0, double* g1) {
+  const unsigned g = (unsigned)((nh + ADJ_T * ADJ_R - 1) / (ADJ_T * ADJ_R));
+  k_toi_adjoint_k1<D, VF><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, order, skey, kind_shift, nh,
+                                                      c->d_rec, eta, g0, g1, c->d_adj_q, (long long)c->cap_adj_q,
+                                                      c->d_adj_n);
 }
 
 }  // namespace mclg
@@ -323,22 +529,74 @@ extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* 
     g1 = c->d_adj_g + (size_t)c->dim * c->nv;
     if (nh > 0) HIPCHECK(c, hipMemcpyAsync(gw, w, sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
   }
+  c->timing = true;
+  c->mark(14);
   launch_fill_u64(reinterpret_cast<unsigned long long*>(g0), (long long)c->dim * c->nv, 0ull, c->stream);
   launch_fill_u64(reinterpret_cast<unsigned long long*>(g1), (long long)c->dim * c->nv, 0ull, c->stream);
   if (nh > 0) {
-    const int g = (int)((nh + 255) / 256);
-    if (c->dim == 3)
-      k_toi_adjoint<3><<<g, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, nh, c->d_rec, c->opt.vf_ccd_eta,
-                                                 c->opt.ee_ccd_eta, g0, g1);
-    else
-      k_toi_adjoint<2><<<g, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, nh, c->d_rec, c->opt.vf_ccd_eta,
-                                                 c->opt.ee_ccd_eta, g0, g1);
+    CHECK(c->grow((void**)&c->d_adj_n, &c->cap_adj_n, sizeof(unsigned long long), "adjoint"));
+    if (c->cap_adj_q == 0) {
+      // queue of multi-step hits (rare); an overflow rescans all hits instead
+      CHECK(c->grow((void**)&c->d_adj_q, &c->cap_adj_q_bytes, sizeof(long long) * kAdjQueue, "adjoint"));
+      c->cap_adj_q = kAdjQueue;
+    }
+    launch_fill_u64(c->d_adj_n, 1, 0ull, c->stream);
+    // Visit hits sorted by (kind, first vertex) so that each block's LDS table
+    // merges most of its contributions; the caller's hit order is untouched.
+    int vbits = 1;
+    while (vbits < 31 && (1ll << vbits) < (long long)c->nv) ++vbits;
+    const int kind_shift = vbits;
+    const uint32_t *order = nullptr, *skey = nullptr;
+    if (nh <= (long long)INT32_MAX && vbits <= 30) {
+      const size_t b4 = sizeof(uint32_t) * (size_t)nh;
+      CHECK(c->grow((void**)&c->d_adj_keys, &c->cap_adj_keys, b4, "adjoint"));
+      CHECK(c->grow((void**)&c->d_adj_vals, &c->cap_adj_vals, b4, "adjoint"));
+      CHECK(c->grow((void**)&c->d_adj_ktmp, &c->cap_adj_ktmp, b4, "adjoint"));
+      CHECK(c->grow((void**)&c->d_adj_vtmp, &c->cap_adj_vtmp, b4, "adjoint"));
+      CHECK(c->grow((void**)&c->d_adj_hist, &c->cap_adj_hist, sizeof(uint32_t) * radix_hist_words((int)nh),
+                    "adjoint"));
+      k_adj_keys<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->dim,
+                                                                       kind_shift, c->d_adj_keys, c->d_adj_vals);
+      launch_radix_sort(c->d_adj_keys, c->d_adj_vals, c->d_adj_ktmp, c->d_adj_vtmp, c->d_adj_hist, (int)nh,
+                        vbits + 1, c->stream);
+      order = c->d_adj_vals;
+      skey = c->d_adj_keys;
+    }
+    if (c->dim == 3) {
+      launch_adjoint<3, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1);
+      launch_adjoint<3, false>(c, gw, order, skey, kind_shift, nh, c->opt.ee_ccd_eta, g0, g1);
+    } else {
+      launch_adjoint<2, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1);
+    }
     HIPCHECK(c, hipGetLastError());
+    unsigned long long nq = 0;
+    HIPCHECK(c, hipMemcpyAsync(&nq, c->d_adj_n, sizeof(nq), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    if (nq > 0) {
+      const bool all = nq > (unsigned long long)c->cap_adj_q;
+      const long long n = all ? nh : (long long)nq;
+      const long long* q = all ? nullptr : c->d_adj_q;
+      const unsigned g = (unsigned)((n + ADJ_T - 1) / ADJ_T);
+      if (c->dim == 3) {
+        k_toi_adjoint<3, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                           c->opt.vf_ccd_eta, g0, g1);
+        k_toi_adjoint<3, false><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                            c->opt.ee_ccd_eta, g0, g1);
+      } else {
+        k_toi_adjoint<2, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                           c->opt.vf_ccd_eta, g0, g1);
+      }
+      HIPCHECK(c, hipGetLastError());
+    }
+    c->adj_multi = (long long)nq;
   }
+  c->mark(15);
   if (!device_ptrs) {
     HIPCHECK(c, hipMemcpyAsync(dV0, g0, gbytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(dV1, g1, gbytes, hipMemcpyDeviceToHost, c->stream));
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev[14], c->ev[15]) == hipSuccess) c->times[10] = ms;
   return MCLG_OK;
 }

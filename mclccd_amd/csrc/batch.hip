@@ -95,7 +95,7 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&c->d_order, &c->cap_order, sizeof(uint32_t) * (size_t)np, "order"));
   CHECK(c->grow((void**)&c->d_ktmp, &c->cap_ktmp, sizeof(uint32_t) * (size_t)np, "keys tmp"));
   CHECK(c->grow((void**)&c->d_vtmp, &c->cap_vtmp, sizeof(uint32_t) * (size_t)np, "vals tmp"));
-  CHECK(c->grow((void**)&c->d_hist, &c->cap_hist, sizeof(uint32_t) * 256 * (size_t)radix_tiles(np), "histogram"));
+  CHECK(c->grow((void**)&c->d_hist, &c->cap_hist, sizeof(uint32_t) * radix_hist_words(np), "histogram"));
   CHECK(c->grow((void**)&b->d_khi, &b->cap_khi, sizeof(uint32_t) * (size_t)np, "scene keys"));
   CHECK(c->grow((void**)&b->d_mort, &b->cap_mort, sizeof(uint32_t) * (size_t)np, "morton"));
   CHECK(c->grow((void**)&c->d_nodes, &c->cap_nodes, node * (size_t)(2 * np - 1), "nodes"));

@@ -414,7 +414,7 @@ void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32
 // ---------------------------------------------------------------------------
 // LSD radix sort of (uint32 key, uint32 value), 8-bit digits.
 //   hist:    per-tile digit counts (LDS atomics)      -> hist[digit*ntiles+tile]
-//   scan:    exclusive scan of hist in digit-major order (one block)
+//   scan:    exclusive scan of hist in digit-major order (three phases)
 //   scatter: stable per-tile ranking by wave match (8 ballots per key)
 // ---------------------------------------------------------------------------
 constexpr int RS_T = 256, RS_ROUNDS = 16, RS_TILE = RS_T * RS_ROUNDS;
@@ -433,27 +433,90 @@ __global__ __launch_bounds__(RS_T) void k_radix_hist(const uint32_t* __restrict_
   hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* hist, int total) {
-  __shared__ uint32_t part[1024];
-  const int per = (total + 1023) / 1024;
-  const int b = threadIdx.x * per;
-  const int e = b + per < total ? b + per : total;
-  uint32_t s = 0;
-  for (int i = b; i < e; ++i) s += hist[i];
-  part[threadIdx.x] = s;
+// Exclusive scan of the digit-major histogram in three phases: per-chunk
+// sums, a one-block scan of those, then each chunk scanned in place from its
+// offset.  SC_T threads x SC_PER entries per chunk.
+constexpr int SC_T = 1024, SC_PER = 8, SC_CHUNK = SC_T * SC_PER;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t u = __shfl_up(v, o);
+    if (lane_id() >= o) v += u;
+  }
+  return v;
+}
+
+// Block-wide exclusive scan of one value per thread (SC_T threads); also returns the total.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
+  const int w = threadIdx.x / kWave, nw = SC_T / kWave;
+  const uint32_t incl = wave_incl_scan(v);
+  if (lane_id() == kWave - 1) s_w[w] = incl;
   __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    const uint32_t v = threadIdx.x >= (unsigned)o ? part[threadIdx.x - o] : 0u;
-    __syncthreads();
-    part[threadIdx.x] += v;
+  if (w == 0) {
+    const uint32_t x = lane_id() < nw ? s_w[lane_id()] : 0u;
+    const uint32_t xi = wave_incl_scan(x);
+    if (lane_id() < nw) s_w[lane_id()] = xi - x;
+    if (lane_id() == nw - 1) s_w[nw] = xi;
+  }
+  __syncthreads();
+  total = s_w[nw];
+  return s_w[w] + incl - v;
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_sums(const uint32_t* __restrict__ in, int n,
+                                                    uint32_t* __restrict__ part) {
+  __shared__ uint32_t s_w[SC_T / kWave + 1];
+  const long long base = (long long)blockIdx.x * SC_CHUNK;
+  uint32_t v = 0;
+#pragma unroll
+  for (int r = 0; r < SC_PER; ++r) {
+    const long long i = base + r * SC_T + threadIdx.x;
+    if (i < n) v += in[i];
+  }
+  uint32_t total;
+  (void)block_excl_scan(v, s_w, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_top(uint32_t* part, int nb) {
+  __shared__ uint32_t s_w[SC_T / kWave + 1];
+  uint32_t carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += SC_T) {
+    const int i = b0 + threadIdx.x;
+    const uint32_t v = i < nb ? part[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan(v, s_w, total);
+    if (i < nb) part[i] = carry + ex;
+    carry += total;
     __syncthreads();
   }
-  uint32_t run = part[threadIdx.x] - s;  // exclusive
-  for (int i = b; i < e; ++i) {
-    const uint32_t v = hist[i];
-    hist[i] = run;
-    run += v;
+}
+
+__global__ __launch_bounds__(SC_T) void k_scan_apply(uint32_t* __restrict__ data, int n,
+                                                     const uint32_t* __restrict__ part) {
+  __shared__ uint32_t s_w[SC_T / kWave + 1];
+  const long long base = (long long)blockIdx.x * SC_CHUNK + (long long)threadIdx.x * SC_PER;
+  uint32_t x[SC_PER], sum = 0;
+#pragma unroll
+  for (int r = 0; r < SC_PER; ++r) {
+    x[r] = base + r < n ? data[base + r] : 0u;
+    sum += x[r];
   }
+  uint32_t total;
+  uint32_t run = part[blockIdx.x] + block_excl_scan(sum, s_w, total);
+#pragma unroll
+  for (int r = 0; r < SC_PER; ++r) {
+    if (base + r < n) data[base + r] = run;
+    run += x[r];
+  }
+}
+
+static void launch_scan(uint32_t* hist, int total, uint32_t* part, hipStream_t s) {
+  const int nb = (total + SC_CHUNK - 1) / SC_CHUNK;
+  k_scan_sums<<<nb, SC_T, 0, s>>>(hist, total, part);
+  k_scan_top<<<1, SC_T, 0, s>>>(part, nb);
+  k_scan_apply<<<nb, SC_T, 0, s>>>(hist, total, part);
 }
 
 __global__ __launch_bounds__(RS_T) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -498,6 +561,10 @@ __global__ __launch_bounds__(RS_T) void k_radix_scatter(const uint32_t* __restri
 }
 
 int radix_tiles(int n) { return (n + RS_TILE - 1) / RS_TILE; }
+size_t radix_hist_words(int n) {
+  const size_t h = 256 * (size_t)radix_tiles(n);
+  return h + (h + SC_CHUNK - 1) / SC_CHUNK;  // digit counts, then the scan's chunk sums
+}
 
 // Sorts (keys, vals) in place (ping-pong through tmp buffers); nbits rounded up to 8.
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
@@ -510,7 +577,7 @@ void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t*
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     k_radix_hist<<<tiles, RS_T, 0, s>>>(ki, n, shift, hist);
-    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * tiles);
+    launch_scan(hist, 256 * tiles, hist + 256 * (size_t)tiles, s);
     k_radix_scatter<<<tiles, RS_T, 0, s>>>(ki, vi, ko, vo, n, shift, hist);
     uint32_t* t;
     t = ki; ki = ko; ko = t;

@@ -229,7 +229,7 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_order, &cap_order, sizeof(uint32_t) * (size_t)np, "order"));
   CHECK(grow((void**)&d_ktmp, &cap_ktmp, sizeof(uint32_t) * (size_t)np, "keys tmp"));
   CHECK(grow((void**)&d_vtmp, &cap_vtmp, sizeof(uint32_t) * (size_t)np, "vals tmp"));
-  CHECK(grow((void**)&d_hist, &cap_hist, sizeof(uint32_t) * 256 * (size_t)radix_tiles(np), "histogram"));
+  CHECK(grow((void**)&d_hist, &cap_hist, sizeof(uint32_t) * radix_hist_words(np), "histogram"));
   launch_morton(dim, d_boxes, np, d_cbounds, SceneArgs(), d_keys, d_order, nullptr, stream);
   mark(4);
   launch_radix_sort(d_keys, d_order, d_ktmp, d_vtmp, d_hist, np, 32, stream);
@@ -387,7 +387,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
                   d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
-                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g,
+                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

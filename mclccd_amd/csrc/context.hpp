@@ -9,6 +9,7 @@
 namespace mclg {
 
 constexpr int kNumEvents = 16;
+constexpr size_t kAdjQueue = size_t(1) << 22;  // adjoint multi-step queue entries
 
 struct BatchState;
 
@@ -76,6 +77,12 @@ struct Ctx {
   int* d_minmax = nullptr; size_t cap_minmax = 0;
   double* d_adj_w = nullptr; size_t cap_adj_w = 0;
   double* d_adj_g = nullptr; size_t cap_adj_g = 0;
+  long long* d_adj_q = nullptr; size_t cap_adj_q_bytes = 0; size_t cap_adj_q = 0;  // multi-step hit queue
+  unsigned long long* d_adj_n = nullptr; size_t cap_adj_n = 0;
+  long long adj_multi = 0;  // multi-step hits in the last adjoint
+  uint32_t *d_adj_keys = nullptr, *d_adj_vals = nullptr, *d_adj_ktmp = nullptr, *d_adj_vtmp = nullptr,
+           *d_adj_hist = nullptr;  // adjoint visiting order (radix sort of the hits)
+  size_t cap_adj_keys = 0, cap_adj_vals = 0, cap_adj_ktmp = 0, cap_adj_vtmp = 0, cap_adj_hist = 0;
   double* d_stage_v0 = nullptr; size_t cap_stage_v0 = 0;
   double* d_stage_v1 = nullptr; size_t cap_stage_v1 = 0;
   int* d_stage_p = nullptr; size_t cap_stage_p = 0;

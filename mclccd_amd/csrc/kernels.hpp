@@ -38,6 +38,7 @@ void launch_morton(int dim, const void* boxes, int np, const unsigned long long*
 void launch_scene_keys(const uint32_t* order, int n, const int* prim_scene, uint32_t* hi, hipStream_t s);
 void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32_t* dst, hipStream_t s);
 int radix_tiles(int n);
+size_t radix_hist_words(int n);  // uint32 words of the hist buffer launch_radix_sort needs
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
 int lbvh_frontier_cap(int n);
