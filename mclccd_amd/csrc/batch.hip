@@ -99,6 +99,8 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&b->d_khi, &b->cap_khi, sizeof(uint32_t) * (size_t)np, "scene keys"));
   CHECK(c->grow((void**)&b->d_mort, &b->cap_mort, sizeof(uint32_t) * (size_t)np, "morton"));
   CHECK(c->grow((void**)&c->d_nodes, &c->cap_nodes, node * (size_t)(2 * np - 1), "nodes"));
+  CHECK(c->grow(&c->d_sboxes, &c->cap_sboxes, box * (size_t)np, "sorted leaf boxes"));
+  CHECK(c->grow(&c->d_face, &c->cap_face, face_bytes(dim) * (size_t)np, "face records"));
   const size_t ni = (size_t)std::max(np - 1, 1);
   CHECK(c->grow((void**)&c->d_lchild, &c->cap_lchild, sizeof(int) * ni, "lchild"));
   CHECK(c->grow((void**)&c->d_rchild, &c->cap_rchild, sizeof(int) * ni, "rchild"));
@@ -114,6 +116,10 @@ static int batch_buffers(Ctx* c) {
   if (c->disc_cap == 0) c->disc_cap = std::max<unsigned long long>(1024, (unsigned long long)np);
   if (c->hit_cap == 0) c->hit_cap = std::max<unsigned long long>(1024, 16ull * (unsigned long long)np);
   CHECK(c->grow((void**)&c->d_pairs, &c->cap_pairs_bytes, sizeof(int2) * c->pair_cap, "leaf pairs"));
+  const long long nch = expand_chunks(c->pair_cap) + 1;
+  CHECK(c->grow((void**)&c->d_ex_off, &c->cap_ex_off, sizeof(unsigned long long) * 2 * (size_t)nch, "candidate offsets"));
+  CHECK(c->grow((void**)&c->d_ex_part, &c->cap_ex_part, sizeof(unsigned long long) * (size_t)scan_part_words(nch),
+                "candidate offsets"));
   CHECK(c->grow((void**)&c->d_vfq, &c->cap_vfq, sizeof(int4) * c->vf_cap, "VF stencils"));
   if (dim == 3) CHECK(c->grow((void**)&c->d_eeq, &c->cap_eeq, sizeof(int4) * c->ee_cap, "EE stencils"));
   CHECK(c->grow((void**)&c->d_disc, &c->cap_disc, sizeof(int2) * c->disc_cap, "discrete"));
@@ -139,14 +145,19 @@ static void batch_enqueue(Ctx* c) {
   launch_scene_keys(c->d_order, np, b->d_prim_scene, b->d_khi, s);
   launch_radix_sort(b->d_khi, c->d_order, c->d_ktmp, c->d_vtmp, c->d_hist, np, b->scene_bits, s);
   launch_gather_u32(c->d_order, np, b->d_mort, c->d_keys, s);
-  launch_lbvh(dim, c->d_keys, b->d_khi, c->d_order, np, c->d_boxes, c->d_act, c->d_nodes, c->d_lchild, c->d_rchild,
-              c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
-  launch_traverse(dim, c->d_nodes, c->d_boxes, c->d_act, c->d_order, np, nullptr, c->d_pairs, c->pair_cap, c->d_counters,
-                  b->d_khi, b->d_root, c->tr_waves, s);
+  launch_lbvh(dim, c->d_keys, b->d_khi, c->d_order, np, c->d_boxes, c->d_act, c->d_nodes, c->d_sboxes, c->d_lchild,
+              c->d_rchild, c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
+  launch_face_records(dim, c->d_order, np, b->d_P, c->d_flags, nullptr, c->d_rec, c->d_face, s);
+  launch_traverse(dim, c->d_nodes, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters, b->d_khi, b->d_root,
+                  c->tr_waves, s);
   if (!(o.continuous || o.discrete)) return;
   ExpandLaunch E;
   E.pairs = c->d_pairs; E.npairs_dev = c->d_counters + C_LEAF_PAIRS; E.pair_cap = c->pair_cap; E.npairs_hint = -1;
-  E.P = b->d_P; E.flags = c->d_flags; E.rec = c->d_rec; E.discrete = o.discrete; E.continuous = o.continuous;
+  E.faces = c->d_face; E.discrete = o.discrete; E.continuous = o.continuous;
+  {
+    const long long nch = expand_chunks(c->pair_cap) + 1;
+    E.off_vf = c->d_ex_off; E.off_ee = c->d_ex_off + nch; E.scan_part = c->d_ex_part;
+  }
   E.vfq = c->d_vfq; E.vf_cap = c->vf_cap; E.eeq = c->d_eeq; E.ee_cap = dim == 3 ? c->ee_cap : 0;
   E.disc = c->d_disc; E.disc_cap = c->disc_cap; E.counters = c->d_counters;
   E.num_cus = c->num_cus; E.blocks_per_cu = c->blocks_per_cu; E.chunk = c->chunk;

@@ -433,29 +433,29 @@ __global__ __launch_bounds__(RS_T) void k_radix_hist(const uint32_t* __restrict_
   hist[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram in three phases: per-chunk
-// sums, a one-block scan of those, then each chunk scanned in place from its
-// offset.  SC_T threads x SC_PER entries per chunk.
+// Exclusive scan in three phases: per-chunk sums, a one-block scan of those,
+// then each chunk scanned in place from its offset.  SC_T threads x SC_PER
+// entries per chunk; T is uint32_t (radix histograms) or unsigned long long.
 constexpr int SC_T = 1024, SC_PER = 8, SC_CHUNK = SC_T * SC_PER;
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+template <typename T> __device__ __forceinline__ T wave_incl_scan_t(T v) {
 #pragma unroll
   for (int o = 1; o < kWave; o <<= 1) {
-    const uint32_t u = __shfl_up(v, o);
+    const T u = __shfl_up(v, o);
     if (lane_id() >= o) v += u;
   }
   return v;
 }
 
 // Block-wide exclusive scan of one value per thread (SC_T threads); also returns the total.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
+template <typename T> __device__ __forceinline__ T block_excl_scan(T v, T* s_w, T& total) {
   const int w = threadIdx.x / kWave, nw = SC_T / kWave;
-  const uint32_t incl = wave_incl_scan(v);
+  const T incl = wave_incl_scan_t(v);
   if (lane_id() == kWave - 1) s_w[w] = incl;
   __syncthreads();
   if (w == 0) {
-    const uint32_t x = lane_id() < nw ? s_w[lane_id()] : 0u;
-    const uint32_t xi = wave_incl_scan(x);
+    const T x = lane_id() < nw ? s_w[lane_id()] : T(0);
+    const T xi = wave_incl_scan_t(x);
     if (lane_id() < nw) s_w[lane_id()] = xi - x;
     if (lane_id() == nw - 1) s_w[nw] = xi;
   }
@@ -464,47 +464,47 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* s_w, u
   return s_w[w] + incl - v;
 }
 
-__global__ __launch_bounds__(SC_T) void k_scan_sums(const uint32_t* __restrict__ in, int n,
-                                                    uint32_t* __restrict__ part) {
-  __shared__ uint32_t s_w[SC_T / kWave + 1];
+template <typename T>
+__global__ __launch_bounds__(SC_T) void k_scan_sums(const T* __restrict__ in, long long n, T* __restrict__ part) {
+  __shared__ T s_w[SC_T / kWave + 1];
   const long long base = (long long)blockIdx.x * SC_CHUNK;
-  uint32_t v = 0;
+  T v = 0;
 #pragma unroll
   for (int r = 0; r < SC_PER; ++r) {
     const long long i = base + r * SC_T + threadIdx.x;
     if (i < n) v += in[i];
   }
-  uint32_t total;
+  T total;
   (void)block_excl_scan(v, s_w, total);
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(SC_T) void k_scan_top(uint32_t* part, int nb) {
-  __shared__ uint32_t s_w[SC_T / kWave + 1];
-  uint32_t carry = 0;
-  for (int b0 = 0; b0 < nb; b0 += SC_T) {
-    const int i = b0 + threadIdx.x;
-    const uint32_t v = i < nb ? part[i] : 0u;
-    uint32_t total;
-    const uint32_t ex = block_excl_scan(v, s_w, total);
+template <typename T> __global__ __launch_bounds__(SC_T) void k_scan_top(T* part, long long nb) {
+  __shared__ T s_w[SC_T / kWave + 1];
+  T carry = 0;
+  for (long long b0 = 0; b0 < nb; b0 += SC_T) {
+    const long long i = b0 + threadIdx.x;
+    const T v = i < nb ? part[i] : T(0);
+    T total;
+    const T ex = block_excl_scan(v, s_w, total);
     if (i < nb) part[i] = carry + ex;
     carry += total;
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(SC_T) void k_scan_apply(uint32_t* __restrict__ data, int n,
-                                                     const uint32_t* __restrict__ part) {
-  __shared__ uint32_t s_w[SC_T / kWave + 1];
+template <typename T>
+__global__ __launch_bounds__(SC_T) void k_scan_apply(T* __restrict__ data, long long n, const T* __restrict__ part) {
+  __shared__ T s_w[SC_T / kWave + 1];
   const long long base = (long long)blockIdx.x * SC_CHUNK + (long long)threadIdx.x * SC_PER;
-  uint32_t x[SC_PER], sum = 0;
+  T x[SC_PER], sum = 0;
 #pragma unroll
   for (int r = 0; r < SC_PER; ++r) {
-    x[r] = base + r < n ? data[base + r] : 0u;
+    x[r] = base + r < n ? data[base + r] : T(0);
     sum += x[r];
   }
-  uint32_t total;
-  uint32_t run = part[blockIdx.x] + block_excl_scan(sum, s_w, total);
+  T total;
+  T run = part[blockIdx.x] + block_excl_scan(sum, s_w, total);
 #pragma unroll
   for (int r = 0; r < SC_PER; ++r) {
     if (base + r < n) data[base + r] = run;
@@ -512,11 +512,22 @@ __global__ __launch_bounds__(SC_T) void k_scan_apply(uint32_t* __restrict__ data
   }
 }
 
+template <typename T> static void launch_scan_t(T* data, long long n, T* part, hipStream_t s) {
+  if (n <= 0) return;
+  const unsigned nb = (unsigned)((n + SC_CHUNK - 1) / SC_CHUNK);
+  k_scan_sums<T><<<nb, SC_T, 0, s>>>(data, n, part);
+  k_scan_top<T><<<1, SC_T, 0, s>>>(part, (long long)nb);
+  k_scan_apply<T><<<nb, SC_T, 0, s>>>(data, n, part);
+}
+
 static void launch_scan(uint32_t* hist, int total, uint32_t* part, hipStream_t s) {
-  const int nb = (total + SC_CHUNK - 1) / SC_CHUNK;
-  k_scan_sums<<<nb, SC_T, 0, s>>>(hist, total, part);
-  k_scan_top<<<1, SC_T, 0, s>>>(part, nb);
-  k_scan_apply<<<nb, SC_T, 0, s>>>(hist, total, part);
+  launch_scan_t<uint32_t>(hist, total, part, s);
+}
+
+long long scan_part_words(long long n) { return (n + SC_CHUNK - 1) / SC_CHUNK; }
+
+void launch_scan_u64(unsigned long long* data, long long n, unsigned long long* part, hipStream_t s) {
+  launch_scan_t<unsigned long long>(data, n, part, s);
 }
 
 __global__ __launch_bounds__(RS_T) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
@@ -648,7 +659,8 @@ template <int D>
 __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ hi,
                                                      const uint32_t* __restrict__ order, int n,
                                                      const LeafBox<D>* __restrict__ boxes,
-                                                     const uint8_t* __restrict__ act, NodeF<D>* nodes, int* lchild,
+                                                     const uint8_t* __restrict__ act, NodeF<D>* nodes,
+                                                     LeafBox<D>* __restrict__ sboxes, int* lchild,
                                                      int* rchild, int* rng_l, int* rng_r, int* visits, int4* frontier,
                                                      unsigned long long* nfrontier, SceneArgs S) {
   __shared__ NodeF<D> s_node[2 * LB_T];  // [0, LB_T): leaves, [LB_T, 2 LB_T): internal nodes
@@ -659,7 +671,9 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
   if (t == LB_T - 1 && s < n - 1) visits[s] = 0;  // boundary split: phase B's counter
   if (s < n) {
     const uint32_t prim = order[s];
-    const NodeF<D> L = leaf_node<D>(boxes[prim], (int)prim, act[prim] != 0);
+    const LeafBox<D> lb = boxes[prim];
+    sboxes[s] = lb;
+    const NodeF<D> L = leaf_node<D>(lb, (int)prim, act[prim] != 0);
     nodes[(n - 1) + s] = L;
     s_node[t] = L;
     if (S.root) {
@@ -755,22 +769,24 @@ __global__ void k_lbvh_ropes(int n, NodeF<D>* nodes, const int* __restrict__ rch
 int lbvh_frontier_cap(int n) { return 2 * ((n + LB_T - 1) / LB_T) + 2; }
 
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
-                 const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
-                 int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s) {
+                 const uint8_t* act, void* nodes, void* sboxes, int* lchild, int* rchild, int* rng_l, int* rng_r,
+                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s) {
   if (n < 1) return;
   fill<unsigned long long>(nfrontier, 1, 0ull, s);
   const int g = (n + LB_T - 1) / LB_T;
   const int gf = (lbvh_frontier_cap(n) + 255) / 256;
   const int g2 = (2 * n - 1 + 255) / 256;
   if (dim == 3) {
-    k_lbvh_local<3><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes, lchild,
-                                       rchild, rng_l, rng_r, visits, frontier, nfrontier, S);
+    k_lbvh_local<3><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes,
+                                       (LeafBox<3>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
+                                       nfrontier, S);
     if (n > 1) k_lbvh_cross<3><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<3>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);
     k_lbvh_ropes<3><<<g2, 256, 0, s>>>(n, (NodeF<3>*)nodes, rchild, rng_r);
   } else {
-    k_lbvh_local<2><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes, lchild,
-                                       rchild, rng_l, rng_r, visits, frontier, nfrontier, S);
+    k_lbvh_local<2><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes,
+                                       (LeafBox<2>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
+                                       nfrontier, S);
     if (n > 1) k_lbvh_cross<2><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<2>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);
     k_lbvh_ropes<2><<<g2, 256, 0, s>>>(n, (NodeF<2>*)nodes, rchild, rng_r);

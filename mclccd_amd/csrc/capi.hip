@@ -244,8 +244,9 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_visits, &cap_visits, sizeof(int) * ni, "visits"));
   CHECK(grow((void**)&d_frontier, &cap_frontier, sizeof(int4) * (size_t)lbvh_frontier_cap(np), "frontier"));
   CHECK(grow((void**)&d_nfrontier, &cap_nfrontier, sizeof(unsigned long long), "frontier"));
-  launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits,
-              d_frontier, d_nfrontier, SceneArgs(), stream);
+  CHECK(grow(&d_sboxes, &cap_sboxes, ldbox_bytes(dim) * (size_t)np, "sorted leaf boxes"));
+  launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_sboxes, d_lchild, d_rchild, d_rngl, d_rngr,
+              d_visits, d_frontier, d_nfrontier, SceneArgs(), stream);
   mark(6);
   HIPCHECK(this, hipGetLastError());
   HIPCHECK(this, hipStreamSynchronize(stream));
@@ -278,14 +279,16 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   mark(7);
   CHECK(check_indices(P, (long long)np * dim, nv, "traverse: P"));
   launch_pack(dim, V0, V1, nv, d_rec, stream);
+  CHECK(grow(&d_face, &cap_face, face_bytes(dim) * (size_t)np, "face records"));
+  launch_face_records(dim, d_order, np, P, d_flags, rank_set ? d_rank : nullptr, d_rec, d_face, stream);
+  pairs_are_prims = false;
   CHECK(grow((void**)&d_counters, &cap_counters, sizeof(unsigned long long) * C_COUNT, "counters"));
   launch_fill_u64(d_counters, C_COUNT, 0ull, stream);
   if (pair_cap == 0) pair_cap = std::max<unsigned long long>(1024, 16ull * (unsigned long long)np);
   CHECK(grow((void**)&d_pairs, &cap_pairs_bytes, sizeof(int2) * pair_cap, "leaf pairs"));
   for (;;) {
     mark(8);
-    launch_traverse(dim, d_nodes, d_boxes, d_act, d_order, np, rank_set ? d_rank : nullptr, d_pairs, pair_cap, d_counters,
-                    nullptr, nullptr, tr_waves, stream);
+    launch_traverse(dim, d_nodes, d_sboxes, np, d_pairs, pair_cap, d_counters, nullptr, nullptr, tr_waves, stream);
     mark(9);
     HIPCHECK(this, hipGetLastError());
     CHECK(read_counters());
@@ -310,7 +313,12 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       launch_fill_u64(d_counters + C_CAND_VF, C_COUNT - C_CAND_VF, 0ull, stream);
       ExpandLaunch E;
       E.pairs = d_pairs; E.npairs_dev = d_counters + C_LEAF_PAIRS; E.pair_cap = pair_cap; E.npairs_hint = npairs;
-      E.P = P; E.flags = d_flags; E.rec = d_rec; E.blocks_per_cu = blocks_per_cu; E.chunk = chunk;
+      E.faces = d_face; E.blocks_per_cu = blocks_per_cu; E.chunk = chunk;
+      const long long nch = expand_chunks(pair_cap) + 1;
+      CHECK(grow((void**)&d_ex_off, &cap_ex_off, sizeof(unsigned long long) * 2 * (size_t)nch, "candidate offsets"));
+      CHECK(grow((void**)&d_ex_part, &cap_ex_part, sizeof(unsigned long long) * (size_t)scan_part_words(nch),
+                 "candidate offsets"));
+      E.off_vf = d_ex_off; E.off_ee = d_ex_off + nch; E.scan_part = d_ex_part;
       E.discrete = opt.discrete; E.continuous = opt.continuous;
       E.vfq = d_vfq; E.vf_cap = vf_cap; E.eeq = d_eeq; E.ee_cap = dim == 3 ? ee_cap : 0;
       E.disc = d_disc; E.disc_cap = disc_cap; E.counters = d_counters; E.num_cus = num_cus;
@@ -365,6 +373,11 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       counts.dist_evals = (int64_t)h_counters[C_DIST_EVALS];
     }
   }
+  if (keep_pairs && npairs > 0) {  // sorted positions -> primitive pairs, lower rank first
+    launch_pairs_to_prims(dim, d_pairs, npairs, d_face, stream);
+    HIPCHECK(this, hipGetLastError());
+    pairs_are_prims = true;
+  }
   HIPCHECK(this, hipStreamSynchronize(stream));
   float ms = 0;
   (void)hipEventElapsedTime(&ms, ev[7], ev[8]);
@@ -386,7 +399,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
-                  d_vtmp, d_hist, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
+                  d_vtmp, d_hist, d_sboxes, d_face, d_ex_off, d_ex_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)
@@ -512,6 +525,8 @@ int mclg_copy_leaf_pairs(mclg_ctx* h, int* pairs2, int64_t cap) {
   if (!c) return MCLG_ERR_ARG;
   const int64_t n = c->counts.leaf_pairs;
   if (cap < n) return c->fail(MCLG_ERR_CAPACITY, "copy_leaf_pairs: cap < count");
+  if (n > 0 && !c->pairs_are_prims)
+    return c->fail(MCLG_ERR_STATE, "copy_leaf_pairs: the last traverse did not keep them (MCLG_KEEP_LEAF_PAIRS)");
   if (pairs2) CHECK(copy_out(c, pairs2, c->d_pairs, sizeof(int2) * (size_t)n));
   return MCLG_OK;
 }
@@ -540,7 +555,7 @@ int mclg_get_device_results(const mclg_ctx* h, mclg_device_results* out) {
   out->ee_sten4 = (const int*)c->d_eeq;
   out->ee_toi = c->keep_cand ? c->d_ee_toi : nullptr;
   out->n_ee = c->n_ee;
-  out->leaf_pairs2 = (const int*)c->d_pairs;
+  out->leaf_pairs2 = c->pairs_are_prims ? (const int*)c->d_pairs : nullptr;
   return MCLG_OK;
 }
 

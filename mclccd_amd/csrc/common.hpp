@@ -36,6 +36,45 @@ template <> struct alignas(16) NodeF<2> {
   int pad[2];
 };
 
+// Per-face record in sorted (Morton) order, rebuilt by every traverse from
+// its P, the reptri flags and V1: the candidate kernel's one gather per face
+// (a single 128 B line in 3D).  rank orients pairs (lower rank first); it is
+// the primitive index unless a rank was set.
+template <int D> struct FaceRec;
+template <> struct alignas(128) FaceRec<3> {
+  int p[3];
+  uint32_t flags;
+  int prim, rank, pad[2];
+  double v1[9];
+  double pad2[3];
+};
+template <> struct alignas(64) FaceRec<2> {
+  int p[2];
+  uint32_t flags;
+  int prim, rank, pad[3];
+  double v1[4];
+};
+
+// The used prefix of a FaceRec (header + V1), loaded as one unit.
+template <int D> struct FaceHead;
+template <> struct FaceHead<3> {
+  int p[3];
+  uint32_t flags;
+  int prim, rank, pad[2];
+  double v1[9];
+  double pad2;
+};
+template <> struct FaceHead<2> {
+  int p[2];
+  uint32_t flags;
+  int prim, rank, pad[3];
+  double v1[4];
+};
+static_assert(sizeof(FaceHead<3>) == 112 && sizeof(FaceHead<2>) == 64, "face head layout");
+
+inline size_t face_bytes(int dim) { return dim == 3 ? sizeof(FaceRec<3>) : sizeof(FaceRec<2>); }
+static_assert(sizeof(FaceRec<3>) == 128 && sizeof(FaceRec<2>) == 64, "face record layout");
+
 // Device counters of one traversal (all 64-bit).
 enum Counter : int {
   C_LEAF_PAIRS = 0,
@@ -65,6 +104,17 @@ __device__ __forceinline__ long long wave_append(bool pred, unsigned long long* 
   if (lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
   base = __shfl(base, leader);
   return pred ? (long long)(base + popc_below(m)) : -1;
+}
+
+// Exclusive prefix sum of v over the wave's lanes (inactive lanes count 0).
+__device__ __forceinline__ uint32_t wave_prefix_u32(uint32_t v) {
+  uint32_t incl = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t u = __shfl_up(incl, o);
+    if (lane_id() >= o) incl += u;
+  }
+  return incl - v;
 }
 
 // Order-preserving map of a double onto uint64 (for atomic min/max).

@@ -42,6 +42,11 @@ struct Ctx {
   unsigned long long* d_ekeys = nullptr; size_t cap_ekeys = 0;
   int* d_evals = nullptr; size_t cap_evals = 0;
   void* d_boxes = nullptr; size_t cap_boxes = 0;
+  void* d_sboxes = nullptr; size_t cap_sboxes = 0;  // FP64 leaf boxes in sorted order
+  void* d_face = nullptr; size_t cap_face = 0;      // FaceRec per sorted leaf (rebuilt per traverse)
+  bool pairs_are_prims = false;                     // d_pairs converted to primitive pairs
+  unsigned long long* d_ex_off = nullptr; size_t cap_ex_off = 0;    // candidate offsets per 64-pair chunk
+  unsigned long long* d_ex_part = nullptr; size_t cap_ex_part = 0;  // their scan's chunk sums
   uint8_t* d_act = nullptr; size_t cap_act = 0;
   unsigned long long* d_cbounds = nullptr; size_t cap_cbounds = 0;
   double* d_partial = nullptr; size_t cap_partial = 0;

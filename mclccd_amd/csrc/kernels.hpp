@@ -38,16 +38,27 @@ void launch_morton(int dim, const void* boxes, int np, const unsigned long long*
 void launch_scene_keys(const uint32_t* order, int n, const int* prim_scene, uint32_t* hi, hipStream_t s);
 void launch_gather_u32(const uint32_t* order, int n, const uint32_t* src, uint32_t* dst, hipStream_t s);
 int radix_tiles(int n);
-size_t radix_hist_words(int n);  // uint32 words of the hist buffer launch_radix_sort needs
+size_t radix_hist_words(int n);
+// In-place exclusive scan of n 64-bit values; part needs scan_part_words(n) entries.
+long long scan_part_words(long long n);
+long long expand_chunks(unsigned long long pair_cap);  // 64-pair chunks of the candidate kernels
+void launch_scan_u64(unsigned long long* data, long long n, unsigned long long* part, hipStream_t s);  // uint32 words of the hist buffer launch_radix_sort needs
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
 int lbvh_frontier_cap(int n);
+// Also writes sboxes[s] = boxes[order[s]] (FP64 leaf boxes in sorted order).
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
-                 const uint8_t* act, void* nodes, int* lchild, int* rchild, int* rng_l, int* rng_r, int* visits,
-                 int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s);
-void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
-                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters,
-                     const uint32_t* scene_of_sorted, const int* scene_root, int waves, hipStream_t s);
+                 const uint8_t* act, void* nodes, void* sboxes, int* lchild, int* rchild, int* rng_l, int* rng_r,
+                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s);
+// Leaf pairs as sorted positions (s, k), s < k.
+void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
+                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_root, int waves,
+                     hipStream_t s);
+// faces[s] for sorted position s (prim = order[s]); rank may be null.
+void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
+                         const double* rec, void* faces, hipStream_t s);
+// pairs[i] (sorted positions) -> primitive pairs, lower rank first.
+void launch_pairs_to_prims(int dim, int2* pairs, long long n, const void* faces, hipStream_t s);
 
 // Kernels that follow a variable-size queue read its length from the device
 // counter (so a captured graph needs no host round trip); the *_hint sizes
@@ -57,9 +68,11 @@ struct ExpandLaunch {
   const unsigned long long* npairs_dev;
   unsigned long long pair_cap;
   long long npairs_hint;
-  const int* P;
-  const uint8_t* flags;
-  const double* rec;
+  const void* faces;  // FaceRec<dim> in sorted order (pairs hold sorted positions)
+  // scratch of expand_chunks(pair_cap) + 1 entries each, and scan_part_words of that
+  unsigned long long* off_vf;
+  unsigned long long* off_ee;
+  unsigned long long* scan_part;
   int discrete, continuous;
   int4* vfq;
   unsigned long long vf_cap;

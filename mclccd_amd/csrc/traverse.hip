@@ -4,13 +4,13 @@
 //   from escape(leaf s), i.e. over exactly the subtrees covering leaves s+1..n-1,
 //   so every unordered leaf pair is visited once.  Internal nodes are culled with
 //   the FP32 outward-rounded boxes; at a leaf the reference's exact FP64 test
-//   (BVHTree.cpp:401-426) decides.  Pairs are appended to a device queue with
-//   wave-aggregated atomics.
-// k_narrow: one wave takes 64 leaf pairs, runs the discrete test per pair (one
-//   lane each), expands the representative-triangle candidates
-//   (BVHTree.cpp:305-398) into per-wave LDS lists split by kind (VF / EE), then
-//   runs FP64 ACCD on 64 stencils at a time so that each SIMD step executes a
-//   single distance kernel.
+//   (BVHTree.cpp:401-426) decides, on FP64 boxes kept in sorted order.  Pairs
+//   (sorted positions) are staged per wave in LDS and appended in batches.
+// k_face_records: per sorted leaf, P, reptri flags, rank and V1 in one record.
+// k_expand: one lane per leaf pair: orientation, the discrete test and the
+//   representative-triangle candidates (BVHTree.cpp:252-398) into the VF / EE
+//   stencil queues.
+// k_accd: one lane per stencil, FP64 ACCD (NarrowPhase.cpp:634-711).
 #include "geom.hpp"
 #include "kernels.hpp"
 
@@ -27,42 +27,61 @@ __device__ __forceinline__ void wave_lds_sync() {
 // ---------------------------------------------------------------------------
 // Leaf-pair tests
 // ---------------------------------------------------------------------------
+// The tests below are branch-free: every field is used unconditionally, so a
+// node's or box's loads are all issued at once instead of one dependent load
+// per early exit.
 template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const NodeF<D>& b) {
-  if (((q.meta | b.meta) & 0x80000000u) == 0) return false;
+  bool ok = ((q.meta | b.meta) & 0x80000000u) != 0;
+#pragma unroll
+  for (int k = 0; k < D; ++k) ok &= (q.sw_lo[k] <= b.sw_hi[k]) & (b.sw_lo[k] <= q.sw_hi[k]);
 #pragma unroll
   for (int k = 0; k < D; ++k)
-    if (!(q.sw_lo[k] <= b.sw_hi[k]) || !(b.sw_lo[k] <= q.sw_hi[k])) return false;
-#pragma unroll
-  for (int k = 0; k < D; ++k)
-    if ((q.t0_lo[k] > b.t0_hi[k] && q.t1_lo[k] > b.t1_hi[k]) || (q.t0_hi[k] < b.t0_lo[k] && q.t1_hi[k] < b.t1_lo[k]))
-      return false;
-  return true;
+    ok &= !(((q.t0_lo[k] > b.t0_hi[k]) & (q.t1_lo[k] > b.t1_hi[k])) |
+            ((q.t0_hi[k] < b.t0_lo[k]) & (q.t1_hi[k] < b.t1_lo[k])));
+  return ok;
 }
 
 template <int D> __device__ __forceinline__ bool empty_box(const double* lo, const double* hi) {
+  bool e = false;
 #pragma unroll
-  for (int k = 0; k < D; ++k)
-    if (lo[k] > hi[k]) return true;
-  return false;
+  for (int k = 0; k < D; ++k) e |= lo[k] > hi[k];
+  return e;
 }
 
-// BVHTree.cpp:401-426 on two leaves (symmetric in l and r)
+// BVHTree.cpp:401-426 on two leaves (symmetric in l and r): false if both
+// inactive or the swept boxes are disjoint; true if a t0 or t1 box is empty;
+// else false iff some axis separates both at t0 and at t1.
 template <int D>
 __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const LeafBox<D>& r, bool ra) {
-  if (!la && !ra) return false;
+  bool ov = la | ra;
+#pragma unroll
+  for (int k = 0; k < D; ++k) ov &= (l.sw_lo[k] <= r.sw_hi[k]) & (r.sw_lo[k] <= l.sw_hi[k]);
+  const bool emp = empty_box<D>(l.t0_lo, l.t0_hi) | empty_box<D>(r.t0_lo, r.t0_hi) |
+                   empty_box<D>(l.t1_lo, l.t1_hi) | empty_box<D>(r.t1_lo, r.t1_hi);
+  bool sep = false;
 #pragma unroll
   for (int k = 0; k < D; ++k)
-    if (!(l.sw_lo[k] <= r.sw_hi[k])) return false;
+    sep |= ((l.t0_lo[k] > r.t0_hi[k]) & (l.t1_lo[k] > r.t1_hi[k])) | ((l.t0_hi[k] < r.t0_lo[k]) & (l.t1_hi[k] < r.t1_lo[k]));
+  return ov & (emp | !sep);
+}
+
+// Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
+// PIN the loaded registers are consumed by an empty asm statement (no memory
+// access in it), so the compiler cannot split off and delay any part.
+template <typename T, bool PIN = false> __device__ __forceinline__ T load16(const T* p) {
+  static_assert(sizeof(T) % 16 == 0, "16-byte multiple");
+  int4 raw[sizeof(T) / 16];
+  const int4* s = reinterpret_cast<const int4*>(p);
 #pragma unroll
-  for (int k = 0; k < D; ++k)
-    if (!(r.sw_lo[k] <= l.sw_hi[k])) return false;
-  if (empty_box<D>(l.t0_lo, l.t0_hi) || empty_box<D>(r.t0_lo, r.t0_hi)) return true;
-  if (empty_box<D>(l.t1_lo, l.t1_hi) || empty_box<D>(r.t1_lo, r.t1_hi)) return true;
+  for (int i = 0; i < (int)(sizeof(T) / 16); ++i) raw[i] = s[i];
+  if (PIN) {
 #pragma unroll
-  for (int k = 0; k < D; ++k)
-    if ((l.t0_lo[k] > r.t0_hi[k] && l.t1_lo[k] > r.t1_hi[k]) || (l.t0_hi[k] < r.t0_lo[k] && l.t1_hi[k] < r.t1_lo[k]))
-      return false;
-  return true;
+    for (int i = 0; i < (int)(sizeof(T) / 16); ++i)
+      asm volatile("" : : "v"(raw[i].x), "v"(raw[i].y), "v"(raw[i].z), "v"(raw[i].w));
+  }
+  T v;
+  __builtin_memcpy(&v, raw, sizeof(T));
+  return v;
 }
 
 // Pairs found by a wave are staged in LDS and written out in batches: one
@@ -84,11 +103,9 @@ __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __re
 
 template <int D>
 __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D>* __restrict__ nodes,
-                                                               const LeafBox<D>* __restrict__ boxes,
-                                                               const uint8_t* __restrict__ act,
-                                                               const uint32_t* __restrict__ order, int n,
-                                                               const int* __restrict__ rank, int2* __restrict__ pairs,
-                                                               unsigned long long cap, unsigned long long* counters,
+                                                               const LeafBox<D>* __restrict__ sboxes, int n,
+                                                               int2* __restrict__ pairs, unsigned long long cap,
+                                                               unsigned long long* counters,
                                                                const uint32_t* __restrict__ scene_of_sorted,
                                                                const int* __restrict__ scene_root) {
   __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
@@ -101,14 +118,11 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
   NodeF<D> q;
   LeafBox<D> qb;
   bool qa = false;
-  int a = 0, ra = 0;
   int node = -1, end = -1;
   if (valid) {
-    q = nodes[(n - 1) + s];
-    a = (int)order[s];
-    qb = boxes[a];
-    qa = act[a] != 0;
-    ra = rank ? rank[a] : a;
+    q = load16(nodes + (n - 1) + s);
+    qb = load16(sboxes + s);
+    qa = (q.meta >> 31) != 0;
     node = (int)(q.meta & 0x7fffffffu) - 1;
     // several scenes: the walk ends where the pre-order leaves this leaf's scene subtree
     if (scene_root) end = (int)(nodes[scene_root[scene_of_sorted[s]]].meta & 0x7fffffffu) - 1;
@@ -122,15 +136,15 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
       atomicAdd(counters + C_ERRORS, 1ull);
       break;
     }
-    const NodeF<D> nd = nodes[node];
+    const NodeF<D> nd = load16<NodeF<D>, true>(nodes + node);
     bool emit = false;
-    int b = 0;
+    int k = 0;
     int next = (int)(nd.meta & 0x7fffffffu) - 1;
     if (cull_f32<D>(q, nd)) {
       if (node >= n - 1) {
-        b = nd.child;
-        const LeafBox<D> bb = boxes[b];
-        emit = leaf_test<D>(qb, qa, bb, act[b] != 0);
+        k = node - (n - 1);
+        const LeafBox<D> bb = load16(sboxes + k);
+        emit = leaf_test<D>(qb, qa, bb, (nd.meta >> 31) != 0);
       } else {
         next = nd.child;
       }
@@ -139,10 +153,7 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
     const uint64_t m = __ballot(emit);
     if (m) {
       const int cnt = s_cnt[w];
-      if (emit) {
-        const int rb = rank ? rank[b] : b;
-        s_buf[w][cnt + popc_below(m)] = ra < rb ? make_int2(a, b) : make_int2(b, a);
-      }
+      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(s, k);
       const int nc = cnt + __popcll(m);
       wave_lds_sync();
       if (nc > TR_BUF - kWave) {
@@ -161,18 +172,69 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
   if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
 }
 
-void launch_traverse(int dim, const void* nodes, const void* boxes, const uint8_t* act, const uint32_t* order, int n,
-                     const int* rank, int2* pairs, unsigned long long cap, unsigned long long* counters,
-                     const uint32_t* scene_of_sorted, const int* scene_root, int waves, hipStream_t s) {
+void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
+                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_root, int waves,
+                     hipStream_t s) {
   if (n < 2) return;
   waves = waves < 1 ? 4 : (waves > TR_MAX_WAVES ? TR_MAX_WAVES : waves);
   const int g = xcd_grid(n - 1, waves * kWave);
   if (dim == 3)
-    k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)boxes, act, order, n, rank,
-                                                 pairs, cap, counters, scene_of_sorted, scene_root);
+    k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)sboxes, n, pairs, cap,
+                                                 counters, scene_of_sorted, scene_root);
   else
-    k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)boxes, act, order, n, rank,
-                                                 pairs, cap, counters, scene_of_sorted, scene_root);
+    k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)sboxes, n, pairs, cap,
+                                                 counters, scene_of_sorted, scene_root);
+}
+
+// ---------------------------------------------------------------------------
+// Face records and pair conversion
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ void k_face_records(const uint32_t* __restrict__ order, int n, const int* __restrict__ P,
+                               const uint8_t* __restrict__ flags, const int* __restrict__ rank,
+                               const double* __restrict__ rec, FaceRec<D>* __restrict__ faces) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int prim = (int)order[s];
+  FaceRec<D> f = {};
+#pragma unroll
+  for (int j = 0; j < D; ++j) f.p[j] = P[(size_t)prim * D + j];
+  f.flags = flags[prim];
+  f.prim = prim;
+  f.rank = rank ? rank[prim] : prim;
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    const typename VLoad<D>::V v = VLoad<D>::load_t1(rec, f.p[j]);
+    f.v1[D * j + 0] = v.x;
+    f.v1[D * j + 1] = v.y;
+    if constexpr (D == 3) f.v1[D * j + 2] = v.z;
+  }
+  faces[s] = f;
+}
+
+void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
+                         const double* rec, void* faces, hipStream_t s) {
+  if (n < 1) return;
+  const int g = (n + 255) / 256;
+  if (dim == 3) k_face_records<3><<<g, 256, 0, s>>>(order, n, P, flags, rank, rec, (FaceRec<3>*)faces);
+  else k_face_records<2><<<g, 256, 0, s>>>(order, n, P, flags, rank, rec, (FaceRec<2>*)faces);
+}
+
+template <int D>
+__global__ void k_pairs_to_prims(int2* pairs, long long n, const FaceRec<D>* __restrict__ faces) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int2 pr = pairs[i];
+  const int ra = faces[pr.x].rank, rb = faces[pr.y].rank;
+  const int a = faces[pr.x].prim, b = faces[pr.y].prim;
+  pairs[i] = ra < rb ? make_int2(a, b) : make_int2(b, a);
+}
+
+void launch_pairs_to_prims(int dim, int2* pairs, long long n, const void* faces, hipStream_t s) {
+  if (n < 1) return;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (dim == 3) k_pairs_to_prims<3><<<g, 256, 0, s>>>(pairs, n, (const FaceRec<3>*)faces);
+  else k_pairs_to_prims<2><<<g, 256, 0, s>>>(pairs, n, (const FaceRec<2>*)faces);
 }
 
 // ---------------------------------------------------------------------------
@@ -214,12 +276,10 @@ __device__ __forceinline__ void stage_push(T* buf, int& cnt, bool pred, const T&
 // representative-triangle candidates, appended to a VF and an EE stencil queue.
 // ---------------------------------------------------------------------------
 struct ExpandArgs {
-  const int2* pairs;
+  const int2* pairs;                     // sorted positions
   const unsigned long long* npairs_dev;  // pairs found by the traversal (device counter)
   unsigned long long pair_cap;
-  const int* P;
-  const uint8_t* flags;
-  const double* rec;
+  const void* faces;                     // FaceRec<D> by sorted position
   int discrete, continuous;
   int4* vfq;
   unsigned long long vf_cap;
@@ -228,17 +288,106 @@ struct ExpandArgs {
   int2* disc;
   unsigned long long disc_cap;
   unsigned long long* counters;
+  // per 64-pair chunk: VF / EE candidate counts, then (scanned) output offsets;
+  // entry nchunks_cap holds the totals after the scan
+  unsigned long long* off_vf;
+  unsigned long long* off_ee;
+  long long nchunks_cap;
   int chunk;  // consecutive 64-item chunks a wave takes per grid-stride step
 };
 
-constexpr int EX_WAVES = 4, EX_STAGE = 256, EX_DSTAGE = 128;
+constexpr int EX_WAVES = 4, EX_DSTAGE = 128;
 
+// The pair's two faces, lower rank first (BVHTree.cpp:252-283 visits each leaf
+// pair once in the KdBVH's order; the rank reproduces that orientation).
+template <int D>
+__device__ __forceinline__ void load_pair(const ExpandArgs& A, long long i, FaceHead<D>& ha, FaceHead<D>& hb) {
+  const FaceRec<D>* F = static_cast<const FaceRec<D>*>(A.faces);
+  const int2 sp = A.pairs[i];
+  ha = load16(reinterpret_cast<const FaceHead<D>*>(F + sp.x));  // both records at once
+  hb = load16(reinterpret_cast<const FaceHead<D>*>(F + sp.y));
+  if (!(ha.rank < hb.rank)) {
+    const FaceHead<D> t = ha;
+    ha = hb;
+    hb = t;
+  }
+}
+
+// BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (3i + j)
+template <int D>
+__device__ __forceinline__ void candidate_masks(const FaceHead<D>& ha, const FaceHead<D>& hb, uint32_t& vfm,
+                                                uint32_t& eem) {
+  vfm = 0;
+  eem = 0;
+#pragma unroll
+  for (int x = 0; x < D; ++x) {
+    bool sh0 = false, sh1 = false;
+#pragma unroll
+    for (int y = 0; y < D; ++y) { sh0 |= ha.p[x] == hb.p[y]; sh1 |= hb.p[x] == ha.p[y]; }
+    if (((ha.flags >> x) & 1u) && !sh0) vfm |= 1u << x;
+    if (((hb.flags >> x) & 1u) && !sh1) vfm |= 1u << (3 + x);
+  }
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+#pragma unroll
+      for (int y = 0; y < 3; ++y) {
+        const int a0 = ha.p[x], a1 = ha.p[(x + 1) % 3], b0 = hb.p[y], b1 = hb.p[(y + 1) % 3];
+        const bool ok = ((ha.flags >> (3 + x)) & 1u) && ((hb.flags >> (3 + y)) & 1u) && a0 != b0 && a0 != b1 &&
+                        a1 != b0 && a1 != b1;
+        if (ok) eem |= 1u << (3 * x + y);
+      }
+    }
+  }
+}
+
+// Pass 1: candidate counts per 64-pair chunk (zero past the last pair).
+template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand_count(ExpandArgs A) {
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
+  const long long nchunks = (npairs + kWave - 1) / kWave;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    A.off_vf[A.nchunks_cap] = 0;
+    A.off_ee[A.nchunks_cap] = 0;
+  }
+  const long long stride = (long long)gridDim.x * EX_WAVES;
+  for (long long ch = (long long)xcd_block() * EX_WAVES + w; ch < A.nchunks_cap; ch += stride) {
+    uint32_t cv = 0, ce = 0;
+    if (ch < nchunks && A.continuous) {
+      const long long i = ch * kWave + lane;
+      if (i < npairs) {
+        FaceHead<D> ha, hb;
+        load_pair<D>(A, i, ha, hb);
+        uint32_t vfm, eem;
+        candidate_masks<D>(ha, hb, vfm, eem);
+        cv = (uint32_t)__popc(vfm);
+        ce = (uint32_t)__popc(eem);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        cv += __shfl_xor(cv, o);
+        ce += __shfl_xor(ce, o);
+      }
+    }
+    if (lane == 0) {
+      A.off_vf[ch] = cv;
+      A.off_ee[ch] = ce;
+    }
+  }
+}
+
+__global__ void k_expand_totals(const unsigned long long* off_vf, const unsigned long long* off_ee, long long nch,
+                                unsigned long long* counters) {
+  counters[C_CAND_VF] = off_vf[nch];
+  counters[C_CAND_EE] = off_ee[nch];
+}
+
+// Pass 2: the discrete test and the candidates, written at the scanned offsets
+// (each chunk's stencils in lane order: the queues are deterministic).
 template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
-  __shared__ int4 s_vf[EX_WAVES][EX_STAGE];
-  __shared__ int4 s_ee[EX_WAVES][D == 3 ? EX_STAGE : 1];
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
   const int w = threadIdx.x / kWave, lane = lane_id();
-  int nvf = 0, nee = 0, nd = 0;  // staged records (wave-uniform)
+  int nd = 0;  // staged discrete records (wave-uniform)
   const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
   const long long nchunks = (npairs + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * EX_WAVES * A.chunk;
@@ -246,98 +395,69 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
   for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
     const long long i = ch * kWave + lane;
     const bool valid = i < npairs;
-    int f0[3] = {-1, -1, -1}, f1[3] = {-2, -2, -2};
-    int2 pr = make_int2(0, 0);
-    uint32_t fa = 0, fb = 0;
-    if (valid) {
-      pr = A.pairs[i];
-#pragma unroll
-      for (int j = 0; j < D; ++j) { f0[j] = A.P[(size_t)pr.x * D + j]; f1[j] = A.P[(size_t)pr.y * D + j]; }
-      fa = A.flags[pr.x];
-      fb = A.flags[pr.y];
-    }
+    FaceHead<D> ha = {}, hb = {};
+    if (valid) load_pair<D>(A, i, ha, hb);
     if (A.discrete) {  // BVHTree.cpp:464-500
       bool shared = false;
 #pragma unroll
       for (int x = 0; x < D; ++x)
 #pragma unroll
-        for (int y = 0; y < D; ++y) shared |= f0[x] == f1[y];
+        for (int y = 0; y < D; ++y) shared |= ha.p[x] == hb.p[y];
       int dh = 0;
       if (valid && !shared) {
+        const double* a = ha.v1;
+        const double* b = hb.v1;
         if constexpr (D == 3) {
-          dh = tri_tri3(VLoad<3>::load_t1(A.rec, f0[0]), VLoad<3>::load_t1(A.rec, f0[1]), VLoad<3>::load_t1(A.rec, f0[2]),
-                        VLoad<3>::load_t1(A.rec, f1[0]), VLoad<3>::load_t1(A.rec, f1[1]), VLoad<3>::load_t1(A.rec, f1[2]));
+          dh = tri_tri3(d3{a[0], a[1], a[2]}, d3{a[3], a[4], a[5]}, d3{a[6], a[7], a[8]}, d3{b[0], b[1], b[2]},
+                        d3{b[3], b[4], b[5]}, d3{b[6], b[7], b[8]});
         } else {
-          dh = edge_edge2(VLoad<2>::load_t1(A.rec, f0[0]), VLoad<2>::load_t1(A.rec, f0[1]), VLoad<2>::load_t1(A.rec, f1[0]),
-                          VLoad<2>::load_t1(A.rec, f1[1]));
+          dh = edge_edge2(d2{a[0], a[1]}, d2{a[2], a[3]}, d2{b[0], b[1]}, d2{b[2], b[3]});
         }
       }
-      stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, pr, A.disc, A.disc_cap, A.counters + C_DISCRETE);
+      stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, make_int2(ha.prim, hb.prim), A.disc, A.disc_cap,
+                                  A.counters + C_DISCRETE);
     }
     if (!A.continuous) continue;
-    // BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (6 + 3i + j)
     uint32_t vfm = 0, eem = 0;
-    if (valid) {
-#pragma unroll
-      for (int x = 0; x < D; ++x) {
-        bool sh0 = false, sh1 = false;
-#pragma unroll
-        for (int y = 0; y < D; ++y) { sh0 |= f0[x] == f1[y]; sh1 |= f1[x] == f0[y]; }
-        if (((fa >> x) & 1u) && !sh0) vfm |= 1u << x;
-        if (((fb >> x) & 1u) && !sh1) vfm |= 1u << (3 + x);
-      }
-      if constexpr (D == 3) {
-#pragma unroll
-        for (int x = 0; x < 3; ++x) {
-#pragma unroll
-          for (int y = 0; y < 3; ++y) {
-            const int a0 = f0[x], a1 = f0[(x + 1) % 3], b0 = f1[y], b1 = f1[(y + 1) % 3];
-            const bool ok = ((fa >> (3 + x)) & 1u) && ((fb >> (3 + y)) & 1u) && a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1;
-            if (ok) eem |= 1u << (3 * x + y);
-          }
-        }
-      }
-    }
-    // push one code per lane per round; rounds = the wave's largest count
-    for (;;) {
-      const bool has = vfm != 0;
-      if (__ballot(has) == 0) break;
-      int4 st = make_int4(-1, -1, -1, -1);
-      if (has) {
+    if (valid) candidate_masks<D>(ha, hb, vfm, eem);
+    // vertex ids as plain values (a select between two arrays would put them in scratch)
+    const int x0 = ha.p[0], x1 = ha.p[1], x2 = ha.p[D - 1], y0 = hb.p[0], y1 = hb.p[1], y2 = hb.p[D - 1];
+    // VF: the lane's k-th candidate goes to off_vf[ch] + (lanes before) + k
+    unsigned long long pos = A.off_vf[ch] + wave_prefix_u32((uint32_t)__popc(vfm));
+    while (__ballot(vfm != 0)) {
+      if (vfm) {
         const int code = __ffs(vfm) - 1;
         vfm &= vfm - 1;
         const bool fwd = code < 3;  // vertex of f0 against face f1, else the reverse
         const int k = fwd ? code : code - 3;
-        const int a0 = fwd ? f0[0] : f1[0], a1 = fwd ? f0[1] : f1[1], a2 = fwd ? f0[2] : f1[2];
-        st.x = k == 0 ? a0 : (k == 1 ? a1 : a2);
-        st.y = fwd ? f1[0] : f0[0];
-        st.z = fwd ? f1[1] : f0[1];
-        st.w = D == 3 ? (fwd ? f1[2] : f0[2]) : -1;
+        int4 st;
+        st.x = fwd ? (k == 0 ? x0 : (k == 1 ? x1 : x2)) : (k == 0 ? y0 : (k == 1 ? y1 : y2));
+        st.y = fwd ? y0 : x0;
+        st.z = fwd ? y1 : x1;
+        st.w = D == 3 ? (fwd ? y2 : x2) : -1;
+        if (pos < A.vf_cap) A.vfq[pos] = st;
+        ++pos;
       }
-      stage_push<int4, EX_STAGE>(s_vf[w], nvf, has, st, A.vfq, A.vf_cap, A.counters + C_CAND_VF);
     }
     if constexpr (D == 3) {
-      for (;;) {
-        const bool has = eem != 0;
-        if (__ballot(has) == 0) break;
-        int4 st = make_int4(-1, -1, -1, -1);
-        if (has) {
+      pos = A.off_ee[ch] + wave_prefix_u32((uint32_t)__popc(eem));
+      while (__ballot(eem != 0)) {
+        if (eem) {
           const int c = __ffs(eem) - 1;
           eem &= eem - 1;
           const int x = c / 3, y = c - 3 * (c / 3);
-          st.x = x == 0 ? f0[0] : (x == 1 ? f0[1] : f0[2]);
-          st.y = x == 0 ? f0[1] : (x == 1 ? f0[2] : f0[0]);
-          st.z = y == 0 ? f1[0] : (y == 1 ? f1[1] : f1[2]);
-          st.w = y == 0 ? f1[1] : (y == 1 ? f1[2] : f1[0]);
+          int4 st;
+          st.x = x == 0 ? x0 : (x == 1 ? x1 : x2);
+          st.y = x == 0 ? x1 : (x == 1 ? x2 : x0);
+          st.z = y == 0 ? y0 : (y == 1 ? y1 : y2);
+          st.w = y == 0 ? y1 : (y == 1 ? y2 : y0);
+          if (pos < A.ee_cap) A.eeq[pos] = st;
+          ++pos;
         }
-        stage_push<int4, EX_STAGE>(s_ee[w], nee, has, st, A.eeq, A.ee_cap, A.counters + C_CAND_EE);
       }
     }
   }
   if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
-  if (nvf) stage_flush(s_vf[w], nvf, A.vfq, A.vf_cap, A.counters + C_CAND_VF);
-  if constexpr (D == 3)
-    if (nee) stage_flush(s_ee[w], nee, A.eeq, A.ee_cap, A.counters + C_CAND_EE);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,16 +561,27 @@ static int grid_for(long long chunks, int waves, int num_cus, int per_cu) {
   return (int)((b + 7) / 8 * 8);
 }
 
+long long expand_chunks(unsigned long long pair_cap) { return (long long)((pair_cap + kWave - 1) / kWave); }
+
 void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
   ExpandArgs A;
-  A.pairs = L.pairs; A.npairs_dev = L.npairs_dev; A.pair_cap = L.pair_cap; A.P = L.P; A.flags = L.flags; A.rec = L.rec;
+  A.pairs = L.pairs; A.npairs_dev = L.npairs_dev; A.pair_cap = L.pair_cap; A.faces = L.faces;
   A.discrete = L.discrete; A.continuous = L.continuous;
   A.vfq = L.vfq; A.vf_cap = L.vf_cap; A.eeq = L.eeq; A.ee_cap = L.ee_cap;
   A.disc = L.disc; A.disc_cap = L.disc_cap; A.counters = L.counters;
+  A.off_vf = L.off_vf; A.off_ee = L.off_ee; A.nchunks_cap = expand_chunks(L.pair_cap);
   A.chunk = L.chunk < 1 ? 1 : L.chunk;
   const long long items = L.npairs_hint >= 0 ? L.npairs_hint : (long long)L.pair_cap;
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave / A.chunk + 1, EX_WAVES, L.num_cus, L.blocks_per_cu);
+  if (L.continuous) {  // pass 1 + scans: exact output offsets, no shared queue counter
+    const int gc = grid_for(A.nchunks_cap, EX_WAVES, L.num_cus, L.blocks_per_cu);
+    if (dim == 3) k_expand_count<3><<<gc, EX_WAVES * kWave, 0, s>>>(A);
+    else k_expand_count<2><<<gc, EX_WAVES * kWave, 0, s>>>(A);
+    launch_scan_u64(A.off_vf, A.nchunks_cap + 1, L.scan_part, s);
+    launch_scan_u64(A.off_ee, A.nchunks_cap + 1, L.scan_part, s);
+    k_expand_totals<<<1, 1, 0, s>>>(A.off_vf, A.off_ee, A.nchunks_cap, A.counters);
+  }
   if (dim == 3) k_expand<3><<<g, EX_WAVES * kWave, 0, s>>>(A);
   else k_expand<2><<<g, EX_WAVES * kWave, 0, s>>>(A);
 }
