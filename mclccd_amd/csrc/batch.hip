@@ -126,6 +126,14 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&c->d_hit_sten, &c->cap_hit_sten, sizeof(int4) * c->hit_cap, "hits"));
   CHECK(c->grow((void**)&c->d_hit_vf, &c->cap_hit_vf, c->hit_cap, "hits"));
   CHECK(c->grow((void**)&c->d_hit_toi, &c->cap_hit_toi, sizeof(double) * c->hit_cap, "hits"));
+  CHECK(c->grow((void**)&c->d_vf_toi, &c->cap_vf_toi, sizeof(double) * c->vf_cap, "candidate toi"));
+  if (dim == 3) CHECK(c->grow((void**)&c->d_ee_toi, &c->cap_ee_toi, sizeof(double) * c->ee_cap, "candidate toi"));
+  {
+    const long long nh = accd_chunks(c->vf_cap) + (dim == 3 ? accd_chunks(c->ee_cap) : 0) + 1;
+    CHECK(c->grow((void**)&c->d_hit_off, &c->cap_hit_off, sizeof(unsigned long long) * (size_t)nh, "hit offsets"));
+    CHECK(c->grow((void**)&c->d_hit_part, &c->cap_hit_part, sizeof(unsigned long long) * (size_t)scan_part_words(nh),
+                  "hit offsets"));
+  }
   return MCLG_OK;
 }
 
@@ -163,17 +171,26 @@ static void batch_enqueue(Ctx* c) {
   E.num_cus = c->num_cus; E.blocks_per_cu = c->blocks_per_cu; E.chunk = c->chunk;
   launch_expand(dim, E, s);
   if (!o.continuous) return;
+  const long long ncv = accd_chunks(c->vf_cap), nce = dim == 3 ? accd_chunks(c->ee_cap) : 0;
+  launch_fill_u64(c->d_hit_off, ncv + nce + 1, 0ull, s);
   AccdLaunch L;
-  L.rec = c->d_rec; L.vert_scene = b->d_vert_scene; L.cand_toi = nullptr;
-  L.hit_sten = c->d_hit_sten; L.hit_vf = c->d_hit_vf; L.hit_toi = c->d_hit_toi; L.hit_cap = c->hit_cap;
-  L.counters = c->d_counters; L.num_cus = c->num_cus; L.blocks_per_cu = c->blocks_per_cu; L.chunk = c->chunk;
+  L.rec = c->d_rec; L.vert_scene = b->d_vert_scene;
+  L.counters = c->d_counters; L.num_cus = c->num_cus; L.blocks_per_cu = c->blocks_per_cu; 
   L.q = c->d_vfq; L.n_dev = c->d_counters + C_CAND_VF; L.cap = c->vf_cap; L.n_hint = -1; L.eta = 0;
-  L.scene_eta = S.vf_eta;
+  L.scene_eta = S.vf_eta; L.cand_toi = c->d_vf_toi; L.chunk_hits = c->d_hit_off;
   launch_accd(dim, true, L, s);
   if (dim == 3) {
     L.q = c->d_eeq; L.n_dev = c->d_counters + C_CAND_EE; L.cap = c->ee_cap; L.scene_eta = S.ee_eta;
+    L.cand_toi = c->d_ee_toi; L.chunk_hits = c->d_hit_off + ncv;
     launch_accd(dim, false, L, s);
   }
+  HitsLaunch H;
+  H.vfq = c->d_vfq; H.n_vf_dev = c->d_counters + C_CAND_VF; H.vf_cap = c->vf_cap; H.vf_toi = c->d_vf_toi;
+  H.eeq = c->d_eeq; H.n_ee_dev = c->d_counters + C_CAND_EE; H.ee_cap = c->ee_cap; H.ee_toi = c->d_ee_toi;
+  H.off = c->d_hit_off; H.scan_part = c->d_hit_part; H.counters = c->d_counters;
+  H.hit_sten = c->d_hit_sten; H.hit_vf = c->d_hit_vf; H.hit_toi = c->d_hit_toi; H.hit_cap = c->hit_cap;
+  H.num_cus = c->num_cus; H.blocks_per_cu = c->blocks_per_cu;
+  launch_hits(dim, H, true, s);
 }
 
 // Runs one frame on the staged inputs; grows and recaptures on overflow.

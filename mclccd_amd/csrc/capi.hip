@@ -341,27 +341,36 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     // FP64 ACCD over the two queues (BVHTree.cpp:428-461)
     if (opt.continuous && n_vf + n_ee > 0) {
       if (hit_cap == 0) hit_cap = std::max<unsigned long long>(1024, (unsigned long long)(n_vf + n_ee) / 2);
-      if (keep_cand) {
-        CHECK(grow((void**)&d_vf_toi, &cap_vf_toi, sizeof(double) * (size_t)std::max(n_vf, 1LL), "candidate toi"));
-        CHECK(grow((void**)&d_ee_toi, &cap_ee_toi, sizeof(double) * (size_t)std::max(n_ee, 1LL), "candidate toi"));
-      }
+      CHECK(grow((void**)&d_vf_toi, &cap_vf_toi, sizeof(double) * (size_t)std::max(n_vf, 1LL), "candidate toi"));
+      CHECK(grow((void**)&d_ee_toi, &cap_ee_toi, sizeof(double) * (size_t)std::max(n_ee, 1LL), "candidate toi"));
+      const long long ncv = accd_chunks(vf_cap), nce = dim == 3 ? accd_chunks(ee_cap) : 0;
+      CHECK(grow((void**)&d_hit_off, &cap_hit_off, sizeof(unsigned long long) * (size_t)(ncv + nce + 1), "hit offsets"));
+      CHECK(grow((void**)&d_hit_part, &cap_hit_part,
+                 sizeof(unsigned long long) * (size_t)scan_part_words(ncv + nce + 1), "hit offsets"));
+      launch_fill_u64(d_hit_off, ncv + nce + 1, 0ull, stream);
+      launch_fill_u64(d_counters + C_HITS, 1, 0ull, stream);
+      launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
+      AccdLaunch L;
+      L.rec = d_rec; L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; 
+      L.vert_scene = nullptr; L.scene_eta = nullptr;
+      mark(12);
+      L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
+      L.eta = opt.vf_ccd_eta; L.cand_toi = d_vf_toi; L.chunk_hits = d_hit_off;
+      launch_accd(dim, true, L, stream);
+      L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
+      L.eta = opt.ee_ccd_eta; L.cand_toi = d_ee_toi; L.chunk_hits = d_hit_off + ncv;
+      if (dim == 3) launch_accd(dim, false, L, stream);
+      HitsLaunch H;
+      H.vfq = d_vfq; H.n_vf_dev = d_counters + C_CAND_VF; H.vf_cap = vf_cap; H.vf_toi = d_vf_toi;
+      H.eeq = d_eeq; H.n_ee_dev = d_counters + C_CAND_EE; H.ee_cap = ee_cap; H.ee_toi = d_ee_toi;
+      H.off = d_hit_off; H.scan_part = d_hit_part; H.counters = d_counters;
+      H.num_cus = num_cus; H.blocks_per_cu = blocks_per_cu;
       for (int attempt = 0;; ++attempt) {
         CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
         CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
         CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
-        launch_fill_u64(d_counters + C_HITS, 1, 0ull, stream);
-        launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
-        AccdLaunch L;
-        L.rec = d_rec; L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
-        L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; L.chunk = chunk;
-        L.vert_scene = nullptr; L.scene_eta = nullptr;
-        mark(12);
-        L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
-        L.eta = opt.vf_ccd_eta; L.cand_toi = keep_cand ? d_vf_toi : nullptr;
-        launch_accd(dim, true, L, stream);
-        L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
-        L.eta = opt.ee_ccd_eta; L.cand_toi = keep_cand ? d_ee_toi : nullptr;
-        if (dim == 3) launch_accd(dim, false, L, stream);
+        H.hit_sten = d_hit_sten; H.hit_vf = d_hit_vf; H.hit_toi = d_hit_toi; H.hit_cap = hit_cap;
+        launch_hits(dim, H, attempt == 0, stream);  // a regrowth redoes the compaction only
         mark(13);
         HIPCHECK(this, hipGetLastError());
         CHECK(read_counters());
@@ -399,7 +408,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
-                  d_vtmp, d_hist, d_sboxes, d_face, d_ex_off, d_ex_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
+                  d_vtmp, d_hist, d_sboxes, d_face, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)

@@ -47,6 +47,8 @@ struct Ctx {
   bool pairs_are_prims = false;                     // d_pairs converted to primitive pairs
   unsigned long long* d_ex_off = nullptr; size_t cap_ex_off = 0;    // candidate offsets per 64-pair chunk
   unsigned long long* d_ex_part = nullptr; size_t cap_ex_part = 0;  // their scan's chunk sums
+  unsigned long long* d_hit_off = nullptr; size_t cap_hit_off = 0;  // hits per 64-stencil chunk, then offsets
+  unsigned long long* d_hit_part = nullptr; size_t cap_hit_part = 0;
   uint8_t* d_act = nullptr; size_t cap_act = 0;
   unsigned long long* d_cbounds = nullptr; size_t cap_cbounds = 0;
   double* d_partial = nullptr; size_t cap_partial = 0;

@@ -96,7 +96,31 @@ struct AccdLaunch {
   double eta;
   const int* vert_scene;
   const double* scene_eta;
-  double* cand_toi;
+  double* cand_toi;                  // per stencil: toi, -1 = no hit
+  unsigned long long* chunk_hits;    // hits per 64-stencil chunk (accd_chunks(cap) entries, zeroed by the caller)
+  unsigned long long* counters;
+  int num_cus;
+  int blocks_per_cu;
+};
+void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
+long long accd_chunks(unsigned long long cap);
+
+// Hit list from the per-stencil TOIs: VF hits, then EE hits, each in queue
+// order.  off holds the hits per 64-stencil chunk written by the ACCD
+// kernels (VF chunks, then EE chunks, accd_chunks(vf_cap) + accd_chunks(ee_cap)
+// entries, plus one); with scan it is scanned here and the total is written
+// to counters[C_HITS].
+struct HitsLaunch {
+  const int4* vfq;
+  const unsigned long long* n_vf_dev;
+  unsigned long long vf_cap;
+  const double* vf_toi;
+  const int4* eeq;
+  const unsigned long long* n_ee_dev;
+  unsigned long long ee_cap;
+  const double* ee_toi;
+  unsigned long long* off;
+  unsigned long long* scan_part;  // scan_part_words(entries of off)
   int4* hit_sten;
   uint8_t* hit_vf;
   double* hit_toi;
@@ -104,8 +128,9 @@ struct AccdLaunch {
   unsigned long long* counters;
   int num_cus;
   int blocks_per_cu;
-  int chunk;
 };
-void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
+// scan=true: scan off and set counters[C_HITS] first; false: only the
+// compaction (after a hit-capacity regrowth).
+void launch_hits(int dim, const HitsLaunch& L, bool scan, hipStream_t s);
 
 }  // namespace mclg

@@ -473,84 +473,72 @@ struct AccdArgs {
   double eta;
   const int* vert_scene;    // several scenes: per-scene gaps
   const double* scene_eta;
-  double* cand_toi;  // per-stencil toi (-1 = no hit) or null
-  int4* hit_sten;
-  uint8_t* hit_vf;
-  double* hit_toi;
-  unsigned long long hit_cap;
+  double* cand_toi;         // per-stencil toi (-1 = no hit)
+  unsigned long long* chunk_hits;  // hits per 64-stencil chunk
   unsigned long long* counters;
-  int chunk;
 };
 
-struct HitRec {
-  int4 s;
-  double t;
-};
+constexpr int AC_WAVES = 4;
 
-constexpr int AC_WAVES = 4, AC_STAGE = 256;
-
+// One lane per stencil, 64 consecutive stencils per wave step; the TOI (or
+// -1) goes to cand_toi, the hit list is compacted afterwards at scanned
+// offsets, so no shared counter is touched per hit.
 template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
   using V = typename VLoad<D>::V;
-  __shared__ HitRec s_h[AC_WAVES][AC_STAGE];
   const int w = threadIdx.x / kWave, lane = lane_id();
-  int nh = 0;
   long long evals = 0;
   const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
-  const long long stride = (long long)gridDim.x * AC_WAVES * A.chunk;
-  for (long long c0 = ((long long)xcd_block() * AC_WAVES + w) * A.chunk; c0 < nchunks; c0 += stride)
-  for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
+  const long long stride = (long long)gridDim.x * AC_WAVES;
+  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
     const long long i = ch * kWave + lane;
-    const bool live = i < n;
-    int4 sv = make_int4(0, 0, 0, 0);
-    int hit = 0, ev = 0;
-    double toi = -1.0;
-    if (live) {
-      sv = A.q[i];
+    int hit = 0;
+    if (i < n) {
+      const int4 sv = A.q[i];
       const int id[4] = {sv.x, sv.y, sv.z, sv.w};
       V v0[D + 1], v1[D + 1];
 #pragma unroll
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
+      double toi = -1.0;
+      int ev = 0;
       hit = accd<D, VF>(v0, v1, eta, toi, ev);
-      if (A.cand_toi) A.cand_toi[i] = hit ? toi : -1.0;
+      A.cand_toi[i] = hit ? toi : -1.0;
+      evals += ev;
     }
-    evals += ev;
-    // hits: staged, then written to the three SoA arrays
     const uint64_t m = __ballot(hit != 0);
-    if (m) {
-      if (hit) s_h[w][nh + popc_below(m)] = HitRec{sv, toi};
-      nh += __popcll(m);
-      if (nh > AC_STAGE - kWave) {
-        wave_lds_sync();
-        unsigned long long base = 0;
-        if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)nh);
-        base = __shfl(base, 0);
-        for (int k = lane; k < nh; k += kWave)
-          if (base + k < A.hit_cap) {
-            A.hit_sten[base + k] = s_h[w][k].s;
-            A.hit_toi[base + k] = s_h[w][k].t;
-            A.hit_vf[base + k] = VF ? 1 : 0;
-          }
-        nh = 0;
-        wave_lds_sync();
-      }
-    }
-  }
-  if (nh) {
-    wave_lds_sync();
-    unsigned long long base = 0;
-    if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)nh);
-    base = __shfl(base, 0);
-    for (int k = lane; k < nh; k += kWave)
-      if (base + k < A.hit_cap) {
-        A.hit_sten[base + k] = s_h[w][k].s;
-        A.hit_toi[base + k] = s_h[w][k].t;
-        A.hit_vf[base + k] = VF ? 1 : 0;
-      }
+    if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
   }
   for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
   if (lane == 0 && evals) atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
+}
+
+__global__ void k_hits_total(const unsigned long long* off, long long last, unsigned long long* counters) {
+  counters[C_HITS] = off[last];
+}
+
+// One wave per 64-stencil chunk: the chunk's hits in lane order at off[ch].
+template <bool VF>
+__global__ __launch_bounds__(256) void k_hits_compact(const int4* __restrict__ q, const unsigned long long* n_dev,
+                                                      unsigned long long cap, const double* __restrict__ toi,
+                                                      const unsigned long long* __restrict__ off, int4* hit_sten,
+                                                      uint8_t* hit_vf, double* hit_toi, unsigned long long hit_cap) {
+  const int lane = lane_id();
+  const long long n = (long long)(*n_dev < cap ? *n_dev : cap);
+  const long long nchunks = (n + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * (blockDim.x / kWave);
+  for (long long ch = (long long)xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave; ch < nchunks; ch += stride) {
+    const long long i = ch * kWave + lane;
+    const double t = i < n ? toi[i] : -1.0;
+    const bool hit = t >= 0.0;
+    const uint64_t m = __ballot(hit);
+    const unsigned long long pos = off[ch] + (unsigned long long)popc_below(m);
+    if (hit && pos < hit_cap) {
+      hit_sten[pos] = q[i];
+      hit_toi[pos] = t;
+      hit_vf[pos] = VF ? 1 : 0;
+    }
+  }
 }
 
 static int grid_for(long long chunks, int waves, int num_cus, int per_cu) {
@@ -586,21 +574,37 @@ void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
   else k_expand<2><<<g, EX_WAVES * kWave, 0, s>>>(A);
 }
 
+long long accd_chunks(unsigned long long cap) { return (long long)((cap + kWave - 1) / kWave); }
+
 void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   AccdArgs A;
   A.q = L.q; A.n_dev = L.n_dev; A.cap = L.cap; A.rec = L.rec; A.eta = L.eta;
   A.vert_scene = L.vert_scene; A.scene_eta = L.scene_eta; A.cand_toi = L.cand_toi;
-  A.hit_sten = L.hit_sten; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
-  A.counters = L.counters;
-  A.chunk = L.chunk < 1 ? 1 : L.chunk;
+  A.chunk_hits = L.chunk_hits; A.counters = L.counters;
   const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
   if (items <= 0) return;
-  const int g = grid_for((items + kWave - 1) / kWave / A.chunk + 1, AC_WAVES, L.num_cus, L.blocks_per_cu);
+  const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);
   if (dim == 3) {
     if (vf) k_accd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
     else k_accd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A);
   } else {
     k_accd<2, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
+  }
+}
+
+void launch_hits(int dim, const HitsLaunch& L, bool scan, hipStream_t s) {
+  const long long nv = accd_chunks(L.vf_cap), ne = dim == 3 ? accd_chunks(L.ee_cap) : 0;
+  if (scan) {
+    launch_scan_u64(L.off, nv + ne + 1, L.scan_part, s);
+    k_hits_total<<<1, 1, 0, s>>>(L.off, nv + ne, L.counters);
+  }
+  const int gv = grid_for(nv, 4, L.num_cus, L.blocks_per_cu);
+  k_hits_compact<true><<<gv, 256, 0, s>>>(L.vfq, L.n_vf_dev, L.vf_cap, L.vf_toi, L.off, L.hit_sten, L.hit_vf,
+                                         L.hit_toi, L.hit_cap);
+  if (dim == 3 && ne > 0) {
+    const int ge = grid_for(ne, 4, L.num_cus, L.blocks_per_cu);
+    k_hits_compact<false><<<ge, 256, 0, s>>>(L.eeq, L.n_ee_dev, L.ee_cap, L.ee_toi, L.off + nv, L.hit_sten,
+                                            L.hit_vf, L.hit_toi, L.hit_cap);
   }
 }
 
