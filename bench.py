@@ -34,6 +34,60 @@ FLOPS_ACCD_SETUP = 73  # displacements, centring, l_p (4 norms)
 FLOPS_PER_EVAL = 105   # one ACCD iteration: step x += t_l p (24), distance (~68), eps and t_l (~13)
 
 
+# HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) on
+# this workload, scaled by the same counters on a 1 GiB device copy; the
+# summary (with the commands) is the file below.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "stacked_layers_1M_pmc_traffic.md")
+
+
+def profiled_traffic(kernel):
+    """'traffic_bytes_per_launch <kernel>: <bytes>' from the committed PMC summary, else None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if line.startswith("traffic_bytes_per_launch ") and ":" in line:
+                    k, v = line[len("traffic_bytes_per_launch "):].rsplit(":", 1)
+                    if k.strip() == kernel:
+                        return float(v)
+    except OSError:
+        pass
+    return None
+
+
+def roofline(c, avg, n):
+    """Roofline of the longest single kernel of the step (HIP-event times)."""
+    kern = {"traverse": avg["traverse"], "accd_vf": avg["accd_vf"], "accd_ee": avg["accd_ee"],
+            "expand": avg["expand"]}
+    dom = max(kern, key=kern.get)
+    if dom == "traverse":
+        # compulsory bytes: sorted nodes and FP64 leaf boxes once, 8 B per emitted pair
+        byts = 80 * (2 * n - 1) + 144 * n + 8 * c["leaf_pairs"]
+        achieved = byts / (avg["traverse"] * 1e-3) / 1e9
+        return {"kernel": "k_traverse<3>", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": profiled_traffic("k_traverse<3>"),
+                "algorithmic_bytes": byts, "ms": round(avg["traverse"], 3),
+                "note": "algorithmic bytes = 80*(2n-1) nodes + 144*n FP64 leaf boxes + 8*leaf_pairs; traffic = "
+                        "rocprof FETCH_SIZE+WRITE_SIZE per launch (calibrated on a 1 GiB copy)"}
+    if dom == "expand":
+        byts = 16 * c["leaf_pairs"] + 128 * n + 16 * c["candidates"] + 8 * c["discrete"]
+        achieved = byts / (avg["expand"] * 1e-3) / 1e9
+        return {"kernel": "k_expand<3> (+ count pass and scans)", "bound": "hbm", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": profiled_traffic("k_expand<3>"), "algorithmic_bytes": byts,
+                "note": "algorithmic bytes = pairs read twice (8+8) + 128 B face record per leaf + 16 B per "
+                        "stencil + 8 B per discrete hit"}
+    flops = FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]
+    ms = avg["accd_vf"] + avg["accd_ee"]
+    achieved = flops / (ms * 1e-3) / 1e12
+    return {"kernel": "k_accd<3,VF> + k_accd<3,EE> (FP64 ACCD)", "bound": "valu", "achieved": round(achieved, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+            "traffic": profiled_traffic("k_accd<3, false>"),
+            "note": f"algorithmic flops = {FLOPS_ACCD_SETUP}*candidates + {FLOPS_PER_EVAL}*distance evaluations, "
+                    "div/sqrt = 1 flop; non-FMA FP64 peak is 39.3 T ops/s"}
+
+
 def _cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -162,6 +216,8 @@ def main():
     ap.add_argument("--workload", default="layers", choices=["layers", "cloth"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes comparison")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -175,6 +231,12 @@ def main():
 
     V0, V1, F, th, desc = make_workload(args.workload)
     dev = torch.device("cuda:0")
+    if args.calibrate_copy:  # reads and writes exactly 1 GiB
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     dV0 = torch.from_numpy(V0).to(dev)
     dV1 = torch.from_numpy(V1).to(dev)
     dF = torch.from_numpy(F).to(dev)
@@ -206,25 +268,13 @@ def main():
     print(f"[bench] counts {c}", file=sys.stderr, flush=True)
     print(f"[bench] phase ms {avg}", file=sys.stderr, flush=True)
 
-    # dominant kernel roofline
-    flops = FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]
-    accd_ms = avg["accd"]
-    trav_ms = avg["traverse"]
-    if accd_ms >= trav_ms:
-        achieved = flops / (accd_ms * 1e-3) / 1e12
-        roof = {"kernel": "k_accd<3,VF> + k_accd<3,EE> (FP64 ACCD over the stencil queues)", "bound": "valu",
-                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "note": "FP64 vector ALU bound (no MFMA on this path); algorithmic flops = "
-                        f"{FLOPS_ACCD_SETUP}*candidates + {FLOPS_PER_EVAL}*distance evaluations, div/sqrt = 1 flop; "
-                        "non-FMA FP64 peak is 39.3 T ops/s"}
-    else:
-        # traversal: compulsory bytes = one pass over nodes and leaf boxes + 8 B per emitted pair
-        n = len(F)
-        byts = 8 * c["leaf_pairs"] + 80 * (2 * n - 1) + 145 * n
-        achieved = byts / (trav_ms * 1e-3) / 1e9
-        roof = {"kernel": "k_traverse<3>", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    roof = roofline(c, avg, len(F))
+    # the other large kernels, for context
+    others = {"traverse_ms": round(avg["traverse"], 3), "expand_ms": round(avg["expand"], 3),
+              "accd_vf_ms": round(avg["accd_vf"], 3), "accd_ee_ms": round(avg["accd_ee"], 3),
+              "accd_tflops": round((FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]) /
+                                   ((avg["accd_vf"] + avg["accd_ee"]) * 1e-3) / 1e12, 3)
+              if avg["accd_vf"] + avg["accd_ee"] > 0 else None}
 
     # TOI adjoint over every hit of the last traverse, w_i ~ U(0,1) (seed 4)
     import ctypes as C
@@ -274,6 +324,7 @@ def main():
                    "discrete": c["discrete"], "dist_evals": c["dist_evals"],
                    "phase_ms": {k: round(v, 4) for k, v in avg.items()}},
         "roofline": roof,
+        "kernels": others,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "batched": batched,

@@ -108,9 +108,11 @@ int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
  *   candidates: sten4 / is_vf as above plus the ACCD toi of each candidate
  *               (-1 when it does not hit), every candidate
  *               (MCLG_KEEP_CANDIDATES)
- * Records come in no particular order (the reference's callback order is
- * not deterministic either).
- *   leaf pairs: int[2*cap] (MCLG_KEEP_LEAF_PAIRS)
+ * Hits are the VF hits then the EE hits, each in candidate order; candidates
+ * follow the leaf-pair order, which varies from run to run (as does the
+ * reference's concurrent callback order), so compare them as sets.
+ *   leaf pairs: int[2*cap] (only when the traverse had MCLG_KEEP_LEAF_PAIRS;
+ *               MCLG_ERR_STATE otherwise)
  * Stencils follow BVHTree.cpp:336-395: VF (v, f0, f1, f2) with -1 padding in
  * 2D, EE (a0, a1, b0, b1).  Pair orientation: the lower primitive index
  * first, or the lower rank when a rank is set (mclg_set_rank). */
@@ -128,7 +130,8 @@ int mclg_copy_leaves(mclg_ctx* ctx, double* boxes, unsigned char* active, unsign
 
 /* Device pointers to the last traverse's results (valid until the next traverse).
  * Candidates live in two queues, VF then EE; their per-candidate toi arrays are
- * set only when the traverse kept candidates. */
+ * set only when the traverse kept candidates, leaf_pairs2 only when it kept
+ * leaf pairs (NULL otherwise). */
 typedef struct mclg_device_results {
   const int* hit_sten4;
   const unsigned char* hit_is_vf;
@@ -157,9 +160,10 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
  *   0 validate+pack  1 reptri  2 leaf boxes  3 morton  4 sort  5 lbvh (update)
  *   6 validate+pack  7 traversal kernel  8 candidate + discrete kernel
  *   9 ACCD kernels (traverse; 7-9 bracket the final launches only)
- *  10 TOI adjoint (last mclg_toi_grad: gradient clear + adjoint kernel)
+ *  10 TOI adjoint (last mclg_toi_grad: sort, adjoint kernels)
+ *  11 VF ACCD kernel  12 EE ACCD kernel (both inside 9, which adds the hit compaction)
  * Writes min(n, MCLG_NUM_TIMINGS) values. */
-#define MCLG_NUM_TIMINGS 11
+#define MCLG_NUM_TIMINGS 13
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
 
 /* TOI adjoint.  No reference counterpart.  For the CCD hits of the last
@@ -167,7 +171,11 @@ int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
  * writes dL/dV0 and dL/dV1 (nv x dim each) of L = sum_i w_i toi_i, where toi_i
  * is the ACCD time of impact as computed by NarrowPhase.cpp:634-711 (its
  * derivative wherever it is smooth; hits already within the gap have toi 0 and
- * contribute nothing).  Host pointers, or device pointers when device_ptrs. */
+ * contribute nothing).  Each distance is differentiated at the closest points
+ * the forward pass computed (envelope theorem), except in the reference's
+ * point_on_triangle quirk branch (Projection.hpp:131), which is differentiated
+ * through.  Host pointers, or device pointers when device_ptrs.  Allocates
+ * scratch on the first call (about 16 bytes per hit for the visiting order). */
 int mclg_toi_grad(mclg_ctx* ctx, const double* w, double* dV0, double* dV1, int device_ptrs);
 
 /* ------------------------------------------------------------------------

@@ -22,9 +22,9 @@ ERR_CAPACITY = -6
 
 KEEP_CANDIDATES = 1
 KEEP_LEAF_PAIRS = 2
-NUM_TIMINGS = 11
+NUM_TIMINGS = 13
 TIMING_NAMES = ("update_pack", "update_reptri", "update_leaf_boxes", "update_morton", "update_sort",
-                "update_lbvh", "traverse_pack", "traverse", "expand", "accd", "adjoint")
+                "update_lbvh", "traverse_pack", "traverse", "expand", "accd", "adjoint", "accd_vf", "accd_ee")
 
 
 class Options(C.Structure):

@@ -357,9 +357,11 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
       L.eta = opt.vf_ccd_eta; L.cand_toi = d_vf_toi; L.chunk_hits = d_hit_off;
       launch_accd(dim, true, L, stream);
+      mark(16);
       L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
       L.eta = opt.ee_ccd_eta; L.cand_toi = d_ee_toi; L.chunk_hits = d_hit_off + ncv;
       if (dim == 3) launch_accd(dim, false, L, stream);
+      mark(17);
       HitsLaunch H;
       H.vfq = d_vfq; H.n_vf_dev = d_counters + C_CAND_VF; H.vf_cap = vf_cap; H.vf_toi = d_vf_toi;
       H.eeq = d_eeq; H.n_ee_dev = d_counters + C_CAND_EE; H.ee_cap = ee_cap; H.ee_toi = d_ee_toi;
@@ -393,13 +395,17 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   times[6] = ms;
   (void)hipEventElapsedTime(&ms, ev[8], ev[9]);
   times[7] = ms;
-  times[8] = times[9] = 0;
+  times[8] = times[9] = times[11] = times[12] = 0;
   if (npairs > 0 && (opt.continuous || opt.discrete)) {
     (void)hipEventElapsedTime(&ms, ev[10], ev[11]);
     times[8] = ms;
     if (opt.continuous && n_vf + n_ee > 0) {
       (void)hipEventElapsedTime(&ms, ev[12], ev[13]);
       times[9] = ms;
+      (void)hipEventElapsedTime(&ms, ev[12], ev[16]);
+      times[11] = ms;
+      (void)hipEventElapsedTime(&ms, ev[16], ev[17]);
+      times[12] = ms;
     }
   }
   return MCLG_OK;

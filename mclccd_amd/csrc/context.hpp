@@ -8,7 +8,7 @@
 
 namespace mclg {
 
-constexpr int kNumEvents = 16;
+constexpr int kNumEvents = 19;
 constexpr size_t kAdjQueue = size_t(1) << 22;  // adjoint multi-step queue entries
 
 struct BatchState;

@@ -356,8 +356,20 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand_co
     if (ch < nchunks && A.continuous) {
       const long long i = ch * kWave + lane;
       if (i < npairs) {
-        FaceHead<D> ha, hb;
-        load_pair<D>(A, i, ha, hb);
+        // counts do not depend on orientation or V1: the records' first 16 bytes (P, flags) suffice
+        const FaceRec<D>* F = static_cast<const FaceRec<D>*>(A.faces);
+        const int2 sp = A.pairs[i];
+        const int4 a = *reinterpret_cast<const int4*>(F + sp.x);
+        const int4 b = *reinterpret_cast<const int4*>(F + sp.y);
+        FaceHead<D> ha = {}, hb = {};
+        ha.p[0] = a.x; ha.p[1] = a.y; hb.p[0] = b.x; hb.p[1] = b.y;
+        if constexpr (D == 3) {
+          ha.p[2] = a.z; ha.flags = (uint32_t)a.w;
+          hb.p[2] = b.z; hb.flags = (uint32_t)b.w;
+        } else {
+          ha.flags = (uint32_t)a.z;
+          hb.flags = (uint32_t)b.z;
+        }
         uint32_t vfm, eem;
         candidate_masks<D>(ha, hb, vfm, eem);
         cv = (uint32_t)__popc(vfm);
