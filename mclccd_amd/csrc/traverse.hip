@@ -65,6 +65,54 @@ __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const Le
   return ov & (emp | !sep);
 }
 
+// The same test decided from the FP32 node boxes when it can be.  A node's
+// lower bound a = f32_down(lo) brackets the FP64 value as lo in [a, a+) and an
+// upper bound b = f32_up(hi) as hi in (b-, b] (one float step each way), so
+// every comparison of the test is certainly true, certainly false, or
+// undecided; the FP64 boxes are read only when the combination is undecided.
+struct Tri {
+  bool t, f;  // certainly true / certainly false
+};
+__device__ __forceinline__ Tri tri_and(Tri x, Tri y) { return {bool(x.t & y.t), bool(x.f | y.f)}; }
+__device__ __forceinline__ Tri tri_or(Tri x, Tri y) { return {bool(x.t | y.t), bool(x.f & y.f)}; }
+// lo <= hi from a = bound of lo (with a+ = step_up(a)) and b = bound of hi (with b- = step_down(b))
+__device__ __forceinline__ Tri lo_le_hi(float a, float ap, float b, float bm) {
+  const bool fin = __builtin_isfinite(a) && __builtin_isfinite(b);
+  return {bool(fin & (ap <= bm)), bool(fin & (a > b))};
+}
+__device__ __forceinline__ Tri lo_gt_hi(float a, float ap, float b, float bm) {
+  const Tri r = lo_le_hi(a, ap, b, bm);
+  return {r.f, r.t};
+}
+
+// leaf_test(query, leaf) in three-valued logic on the FP32 bounds.
+template <int D> __device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, bool qa, const NodeF<D>& b, bool ba) {
+  Tri ov = {bool(qa | ba), !(qa | ba)};
+  Tri emp = {false, true};
+  Tri sep = {false, true};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const float q_sw_lo_p = step_up(q.sw_lo[k]), q_sw_hi_m = step_down(q.sw_hi[k]);
+    const float q_t0_lo_p = step_up(q.t0_lo[k]), q_t0_hi_m = step_down(q.t0_hi[k]);
+    const float q_t1_lo_p = step_up(q.t1_lo[k]), q_t1_hi_m = step_down(q.t1_hi[k]);
+    const float b_sw_lo_p = step_up(b.sw_lo[k]), b_sw_hi_m = step_down(b.sw_hi[k]);
+    const float b_t0_lo_p = step_up(b.t0_lo[k]), b_t0_hi_m = step_down(b.t0_hi[k]);
+    const float b_t1_lo_p = step_up(b.t1_lo[k]), b_t1_hi_m = step_down(b.t1_hi[k]);
+    ov = tri_and(ov, lo_le_hi(q.sw_lo[k], q_sw_lo_p, b.sw_hi[k], b_sw_hi_m));
+    ov = tri_and(ov, lo_le_hi(b.sw_lo[k], b_sw_lo_p, q.sw_hi[k], q_sw_hi_m));
+    emp = tri_or(emp, lo_gt_hi(q.t0_lo[k], q_t0_lo_p, q.t0_hi[k], q_t0_hi_m));
+    emp = tri_or(emp, lo_gt_hi(b.t0_lo[k], b_t0_lo_p, b.t0_hi[k], b_t0_hi_m));
+    emp = tri_or(emp, lo_gt_hi(q.t1_lo[k], q_t1_lo_p, q.t1_hi[k], q_t1_hi_m));
+    emp = tri_or(emp, lo_gt_hi(b.t1_lo[k], b_t1_lo_p, b.t1_hi[k], b_t1_hi_m));
+    const Tri above = tri_and(lo_gt_hi(q.t0_lo[k], q_t0_lo_p, b.t0_hi[k], b_t0_hi_m),
+                              lo_gt_hi(q.t1_lo[k], q_t1_lo_p, b.t1_hi[k], b_t1_hi_m));
+    const Tri below = tri_and(lo_gt_hi(b.t0_lo[k], b_t0_lo_p, q.t0_hi[k], q_t0_hi_m),
+                              lo_gt_hi(b.t1_lo[k], b_t1_lo_p, q.t1_hi[k], q_t1_hi_m));
+    sep = tri_or(sep, tri_or(above, below));
+  }
+  return tri_and(ov, tri_or(emp, Tri{sep.f, sep.t}));
+}
+
 // Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
 // PIN the loaded registers are consumed by an empty asm statement (no memory
 // access in it), so the compiler cannot split off and delay any part.
@@ -116,12 +164,10 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
   if (lane_id() == 0) s_cnt[w] = 0;
   wave_lds_sync();
   NodeF<D> q;
-  LeafBox<D> qb;
   bool qa = false;
   int node = -1, end = -1;
   if (valid) {
     q = load16(nodes + (n - 1) + s);
-    qb = load16(sboxes + s);
     qa = (q.meta >> 31) != 0;
     node = (int)(q.meta & 0x7fffffffu) - 1;
     // several scenes: the walk ends where the pre-order leaves this leaf's scene subtree
@@ -143,8 +189,13 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
     if (cull_f32<D>(q, nd)) {
       if (node >= n - 1) {
         k = node - (n - 1);
-        const LeafBox<D> bb = load16(sboxes + k);
-        emit = leaf_test<D>(qb, qa, bb, (nd.meta >> 31) != 0);
+        const bool ba = (nd.meta >> 31) != 0;
+        const Tri r = leaf_test_f32<D>(q, qa, nd, ba);
+        if (r.t | r.f) {
+          emit = r.t;
+        } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
+          emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
+        }
       } else {
         next = nd.child;
       }
