@@ -100,7 +100,8 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&b->d_mort, &b->cap_mort, sizeof(uint32_t) * (size_t)np, "morton"));
   CHECK(c->grow((void**)&c->d_nodes, &c->cap_nodes, node * (size_t)(2 * np - 1), "nodes"));
   CHECK(c->grow(&c->d_sboxes, &c->cap_sboxes, box * (size_t)np, "sorted leaf boxes"));
-  CHECK(c->grow(&c->d_face, &c->cap_face, face_bytes(dim) * (size_t)np, "face records"));
+  CHECK(c->grow(&c->d_face, &c->cap_face, face_bytes((size_t)np), "face arrays"));
+  CHECK(c->grow((void**)&c->d_v1, &c->cap_v1, sizeof(double) * v1_stride(dim) * (size_t)nv, "V1"));
   const size_t ni = (size_t)std::max(np - 1, 1);
   CHECK(c->grow((void**)&c->d_lchild, &c->cap_lchild, sizeof(int) * ni, "lchild"));
   CHECK(c->grow((void**)&c->d_rchild, &c->cap_rchild, sizeof(int) * ni, "rchild"));
@@ -145,7 +146,7 @@ static void batch_enqueue(Ctx* c) {
   hipStream_t s = c->stream;
   const SceneArgs S = scene_args(b);
   const mclg_options& o = b->opts[0];
-  launch_pack(dim, b->d_V0, b->d_V1, b->nv, c->d_rec, s);
+  launch_pack(dim, b->d_V0, b->d_V1, b->nv, c->d_rec, s, c->d_v1);
   launch_fill_u64(c->d_counters, C_COUNT, 0ull, s);
   launch_leaf_boxes(dim, c->d_rec, b->d_P, np, dim, nullptr, 0.0, c->d_boxes, c->d_act, c->d_partial, c->d_cbounds, S, s);
   launch_morton(dim, c->d_boxes, np, c->d_cbounds, S, c->d_keys, c->d_order, b->d_mort, s);
@@ -155,13 +156,14 @@ static void batch_enqueue(Ctx* c) {
   launch_gather_u32(c->d_order, np, b->d_mort, c->d_keys, s);
   launch_lbvh(dim, c->d_keys, b->d_khi, c->d_order, np, c->d_boxes, c->d_act, c->d_nodes, c->d_sboxes, c->d_lchild,
               c->d_rchild, c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
-  launch_face_records(dim, c->d_order, np, b->d_P, c->d_flags, nullptr, c->d_rec, c->d_face, s);
+  const FaceArrays F = face_arrays(c->d_face, (size_t)np, c->d_v1);
+  launch_face_records(dim, c->d_order, np, b->d_P, c->d_flags, nullptr, F, s);
   launch_traverse(dim, c->d_nodes, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters, b->d_khi, b->d_root,
                   c->tr_waves, s);
   if (!(o.continuous || o.discrete)) return;
   ExpandLaunch E;
   E.pairs = c->d_pairs; E.npairs_dev = c->d_counters + C_LEAF_PAIRS; E.pair_cap = c->pair_cap; E.npairs_hint = -1;
-  E.faces = c->d_face; E.discrete = o.discrete; E.continuous = o.continuous;
+  E.faces = F; E.discrete = o.discrete; E.continuous = o.continuous;
   {
     const long long nch = expand_chunks(c->pair_cap) + 1;
     E.off_vf = c->d_ex_off; E.off_ee = c->d_ex_off + nch; E.scan_part = c->d_ex_part;

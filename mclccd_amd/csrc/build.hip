@@ -11,24 +11,35 @@ namespace mclg {
 // Vertex records (both time steps of a vertex in one 64 B / 32 B record)
 // ---------------------------------------------------------------------------
 template <int D>
-__global__ void k_pack(const double* __restrict__ V0, const double* __restrict__ V1, int nv, double* __restrict__ rec) {
+__global__ void k_pack(const double* __restrict__ V0, const double* __restrict__ V1, int nv, double* __restrict__ rec,
+                       double* __restrict__ v1) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
   double* r = rec + (size_t)v * VRec<D>::stride;
   if constexpr (D == 3) {
+    const double x1 = V1[3 * (size_t)v], y1 = V1[3 * (size_t)v + 1], z1 = V1[3 * (size_t)v + 2];
     r[0] = V0[3 * (size_t)v]; r[1] = V0[3 * (size_t)v + 1]; r[2] = V0[3 * (size_t)v + 2]; r[3] = 0.0;
-    r[4] = V1[3 * (size_t)v]; r[5] = V1[3 * (size_t)v + 1]; r[6] = V1[3 * (size_t)v + 2]; r[7] = 0.0;
+    r[4] = x1; r[5] = y1; r[6] = z1; r[7] = 0.0;
+    if (v1) {
+      double* o = v1 + (size_t)v * 4;
+      o[0] = x1; o[1] = y1; o[2] = z1; o[3] = 0.0;
+    }
   } else {
+    const double x1 = V1[2 * (size_t)v], y1 = V1[2 * (size_t)v + 1];
     r[0] = V0[2 * (size_t)v]; r[1] = V0[2 * (size_t)v + 1];
-    r[2] = V1[2 * (size_t)v]; r[3] = V1[2 * (size_t)v + 1];
+    r[2] = x1; r[3] = y1;
+    if (v1) {
+      v1[2 * (size_t)v] = x1;
+      v1[2 * (size_t)v + 1] = y1;
+    }
   }
 }
 
-void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s) {
+void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1) {
   if (nv <= 0) return;
   const int g = (nv + 255) / 256;
-  if (dim == 3) k_pack<3><<<g, 256, 0, s>>>(V0, V1, nv, rec);
-  else k_pack<2><<<g, 256, 0, s>>>(V0, V1, nv, rec);
+  if (dim == 3) k_pack<3><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1);
+  else k_pack<2><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1);
 }
 
 // ---------------------------------------------------------------------------
@@ -630,15 +641,22 @@ __device__ __forceinline__ bool parent_is_right(const uint32_t* keys, const uint
   return l == 0 || (r != n - 1 && split_key(keys, hi, r) < split_key(keys, hi, l - 1));
 }
 
-template <int D> __device__ __forceinline__ NodeF<D> leaf_node(const LeafBox<D>& b, int prim, bool active) {
+// A leaf whose FP64 t0 or t1 box is empty (a coordinate axis of NaNs) passes
+// the exact test whenever the swept boxes overlap; its FP32 t0 and t1 boxes
+// are set to the swept box so that no culling test (here or at an ancestor,
+// whose boxes are unions) separates it at both times.
+template <int D> __device__ __forceinline__ NodeF<D> leaf_node(const LeafBox<D>& b, bool active) {
   NodeF<D> L;
+  bool emp = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k) emp |= (b.t0_lo[k] > b.t0_hi[k]) | (b.t1_lo[k] > b.t1_hi[k]);
 #pragma unroll
   for (int k = 0; k < D; ++k) {
     L.sw_lo[k] = f32_down(b.sw_lo[k]); L.sw_hi[k] = f32_up(b.sw_hi[k]);
-    L.t0_lo[k] = f32_down(b.t0_lo[k]); L.t0_hi[k] = f32_up(b.t0_hi[k]);
-    L.t1_lo[k] = f32_down(b.t1_lo[k]); L.t1_hi[k] = f32_up(b.t1_hi[k]);
+    L.t0_lo[k] = emp ? L.sw_lo[k] : f32_down(b.t0_lo[k]); L.t0_hi[k] = emp ? L.sw_hi[k] : f32_up(b.t0_hi[k]);
+    L.t1_lo[k] = emp ? L.sw_lo[k] : f32_down(b.t1_lo[k]); L.t1_hi[k] = emp ? L.sw_hi[k] : f32_up(b.t1_hi[k]);
   }
-  L.child = prim;
+  L.child = emp ? 1 : 0;
   L.meta = active ? 0x80000000u : 0u;
   return L;
 }
@@ -673,7 +691,7 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
     const uint32_t prim = order[s];
     const LeafBox<D> lb = boxes[prim];
     sboxes[s] = lb;
-    const NodeF<D> L = leaf_node<D>(lb, (int)prim, act[prim] != 0);
+    const NodeF<D> L = leaf_node<D>(lb, act[prim] != 0);
     nodes[(n - 1) + s] = L;
     s_node[t] = L;
     if (S.root) {

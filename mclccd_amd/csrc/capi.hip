@@ -278,9 +278,10 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   timing = true;
   mark(7);
   CHECK(check_indices(P, (long long)np * dim, nv, "traverse: P"));
-  launch_pack(dim, V0, V1, nv, d_rec, stream);
-  CHECK(grow(&d_face, &cap_face, face_bytes(dim) * (size_t)np, "face records"));
-  launch_face_records(dim, d_order, np, P, d_flags, rank_set ? d_rank : nullptr, d_rec, d_face, stream);
+  CHECK(grow((void**)&d_v1, &cap_v1, sizeof(double) * v1_stride(dim) * (size_t)nv, "V1"));
+  launch_pack(dim, V0, V1, nv, d_rec, stream, d_v1);
+  CHECK(grow(&d_face, &cap_face, face_bytes((size_t)np), "face arrays"));
+  launch_face_records(dim, d_order, np, P, d_flags, rank_set ? d_rank : nullptr, faces(), stream);
   pairs_are_prims = false;
   CHECK(grow((void**)&d_counters, &cap_counters, sizeof(unsigned long long) * C_COUNT, "counters"));
   launch_fill_u64(d_counters, C_COUNT, 0ull, stream);
@@ -313,7 +314,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       launch_fill_u64(d_counters + C_CAND_VF, C_COUNT - C_CAND_VF, 0ull, stream);
       ExpandLaunch E;
       E.pairs = d_pairs; E.npairs_dev = d_counters + C_LEAF_PAIRS; E.pair_cap = pair_cap; E.npairs_hint = npairs;
-      E.faces = d_face; E.blocks_per_cu = blocks_per_cu; E.chunk = chunk;
+      E.faces = faces(); E.blocks_per_cu = blocks_per_cu; E.chunk = chunk;
       const long long nch = expand_chunks(pair_cap) + 1;
       CHECK(grow((void**)&d_ex_off, &cap_ex_off, sizeof(unsigned long long) * 2 * (size_t)nch, "candidate offsets"));
       CHECK(grow((void**)&d_ex_part, &cap_ex_part, sizeof(unsigned long long) * (size_t)scan_part_words(nch),
@@ -385,7 +386,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     }
   }
   if (keep_pairs && npairs > 0) {  // sorted positions -> primitive pairs, lower rank first
-    launch_pairs_to_prims(dim, d_pairs, npairs, d_face, stream);
+    launch_pairs_to_prims(d_pairs, npairs, faces(), stream);
     HIPCHECK(this, hipGetLastError());
     pairs_are_prims = true;
   }
@@ -414,7 +415,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
-                  d_vtmp, d_hist, d_sboxes, d_face, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
+                  d_vtmp, d_hist, d_sboxes, d_face, d_v1, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)

@@ -21,8 +21,10 @@ template <> struct VRec<2> { static constexpr int stride = 4; static constexpr i
 template <int D> struct LeafBox { double sw_lo[D], sw_hi[D], t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D]; };
 
 // Traversal node: the three boxes rounded outward to FP32 (conservative for
-// the culling test), the first child (internal) or primitive index (leaf),
-// and meta = (escape+1) | active<<31 where escape 0 means "end of traversal".
+// the culling test), and meta = (escape+1) | active<<31 where escape 0 means
+// "end of traversal".  child is the first child of an internal node; of a
+// leaf it is 1 if its FP64 t0 or t1 box is empty (then both FP32 boxes hold
+// the swept box, so that no culling test separates it at t0 and t1), else 0.
 template <int D> struct alignas(16) NodeF;
 template <> struct alignas(16) NodeF<3> {
   float sw_lo[3], sw_hi[3], t0_lo[3], t0_hi[3], t1_lo[3], t1_hi[3];
@@ -36,44 +38,31 @@ template <> struct alignas(16) NodeF<2> {
   int pad[2];
 };
 
-// Per-face record in sorted (Morton) order, rebuilt by every traverse from
-// its P, the reptri flags and V1: the candidate kernel's one gather per face
-// (a single 128 B line in 3D).  rank orients pairs (lower rank first); it is
-// the primitive index unless a rank was set.
-template <int D> struct FaceRec;
-template <> struct alignas(128) FaceRec<3> {
-  int p[3];
-  uint32_t flags;
-  int prim, rank, pad[2];
-  double v1[9];
-  double pad2[3];
+// Face data of the candidate kernels, which gather two faces per leaf pair at
+// random: compact arrays in sorted (Morton) order, rebuilt by every traverse,
+// and V1 per vertex.  A 16 B head puts eight leaves in one 128 B line, so the
+// gathers touch far fewer lines than one whole-face record per leaf would.
+//   head[s]  {p0, p1, p2, reptri flags} (3D) / {p0, p1, flags, 0} (2D)
+//   rank[s]  orients pairs (lower rank first); the primitive index unless a rank was set
+//   prim[s]  primitive index (discrete hits, kept leaf pairs)
+//   v1[v]    V1 of vertex v: {x, y, z, 0} (3D) / {x, y} (2D)
+struct FaceArrays {
+  int4* head = nullptr;
+  int* rank = nullptr;
+  int* prim = nullptr;
+  double* v1 = nullptr;
 };
-template <> struct alignas(64) FaceRec<2> {
-  int p[2];
-  uint32_t flags;
-  int prim, rank, pad[3];
-  double v1[4];
-};
-
-// The used prefix of a FaceRec (header + V1), loaded as one unit.
-template <int D> struct FaceHead;
-template <> struct FaceHead<3> {
-  int p[3];
-  uint32_t flags;
-  int prim, rank, pad[2];
-  double v1[9];
-  double pad2;
-};
-template <> struct FaceHead<2> {
-  int p[2];
-  uint32_t flags;
-  int prim, rank, pad[3];
-  double v1[4];
-};
-static_assert(sizeof(FaceHead<3>) == 112 && sizeof(FaceHead<2>) == 64, "face head layout");
-
-inline size_t face_bytes(int dim) { return dim == 3 ? sizeof(FaceRec<3>) : sizeof(FaceRec<2>); }
-static_assert(sizeof(FaceRec<3>) == 128 && sizeof(FaceRec<2>) == 64, "face record layout");
+inline size_t face_bytes(size_t np) { return (sizeof(int4) + 2 * sizeof(int)) * np; }
+inline size_t v1_stride(int dim) { return dim == 3 ? 4 : 2; }
+// The leaf arrays inside one allocation of face_bytes(np) bytes.
+inline FaceArrays face_arrays(void* base, size_t np, double* v1) {
+  FaceArrays f;
+  f.head = static_cast<int4*>(base);
+  f.rank = reinterpret_cast<int*>(f.head + np);
+  f.prim = f.rank + np;
+  f.v1 = v1;
+  return f;
+}
 
 // Device counters of one traversal (all 64-bit).
 enum Counter : int {

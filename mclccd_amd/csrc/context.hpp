@@ -43,7 +43,8 @@ struct Ctx {
   int* d_evals = nullptr; size_t cap_evals = 0;
   void* d_boxes = nullptr; size_t cap_boxes = 0;
   void* d_sboxes = nullptr; size_t cap_sboxes = 0;  // FP64 leaf boxes in sorted order
-  void* d_face = nullptr; size_t cap_face = 0;      // FaceRec per sorted leaf (rebuilt per traverse)
+  void* d_face = nullptr; size_t cap_face = 0;      // face arrays: head, rank, prim per sorted leaf (per traverse)
+  double* d_v1 = nullptr; size_t cap_v1 = 0;        // V1 per vertex (per traverse)
   bool pairs_are_prims = false;                     // d_pairs converted to primitive pairs
   unsigned long long* d_ex_off = nullptr; size_t cap_ex_off = 0;    // candidate offsets per 64-pair chunk
   unsigned long long* d_ex_part = nullptr; size_t cap_ex_part = 0;  // their scan's chunk sums
@@ -111,6 +112,7 @@ struct Ctx {
   int stage(const double* V0, const double* V1, int nv, const int* P, long long nidx, const int* active);
   int update(const double* V0, const double* V1, int nv, const int* P, int np, int pdim, const int* active);
   int traverse(const double* V0, const double* V1, const int* P, unsigned flags);
+  FaceArrays faces() { return face_arrays(d_face, (size_t)np, d_v1); }
 };
 
 }  // namespace mclg

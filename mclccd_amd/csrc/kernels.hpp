@@ -20,7 +20,8 @@ struct SceneArgs {
   int* root = nullptr;                 // per scene subtree root node
 };
 
-void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s);
+// rec: per-vertex records (both time steps); v1 (may be null): V1 per vertex, v1_stride(dim) doubles each.
+void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1 = nullptr);
 void launch_minmax_i32(const int* a, long long n, int* out2, hipStream_t s);
 void launch_fill_u8(uint8_t* p, long long n, uint8_t v, hipStream_t s);
 void launch_fill_i32(int* p, long long n, int v, hipStream_t s);
@@ -54,11 +55,11 @@ void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32
 void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
                      unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_root, int waves,
                      hipStream_t s);
-// faces[s] for sorted position s (prim = order[s]); rank may be null.
+// The leaf arrays of faces (head, rank, prim) for sorted positions (prim = order[s]); rank may be null.
 void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
-                         const double* rec, void* faces, hipStream_t s);
+                         const FaceArrays& faces, hipStream_t s);
 // pairs[i] (sorted positions) -> primitive pairs, lower rank first.
-void launch_pairs_to_prims(int dim, int2* pairs, long long n, const void* faces, hipStream_t s);
+void launch_pairs_to_prims(int2* pairs, long long n, const FaceArrays& faces, hipStream_t s);
 
 // Kernels that follow a variable-size queue read its length from the device
 // counter (so a captured graph needs no host round trip); the *_hint sizes
@@ -68,7 +69,7 @@ struct ExpandLaunch {
   const unsigned long long* npairs_dev;
   unsigned long long pair_cap;
   long long npairs_hint;
-  const void* faces;  // FaceRec<dim> in sorted order (pairs hold sorted positions)
+  FaceArrays faces;  // leaf arrays in sorted order (pairs hold sorted positions) and V1 per vertex
   // scratch of expand_chunks(pair_cap) + 1 entries each, and scan_part_words of that
   unsigned long long* off_vf;
   unsigned long long* off_ee;

@@ -3,10 +3,12 @@
 // k_traverse: one lane per sorted leaf s; the lane walks the LBVH stacklessly
 //   from escape(leaf s), i.e. over exactly the subtrees covering leaves s+1..n-1,
 //   so every unordered leaf pair is visited once.  Internal nodes are culled with
-//   the FP32 outward-rounded boxes; at a leaf the reference's exact FP64 test
-//   (BVHTree.cpp:401-426) decides, on FP64 boxes kept in sorted order.  Pairs
-//   (sorted positions) are staged per wave in LDS and appended in batches.
-// k_face_records: per sorted leaf, P, reptri flags, rank and V1 in one record.
+//   the FP32 outward-rounded boxes; at a leaf the reference's exact test
+//   (BVHTree.cpp:401-426) is decided on the FP32 bounds when they settle it and
+//   on the FP64 boxes (kept in sorted order) otherwise.  Pairs (sorted
+//   positions) are staged per wave in LDS and appended in batches.
+// k_face_records: per sorted leaf, the compact head (P, reptri flags), rank
+//   and primitive index.
 // k_expand: one lane per leaf pair: orientation, the discrete test and the
 //   representative-triangle candidates (BVHTree.cpp:252-398) into the VF / EE
 //   stencil queues.
@@ -67,50 +69,55 @@ __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const Le
 
 // The same test decided from the FP32 node boxes when it can be.  A node's
 // lower bound a = f32_down(lo) brackets the FP64 value as lo in [a, a+) and an
-// upper bound b = f32_up(hi) as hi in (b-, b] (one float step each way), so
-// every comparison of the test is certainly true, certainly false, or
-// undecided; the FP64 boxes are read only when the combination is undecided.
+// upper bound b = f32_up(hi) as hi in (b-, b] (a+ and b- one float step up and
+// down), so lo <= hi is certainly true iff a+ < b (equivalently a < b-) and
+// certainly false iff a > b.  Only the query's bounds are ever stepped, once
+// per lane (QSteps).  Emptiness of the t0 / t1 boxes comes exactly from the
+// FP64 boxes (the leaf's child flag).  The FP64 boxes are read only when the
+// combination is undecided.
 struct Tri {
   bool t, f;  // certainly true / certainly false
 };
 __device__ __forceinline__ Tri tri_and(Tri x, Tri y) { return {bool(x.t & y.t), bool(x.f | y.f)}; }
 __device__ __forceinline__ Tri tri_or(Tri x, Tri y) { return {bool(x.t | y.t), bool(x.f & y.f)}; }
-// lo <= hi from a = bound of lo (with a+ = step_up(a)) and b = bound of hi (with b- = step_down(b))
-__device__ __forceinline__ Tri lo_le_hi(float a, float ap, float b, float bm) {
-  const bool fin = __builtin_isfinite(a) && __builtin_isfinite(b);
-  return {bool(fin & (ap <= bm)), bool(fin & (a > b))};
-}
-__device__ __forceinline__ Tri lo_gt_hi(float a, float ap, float b, float bm) {
-  const Tri r = lo_le_hi(a, ap, b, bm);
-  return {r.f, r.t};
+
+// a+ of the query's lower bounds and b- of its upper bounds
+template <int D> struct QSteps {
+  float sw_lo_p[D], sw_hi_m[D], t0_lo_p[D], t0_hi_m[D], t1_lo_p[D], t1_hi_m[D];
+};
+template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>& q) {
+  QSteps<D> p;
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    p.sw_lo_p[k] = step_up(q.sw_lo[k]); p.sw_hi_m[k] = step_down(q.sw_hi[k]);
+    p.t0_lo_p[k] = step_up(q.t0_lo[k]); p.t0_hi_m[k] = step_down(q.t0_hi[k]);
+    p.t1_lo_p[k] = step_up(q.t1_lo[k]); p.t1_hi_m[k] = step_down(q.t1_hi[k]);
+  }
+  return p;
 }
 
-// leaf_test(query, leaf) in three-valued logic on the FP32 bounds.
-template <int D> __device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, bool qa, const NodeF<D>& b, bool ba) {
-  Tri ov = {bool(qa | ba), !(qa | ba)};
-  Tri emp = {false, true};
+// leaf_test(query, leaf) in three-valued logic on the FP32 bounds; qemp: the
+// query's t0 or t1 box is empty.  A pair that cull_f32 would reject is
+// certainly false here, so leaves need no separate cull.
+template <int D>
+__device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const QSteps<D>& qp, bool qa, bool qemp,
+                                             const NodeF<D>& b, bool ba) {
+  const bool act = qa | ba;
+  Tri ov = {act, !act};
   Tri sep = {false, true};
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    const float q_sw_lo_p = step_up(q.sw_lo[k]), q_sw_hi_m = step_down(q.sw_hi[k]);
-    const float q_t0_lo_p = step_up(q.t0_lo[k]), q_t0_hi_m = step_down(q.t0_hi[k]);
-    const float q_t1_lo_p = step_up(q.t1_lo[k]), q_t1_hi_m = step_down(q.t1_hi[k]);
-    const float b_sw_lo_p = step_up(b.sw_lo[k]), b_sw_hi_m = step_down(b.sw_hi[k]);
-    const float b_t0_lo_p = step_up(b.t0_lo[k]), b_t0_hi_m = step_down(b.t0_hi[k]);
-    const float b_t1_lo_p = step_up(b.t1_lo[k]), b_t1_hi_m = step_down(b.t1_hi[k]);
-    ov = tri_and(ov, lo_le_hi(q.sw_lo[k], q_sw_lo_p, b.sw_hi[k], b_sw_hi_m));
-    ov = tri_and(ov, lo_le_hi(b.sw_lo[k], b_sw_lo_p, q.sw_hi[k], q_sw_hi_m));
-    emp = tri_or(emp, lo_gt_hi(q.t0_lo[k], q_t0_lo_p, q.t0_hi[k], q_t0_hi_m));
-    emp = tri_or(emp, lo_gt_hi(b.t0_lo[k], b_t0_lo_p, b.t0_hi[k], b_t0_hi_m));
-    emp = tri_or(emp, lo_gt_hi(q.t1_lo[k], q_t1_lo_p, q.t1_hi[k], q_t1_hi_m));
-    emp = tri_or(emp, lo_gt_hi(b.t1_lo[k], b_t1_lo_p, b.t1_hi[k], b_t1_hi_m));
-    const Tri above = tri_and(lo_gt_hi(q.t0_lo[k], q_t0_lo_p, b.t0_hi[k], b_t0_hi_m),
-                              lo_gt_hi(q.t1_lo[k], q_t1_lo_p, b.t1_hi[k], b_t1_hi_m));
-    const Tri below = tri_and(lo_gt_hi(b.t0_lo[k], b_t0_lo_p, q.t0_hi[k], q_t0_hi_m),
-                              lo_gt_hi(b.t1_lo[k], b_t1_lo_p, q.t1_hi[k], q_t1_hi_m));
-    sep = tri_or(sep, tri_or(above, below));
+    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < b.sw_hi[k], q.sw_lo[k] > b.sw_hi[k]});  // q.sw_lo <= b.sw_hi
+    ov = tri_and(ov, Tri{b.sw_lo[k] < qp.sw_hi_m[k], b.sw_lo[k] > q.sw_hi[k]});  // b.sw_lo <= q.sw_hi
+    // q above b at t0 / t1, b above q at t0 / t1 (lo > hi: certain iff a > b, excluded iff a+ < b)
+    const Tri a0 = {q.t0_lo[k] > b.t0_hi[k], qp.t0_lo_p[k] < b.t0_hi[k]};
+    const Tri a1 = {q.t1_lo[k] > b.t1_hi[k], qp.t1_lo_p[k] < b.t1_hi[k]};
+    const Tri b0 = {b.t0_lo[k] > q.t0_hi[k], b.t0_lo[k] < qp.t0_hi_m[k]};
+    const Tri b1 = {b.t1_lo[k] > q.t1_hi[k], b.t1_lo[k] < qp.t1_hi_m[k]};
+    sep = tri_or(sep, tri_or(tri_and(a0, a1), tri_and(b0, b1)));
   }
-  return tri_and(ov, tri_or(emp, Tri{sep.f, sep.t}));
+  const bool emp = qemp | (b.child != 0);
+  return emp ? ov : tri_and(ov, Tri{sep.f, sep.t});
 }
 
 // Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
@@ -164,11 +171,14 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
   if (lane_id() == 0) s_cnt[w] = 0;
   wave_lds_sync();
   NodeF<D> q;
-  bool qa = false;
+  QSteps<D> qp;
+  bool qa = false, qemp = false;
   int node = -1, end = -1;
   if (valid) {
     q = load16(nodes + (n - 1) + s);
     qa = (q.meta >> 31) != 0;
+    qemp = q.child != 0;
+    qp = query_steps<D>(q);
     node = (int)(q.meta & 0x7fffffffu) - 1;
     // several scenes: the walk ends where the pre-order leaves this leaf's scene subtree
     if (scene_root) end = (int)(nodes[scene_root[scene_of_sorted[s]]].meta & 0x7fffffffu) - 1;
@@ -186,19 +196,17 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
     bool emit = false;
     int k = 0;
     int next = (int)(nd.meta & 0x7fffffffu) - 1;
-    if (cull_f32<D>(q, nd)) {
-      if (node >= n - 1) {
-        k = node - (n - 1);
-        const bool ba = (nd.meta >> 31) != 0;
-        const Tri r = leaf_test_f32<D>(q, qa, nd, ba);
-        if (r.t | r.f) {
-          emit = r.t;
-        } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
-          emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
-        }
-      } else {
-        next = nd.child;
+    if (node >= n - 1) {
+      k = node - (n - 1);
+      const bool ba = (nd.meta >> 31) != 0;
+      const Tri r = leaf_test_f32<D>(q, qp, qa, qemp, nd, ba);
+      if (r.t | r.f) {
+        emit = r.t;
+      } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
+        emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
       }
+    } else if (cull_f32<D>(q, nd)) {
+      next = nd.child;
     }
     if (next == end) next = -1;
     const uint64_t m = __ballot(emit);
@@ -238,54 +246,42 @@ void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2
 }
 
 // ---------------------------------------------------------------------------
-// Face records and pair conversion
+// Face arrays and pair conversion
 // ---------------------------------------------------------------------------
 template <int D>
 __global__ void k_face_records(const uint32_t* __restrict__ order, int n, const int* __restrict__ P,
-                               const uint8_t* __restrict__ flags, const int* __restrict__ rank,
-                               const double* __restrict__ rec, FaceRec<D>* __restrict__ faces) {
+                               const uint8_t* __restrict__ flags, const int* __restrict__ rank, FaceArrays F) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const int prim = (int)order[s];
-  FaceRec<D> f = {};
-#pragma unroll
-  for (int j = 0; j < D; ++j) f.p[j] = P[(size_t)prim * D + j];
-  f.flags = flags[prim];
-  f.prim = prim;
-  f.rank = rank ? rank[prim] : prim;
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    const typename VLoad<D>::V v = VLoad<D>::load_t1(rec, f.p[j]);
-    f.v1[D * j + 0] = v.x;
-    f.v1[D * j + 1] = v.y;
-    if constexpr (D == 3) f.v1[D * j + 2] = v.z;
-  }
-  faces[s] = f;
+  const int* p = P + (size_t)prim * D;
+  const int fl = (int)flags[prim];
+  F.head[s] = D == 3 ? make_int4(p[0], p[1], p[D - 1], fl) : make_int4(p[0], p[1], fl, 0);
+  F.rank[s] = rank ? rank[prim] : prim;
+  F.prim[s] = prim;
 }
 
 void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
-                         const double* rec, void* faces, hipStream_t s) {
+                         const FaceArrays& faces, hipStream_t s) {
   if (n < 1) return;
   const int g = (n + 255) / 256;
-  if (dim == 3) k_face_records<3><<<g, 256, 0, s>>>(order, n, P, flags, rank, rec, (FaceRec<3>*)faces);
-  else k_face_records<2><<<g, 256, 0, s>>>(order, n, P, flags, rank, rec, (FaceRec<2>*)faces);
+  if (dim == 3) k_face_records<3><<<g, 256, 0, s>>>(order, n, P, flags, rank, faces);
+  else k_face_records<2><<<g, 256, 0, s>>>(order, n, P, flags, rank, faces);
 }
 
-template <int D>
-__global__ void k_pairs_to_prims(int2* pairs, long long n, const FaceRec<D>* __restrict__ faces) {
+__global__ void k_pairs_to_prims(int2* pairs, long long n, FaceArrays F) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int2 pr = pairs[i];
-  const int ra = faces[pr.x].rank, rb = faces[pr.y].rank;
-  const int a = faces[pr.x].prim, b = faces[pr.y].prim;
+  const int ra = F.rank[pr.x], rb = F.rank[pr.y];
+  const int a = F.prim[pr.x], b = F.prim[pr.y];
   pairs[i] = ra < rb ? make_int2(a, b) : make_int2(b, a);
 }
 
-void launch_pairs_to_prims(int dim, int2* pairs, long long n, const void* faces, hipStream_t s) {
+void launch_pairs_to_prims(int2* pairs, long long n, const FaceArrays& faces, hipStream_t s) {
   if (n < 1) return;
   const unsigned g = (unsigned)((n + 255) / 256);
-  if (dim == 3) k_pairs_to_prims<3><<<g, 256, 0, s>>>(pairs, n, (const FaceRec<3>*)faces);
-  else k_pairs_to_prims<2><<<g, 256, 0, s>>>(pairs, n, (const FaceRec<2>*)faces);
+  k_pairs_to_prims<<<g, 256, 0, s>>>(pairs, n, faces);
 }
 
 // ---------------------------------------------------------------------------
@@ -330,7 +326,7 @@ struct ExpandArgs {
   const int2* pairs;                     // sorted positions
   const unsigned long long* npairs_dev;  // pairs found by the traversal (device counter)
   unsigned long long pair_cap;
-  const void* faces;                     // FaceRec<D> by sorted position
+  FaceArrays faces;                      // leaf arrays by sorted position, V1 by vertex
   int discrete, continuous;
   int4* vfq;
   unsigned long long vf_cap;
@@ -349,24 +345,43 @@ struct ExpandArgs {
 
 constexpr int EX_WAVES = 4, EX_DSTAGE = 128;
 
-// The pair's two faces, lower rank first (BVHTree.cpp:252-283 visits each leaf
-// pair once in the KdBVH's order; the rank reproduces that orientation).
-template <int D>
-__device__ __forceinline__ void load_pair(const ExpandArgs& A, long long i, FaceHead<D>& ha, FaceHead<D>& hb) {
-  const FaceRec<D>* F = static_cast<const FaceRec<D>*>(A.faces);
-  const int2 sp = A.pairs[i];
-  ha = load16(reinterpret_cast<const FaceHead<D>*>(F + sp.x));  // both records at once
-  hb = load16(reinterpret_cast<const FaceHead<D>*>(F + sp.y));
-  if (!(ha.rank < hb.rank)) {
-    const FaceHead<D> t = ha;
-    ha = hb;
-    hb = t;
+// A face's vertex ids and reptri flags (bits v0..v2, e0..e2) from its head.
+template <int D> struct FaceIds {
+  int p[D];
+  uint32_t flags;
+};
+template <int D> __device__ __forceinline__ FaceIds<D> face_ids(int4 h) {
+  FaceIds<D> f;
+  f.p[0] = h.x;
+  f.p[1] = h.y;
+  if constexpr (D == 3) {
+    f.p[2] = h.z;
+    f.flags = (uint32_t)h.w;
+  } else {
+    f.flags = (uint32_t)h.z;
   }
+  return f;
+}
+
+// The pair's two faces, lower rank first (BVHTree.cpp:252-283 visits each leaf
+// pair once in the KdBVH's order; the rank reproduces that orientation); pa,
+// pb receive their sorted positions.
+template <int D>
+__device__ __forceinline__ void load_pair(const ExpandArgs& A, long long i, FaceIds<D>& fa, FaceIds<D>& fb, int& pa,
+                                          int& pb) {
+  const int2 sp = A.pairs[i];
+  const int4 x = A.faces.head[sp.x], y = A.faces.head[sp.y];
+  const int rx = A.faces.rank[sp.x], ry = A.faces.rank[sp.y];
+  const bool keep = rx < ry;
+  fa = face_ids<D>(keep ? x : y);
+  fb = face_ids<D>(keep ? y : x);
+  pa = keep ? sp.x : sp.y;
+  pb = keep ? sp.y : sp.x;
 }
 
 // BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (3i + j)
 template <int D>
-__device__ __forceinline__ void candidate_masks(const FaceHead<D>& ha, const FaceHead<D>& hb, uint32_t& vfm,
+__device__ __forceinline__ void candidate_masks(const FaceIds<D>& ha, const FaceIds<D>& hb, uint32_t& vfm,
                                                 uint32_t& eem) {
   vfm = 0;
   eem = 0;
@@ -407,20 +422,9 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand_co
     if (ch < nchunks && A.continuous) {
       const long long i = ch * kWave + lane;
       if (i < npairs) {
-        // counts do not depend on orientation or V1: the records' first 16 bytes (P, flags) suffice
-        const FaceRec<D>* F = static_cast<const FaceRec<D>*>(A.faces);
+        // counts do not depend on orientation: the heads suffice
         const int2 sp = A.pairs[i];
-        const int4 a = *reinterpret_cast<const int4*>(F + sp.x);
-        const int4 b = *reinterpret_cast<const int4*>(F + sp.y);
-        FaceHead<D> ha = {}, hb = {};
-        ha.p[0] = a.x; ha.p[1] = a.y; hb.p[0] = b.x; hb.p[1] = b.y;
-        if constexpr (D == 3) {
-          ha.p[2] = a.z; ha.flags = (uint32_t)a.w;
-          hb.p[2] = b.z; hb.flags = (uint32_t)b.w;
-        } else {
-          ha.flags = (uint32_t)a.z;
-          hb.flags = (uint32_t)b.z;
-        }
+        const FaceIds<D> ha = face_ids<D>(A.faces.head[sp.x]), hb = face_ids<D>(A.faces.head[sp.y]);
         uint32_t vfm, eem;
         candidate_masks<D>(ha, hb, vfm, eem);
         cv = (uint32_t)__popc(vfm);
@@ -445,10 +449,63 @@ __global__ void k_expand_totals(const unsigned long long* off_vf, const unsigned
   counters[C_CAND_EE] = off_ee[nch];
 }
 
+// V1 of a face's vertices (from the per-vertex V1 array).
+template <int D> __device__ __forceinline__ void face_v1(const double* v1, const FaceIds<D>& f, double* x) {
+#pragma unroll
+  for (int j = 0; j < D; ++j) {
+    if constexpr (D == 3) {
+      const double2* r = reinterpret_cast<const double2*>(v1 + 4 * (size_t)f.p[j]);
+      const double2 r0 = r[0], r1 = r[1];
+      x[3 * j] = r0.x; x[3 * j + 1] = r0.y; x[3 * j + 2] = r1.x;
+    } else {
+      const double2 r0 = reinterpret_cast<const double2*>(v1 + 2 * (size_t)f.p[j])[0];
+      x[2 * j] = r0.x; x[2 * j + 1] = r0.y;
+    }
+  }
+}
+
+// A wave's stencils of one chunk, staged in LDS in lane order and written out
+// by consecutive lanes (coalesced) at the chunk's scanned offset.  Vertex ids
+// are plain values (a0..a2 of the first face, b0..b2 of the second): indexing
+// an array, or selecting between two, would place them in scratch memory.
+template <int D, bool VF>
+__device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a0, int a1, int a2, int b0, int b1,
+                                               int b2, int4* __restrict__ dst, unsigned long long base,
+                                               unsigned long long cap) {
+  const uint32_t count = (uint32_t)__popc(mask);
+  uint32_t p = wave_prefix_u32(count);
+  const uint32_t total = __shfl(p + count, kWave - 1);
+  while (mask) {
+    const int c = __ffs(mask) - 1;
+    mask &= mask - 1;
+    int4 st;
+    if constexpr (VF) {  // vertex k of one face against the other face
+      const bool fwd = c < 3;
+      const int k = fwd ? c : c - 3;
+      st.x = fwd ? (k == 0 ? a0 : (k == 1 ? a1 : a2)) : (k == 0 ? b0 : (k == 1 ? b1 : b2));
+      st.y = fwd ? b0 : a0;
+      st.z = fwd ? b1 : a1;
+      st.w = D == 3 ? (fwd ? b2 : a2) : -1;
+    } else {  // edge x of the first face against edge y of the second
+      const int x = c / 3, y = c - 3 * (c / 3);
+      st.x = x == 0 ? a0 : (x == 1 ? a1 : a2);
+      st.y = x == 0 ? a1 : (x == 1 ? a2 : a0);
+      st.z = y == 0 ? b0 : (y == 1 ? b1 : b2);
+      st.w = y == 0 ? b1 : (y == 1 ? b2 : b0);
+    }
+    stage[p++] = st;
+  }
+  wave_lds_sync();
+  for (uint32_t j = lane_id(); j < total; j += kWave)
+    if (base + j < cap) dst[base + j] = stage[j];
+  wave_lds_sync();
+}
+
 // Pass 2: the discrete test and the candidates, written at the scanned offsets
 // (each chunk's stencils in lane order: the queues are deterministic).
 template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
+  __shared__ int4 s_st[EX_WAVES][kWave * 9];  // at most 9 EE (6 VF) stencils per pair
   const int w = threadIdx.x / kWave, lane = lane_id();
   int nd = 0;  // staged discrete records (wave-uniform)
   const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
@@ -458,8 +515,9 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
   for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
     const long long i = ch * kWave + lane;
     const bool valid = i < npairs;
-    FaceHead<D> ha = {}, hb = {};
-    if (valid) load_pair<D>(A, i, ha, hb);
+    FaceIds<D> ha = {}, hb = {};
+    int pa = 0, pb = 0;
+    if (valid) load_pair<D>(A, i, ha, hb, pa, pb);
     if (A.discrete) {  // BVHTree.cpp:464-500
       bool shared = false;
 #pragma unroll
@@ -468,8 +526,9 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
         for (int y = 0; y < D; ++y) shared |= ha.p[x] == hb.p[y];
       int dh = 0;
       if (valid && !shared) {
-        const double* a = ha.v1;
-        const double* b = hb.v1;
+        double a[3 * D], b[3 * D];
+        face_v1<D>(A.faces.v1, ha, a);
+        face_v1<D>(A.faces.v1, hb, b);
         if constexpr (D == 3) {
           dh = tri_tri3(d3{a[0], a[1], a[2]}, d3{a[3], a[4], a[5]}, d3{a[6], a[7], a[8]}, d3{b[0], b[1], b[2]},
                         d3{b[3], b[4], b[5]}, d3{b[6], b[7], b[8]});
@@ -477,48 +536,17 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
           dh = edge_edge2(d2{a[0], a[1]}, d2{a[2], a[3]}, d2{b[0], b[1]}, d2{b[2], b[3]});
         }
       }
-      stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, make_int2(ha.prim, hb.prim), A.disc, A.disc_cap,
-                                  A.counters + C_DISCRETE);
+      int2 rec = make_int2(0, 0);
+      if (dh) rec = make_int2(A.faces.prim[pa], A.faces.prim[pb]);
+      stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, rec, A.disc, A.disc_cap, A.counters + C_DISCRETE);
     }
     if (!A.continuous) continue;
     uint32_t vfm = 0, eem = 0;
     if (valid) candidate_masks<D>(ha, hb, vfm, eem);
-    // vertex ids as plain values (a select between two arrays would put them in scratch)
     const int x0 = ha.p[0], x1 = ha.p[1], x2 = ha.p[D - 1], y0 = hb.p[0], y1 = hb.p[1], y2 = hb.p[D - 1];
-    // VF: the lane's k-th candidate goes to off_vf[ch] + (lanes before) + k
-    unsigned long long pos = A.off_vf[ch] + wave_prefix_u32((uint32_t)__popc(vfm));
-    while (__ballot(vfm != 0)) {
-      if (vfm) {
-        const int code = __ffs(vfm) - 1;
-        vfm &= vfm - 1;
-        const bool fwd = code < 3;  // vertex of f0 against face f1, else the reverse
-        const int k = fwd ? code : code - 3;
-        int4 st;
-        st.x = fwd ? (k == 0 ? x0 : (k == 1 ? x1 : x2)) : (k == 0 ? y0 : (k == 1 ? y1 : y2));
-        st.y = fwd ? y0 : x0;
-        st.z = fwd ? y1 : x1;
-        st.w = D == 3 ? (fwd ? y2 : x2) : -1;
-        if (pos < A.vf_cap) A.vfq[pos] = st;
-        ++pos;
-      }
-    }
-    if constexpr (D == 3) {
-      pos = A.off_ee[ch] + wave_prefix_u32((uint32_t)__popc(eem));
-      while (__ballot(eem != 0)) {
-        if (eem) {
-          const int c = __ffs(eem) - 1;
-          eem &= eem - 1;
-          const int x = c / 3, y = c - 3 * (c / 3);
-          int4 st;
-          st.x = x == 0 ? x0 : (x == 1 ? x1 : x2);
-          st.y = x == 0 ? x1 : (x == 1 ? x2 : x0);
-          st.z = y == 0 ? y0 : (y == 1 ? y1 : y2);
-          st.w = y == 0 ? y1 : (y == 1 ? y2 : y0);
-          if (pos < A.ee_cap) A.eeq[pos] = st;
-          ++pos;
-        }
-      }
-    }
+    write_stencils<D, true>(s_st[w], vfm, x0, x1, x2, y0, y1, y2, A.vfq, A.off_vf[ch], A.vf_cap);
+    if constexpr (D == 3)
+      write_stencils<D, false>(s_st[w], eem, x0, x1, x2, y0, y1, y2, A.eeq, A.off_ee[ch], A.ee_cap);
   }
   if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
 }

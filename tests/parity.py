@@ -107,3 +107,50 @@ def quirk_stencils(count, seed=11):
         v1[1:] += rng.normal(0, 1e-3, (3, 3))
         out.append((v0, v1))
     return out
+
+
+def boundary_pairs(eta: float, exact_floats: bool = False):
+    """Isolated triangle pairs whose leaf boxes meet within a few FP32 / FP64
+    ulps of the leaf test's threshold (BVHTree.cpp:401-426), along each axis,
+    from above and below, on the swept boxes (static pairs) and on the t1 boxes
+    (t0 separated, so only t1 decides).  Pair j sits near 20 j along x, so the
+    FP32 spacing ranges from 1e-7 to ~1e-3 across the scene.  With
+    exact_floats the coordinates are FP32 values (FP32 bounds then equal the
+    FP64 ones for eta = 0, ties included).  Returns V0, V1, F."""
+    V0, V1, F = [], [], []
+    j = 0
+    for kind in ("swept", "t1"):
+        for axis in range(3):
+            for side in (1, -1):
+                for d32 in range(-3, 4):
+                    for d64 in (range(-2, 3) if not exact_floats else (0,)):
+                        base = np.zeros(3)
+                        base[0] = 20.0 * j + 1.0
+                        c = base[axis] if axis == 0 else 1.0 + 0.25 * (j % 7)
+                        if exact_floats:
+                            c = float(np.float32(c))
+                        u32 = float(np.spacing(np.float32(c)))
+                        u64 = float(np.spacing(c))
+                        gap = 2.0 * eta + d32 * u32 + d64 * u64
+                        e = np.eye(3)[axis]
+                        o1, o2 = np.eye(3)[(axis + 1) % 3], np.eye(3)[(axis + 2) % 3]
+                        org = base.copy()
+                        org[axis] = 0.0
+                        # A: extreme coordinate c along `axis` (max if side=1, min if side=-1)
+                        A = [org + side * c * e, org + side * (c - 0.3) * e + 0.1 * o1,
+                             org + side * (c - 0.2) * e + 0.1 * o2]
+                        b = c + gap  # B's extreme coordinate facing A
+                        B = [org + side * b * e + 0.05 * o1, org + side * (b + 0.3) * e + 0.02 * o2,
+                             org + side * (b + 0.2) * e + 0.1 * o1 + 0.1 * o2]
+                        A, B = np.array(A), np.array(B)
+                        if exact_floats:
+                            A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+                        B0 = B.copy()
+                        if kind == "t1":
+                            B0 = B + side * 0.4 * e  # separated at t0 along the same axis
+                        base_i = len(V0)
+                        V0 += list(A) + list(B0)
+                        V1 += list(A) + list(B)
+                        F += [[base_i, base_i + 1, base_i + 2], [base_i + 3, base_i + 4, base_i + 5]]
+                        j += 1
+    return np.array(V0), np.array(V1), np.array(F, np.int32)

@@ -10,7 +10,8 @@ import pytest
 
 from mclccd_amd import scenes
 from oracle.oracle import run_scene
-from tests.parity import (canon_pairs, canon_stencils, compare_hits, hit_map, oriented_pairs, oriented_stencils)
+from tests.parity import (boundary_pairs, canon_pairs, canon_stencils, compare_hits, hit_map, oriented_pairs,
+                          oriented_stencils)
 
 pytestmark = pytest.mark.gpu
 
@@ -192,3 +193,14 @@ def test_bad_indices_rejected(gpu):
 
     with pytest.raises(MclgError):
         t.update(V0, V1, bad)
+
+
+@pytest.mark.parametrize("eta,exact", [(1e-3, False), (1e-6, False), (0.0, True)])
+def test_leaf_test_at_fp32_boundaries(gpu, eta, exact):
+    # leaf boxes meeting within a few FP32 / FP64 ulps of the leaf test's
+    # threshold: the traversal decides such pairs on FP32 bounds only when the
+    # bracket of the FP64 value settles them, otherwise on the FP64 boxes
+    V0, V1, F = boundary_pairs(eta, exact)
+    t, o = run_both(V0, V1, F, eta, strict=False)
+    c = check_parity(t, o, strict=False)
+    assert 0 < c["leaf_pairs"] < len(F) // 2
