@@ -608,7 +608,11 @@ __global__ void k_hits_total(const unsigned long long* off, long long last, unsi
   counters[C_HITS] = off[last];
 }
 
-// One wave per 64-stencil chunk: the chunk's hits in lane order at off[ch].
+// Each chunk's hits in lane order at off[ch].  A wave takes HC_U consecutive
+// 64-stencil chunks per step and issues all their loads before any store, so
+// each wave keeps several independent loads in flight.
+constexpr int HC_U = 4;
+
 template <bool VF>
 __global__ __launch_bounds__(256) void k_hits_compact(const int4* __restrict__ q, const unsigned long long* n_dev,
                                                       unsigned long long cap, const double* __restrict__ toi,
@@ -617,17 +621,30 @@ __global__ __launch_bounds__(256) void k_hits_compact(const int4* __restrict__ q
   const int lane = lane_id();
   const long long n = (long long)(*n_dev < cap ? *n_dev : cap);
   const long long nchunks = (n + kWave - 1) / kWave;
-  const long long stride = (long long)gridDim.x * (blockDim.x / kWave);
-  for (long long ch = (long long)xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave; ch < nchunks; ch += stride) {
-    const long long i = ch * kWave + lane;
-    const double t = i < n ? toi[i] : -1.0;
-    const bool hit = t >= 0.0;
-    const uint64_t m = __ballot(hit);
-    const unsigned long long pos = off[ch] + (unsigned long long)popc_below(m);
-    if (hit && pos < hit_cap) {
-      hit_sten[pos] = q[i];
-      hit_toi[pos] = t;
-      hit_vf[pos] = VF ? 1 : 0;
+  const long long stride = (long long)gridDim.x * (blockDim.x / kWave) * HC_U;
+  for (long long c0 = ((long long)xcd_block() * (blockDim.x / kWave) + threadIdx.x / kWave) * HC_U; c0 < nchunks;
+       c0 += stride) {
+    double t[HC_U];
+    int4 st[HC_U];
+    unsigned long long base[HC_U];
+#pragma unroll
+    for (int u = 0; u < HC_U; ++u) {
+      const long long i = (c0 + u) * kWave + lane;
+      const bool in = i < n;
+      t[u] = in ? toi[i] : -1.0;
+      st[u] = in ? q[i] : make_int4(0, 0, 0, 0);
+      base[u] = c0 + u < nchunks ? off[c0 + u] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < HC_U; ++u) {
+      const bool hit = t[u] >= 0.0;
+      const uint64_t m = __ballot(hit);
+      const unsigned long long pos = base[u] + (unsigned long long)popc_below(m);
+      if (hit && pos < hit_cap) {
+        hit_sten[pos] = st[u];
+        hit_toi[pos] = t[u];
+        hit_vf[pos] = VF ? 1 : 0;
+      }
     }
   }
 }
@@ -689,11 +706,11 @@ void launch_hits(int dim, const HitsLaunch& L, bool scan, hipStream_t s) {
     launch_scan_u64(L.off, nv + ne + 1, L.scan_part, s);
     k_hits_total<<<1, 1, 0, s>>>(L.off, nv + ne, L.counters);
   }
-  const int gv = grid_for(nv, 4, L.num_cus, L.blocks_per_cu);
+  const int gv = grid_for((nv + HC_U - 1) / HC_U, 4, L.num_cus, L.blocks_per_cu);
   k_hits_compact<true><<<gv, 256, 0, s>>>(L.vfq, L.n_vf_dev, L.vf_cap, L.vf_toi, L.off, L.hit_sten, L.hit_vf,
                                          L.hit_toi, L.hit_cap);
   if (dim == 3 && ne > 0) {
-    const int ge = grid_for(ne, 4, L.num_cus, L.blocks_per_cu);
+    const int ge = grid_for((ne + HC_U - 1) / HC_U, 4, L.num_cus, L.blocks_per_cu);
     k_hits_compact<false><<<ge, 256, 0, s>>>(L.eeq, L.n_ee_dev, L.ee_cap, L.ee_toi, L.off + nv, L.hit_sten,
                                             L.hit_vf, L.hit_toi, L.hit_cap);
   }
