@@ -140,36 +140,44 @@ template <typename T, bool PIN = false> __device__ __forceinline__ T load16(cons
 }
 
 // Pairs found by a wave are staged in LDS and written out in batches: one
-// global atomic per flush instead of one per emitting step.
+// global atomic per flush instead of one per emitting step.  A staged entry is
+// (lane | r << 8, k): the r-th pair of that lane since the last flush.  The
+// flush writes each lane's pairs contiguously (a query leaf's pairs stay
+// together, so the candidate kernels find its face data in cache).
 constexpr int TR_MAX_WAVES = 8, TR_BUF = 512;
 
-__device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,
+// Called by the whole wave (converged); mine = this lane's staged pairs.
+__device__ __forceinline__ void flush_pairs(const int2* buf, int* pre, int cnt, uint32_t mine, int s_wave,
+                                            int2* __restrict__ pairs, unsigned long long cap,
                                             unsigned long long* counter) {
-  // executed by the wave's active lanes (inside or after the divergent walk)
-  const uint64_t act = __ballot(1);
-  const int nact = __popcll(act), r = popc_below(act);
-  const int leader = __ffsll((long long)act) - 1;
+  const int lane = lane_id();
+  wave_lds_sync();
   unsigned long long base = 0;
-  if (r == 0) base = atomicAdd(counter, (unsigned long long)cnt);
-  base = __shfl(base, leader);
-  for (int i = r; i < cnt; i += nact)
-    if (base + i < cap) pairs[base + i] = buf[i];
+  if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
+  base = __shfl(base, 0);
+  pre[lane] = (int)wave_prefix_u32(mine);
+  wave_lds_sync();
+  for (int i = lane; i < cnt; i += kWave) {
+    const int2 e = buf[i];
+    const int L = e.x & 255;
+    const unsigned long long pos = base + (unsigned long long)(pre[L] + (e.x >> 8));
+    if (pos < cap) pairs[pos] = make_int2(s_wave + L, e.y);
+  }
+  wave_lds_sync();
 }
 
+// At most 128 VGPRs: four waves per SIMD, the occupancy the LDS staging allows.
 template <int D>
-__global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D>* __restrict__ nodes,
-                                                               const LeafBox<D>* __restrict__ sboxes, int n,
-                                                               int2* __restrict__ pairs, unsigned long long cap,
-                                                               unsigned long long* counters,
-                                                               const uint32_t* __restrict__ scene_of_sorted,
-                                                               const int* __restrict__ scene_root) {
+__global__ __launch_bounds__(TR_MAX_WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4)))
+void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict__ sboxes, int n,
+                int2* __restrict__ pairs, unsigned long long cap, unsigned long long* counters,
+                const uint32_t* __restrict__ scene_of_sorted, const int* __restrict__ scene_root) {
   __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
-  __shared__ int s_cnt[TR_MAX_WAVES];
-  const int w = threadIdx.x / kWave;
+  __shared__ int s_pre[TR_MAX_WAVES][kWave];
+  const int w = threadIdx.x / kWave, lane = lane_id();
   const int s = xcd_block() * blockDim.x + threadIdx.x;
+  const int s_wave = s - lane;
   const bool valid = s < n - 1;
-  if (lane_id() == 0) s_cnt[w] = 0;
-  wave_lds_sync();
   NodeF<D> q;
   QSteps<D> qp;
   bool qa = false, qemp = false;
@@ -187,48 +195,47 @@ __global__ __launch_bounds__(TR_MAX_WAVES * kWave) void k_traverse(const NodeF<D
   // a pre-order walk visits each of the 2n-1 nodes at most once: more steps
   // than that means a malformed tree, which is reported instead of looping
   int budget = 2 * n;
-  while (node >= 0) {
-    if (--budget < 0) {
-      atomicAdd(counters + C_ERRORS, 1ull);
-      break;
-    }
-    const NodeF<D> nd = load16<NodeF<D>, true>(nodes + node);
+  int cnt = 0;        // staged pairs of the wave (uniform)
+  uint32_t mine = 0;  // staged pairs of this lane
+  // the wave steps while any lane walks; appends and flushes run converged
+  while (__ballot(node >= 0)) {
     bool emit = false;
     int k = 0;
-    int next = (int)(nd.meta & 0x7fffffffu) - 1;
-    if (node >= n - 1) {
-      k = node - (n - 1);
-      const bool ba = (nd.meta >> 31) != 0;
-      const Tri r = leaf_test_f32<D>(q, qp, qa, qemp, nd, ba);
-      if (r.t | r.f) {
-        emit = r.t;
-      } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
-        emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
+    if (node >= 0) {
+      if (--budget < 0) {
+        atomicAdd(counters + C_ERRORS, 1ull);
+        node = -1;
+      } else {
+        const NodeF<D> nd = load16<NodeF<D>, true>(nodes + node);
+        int next = (int)(nd.meta & 0x7fffffffu) - 1;
+        if (node >= n - 1) {
+          k = node - (n - 1);
+          const bool ba = (nd.meta >> 31) != 0;
+          const Tri r = leaf_test_f32<D>(q, qp, qa, qemp, nd, ba);
+          if (r.t | r.f) {
+            emit = r.t;
+          } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
+            emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
+          }
+        } else if (cull_f32<D>(q, nd)) {
+          next = nd.child;
+        }
+        node = next == end ? -1 : next;
       }
-    } else if (cull_f32<D>(q, nd)) {
-      next = nd.child;
     }
-    if (next == end) next = -1;
     const uint64_t m = __ballot(emit);
     if (m) {
-      const int cnt = s_cnt[w];
-      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(s, k);
-      const int nc = cnt + __popcll(m);
-      wave_lds_sync();
-      if (nc > TR_BUF - kWave) {
-        flush_pairs(s_buf[w], nc, pairs, cap, counters + C_LEAF_PAIRS);
-        wave_lds_sync();
-        if (popc_below(__ballot(1)) == 0) s_cnt[w] = 0;
-      } else if (popc_below(__ballot(1)) == 0) {
-        s_cnt[w] = nc;
+      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(lane | (int)(mine << 8), k);
+      mine += emit ? 1u : 0u;
+      cnt += __popcll(m);
+      if (cnt > TR_BUF - kWave) {
+        flush_pairs(s_buf[w], s_pre[w], cnt, mine, s_wave, pairs, cap, counters + C_LEAF_PAIRS);
+        cnt = 0;
+        mine = 0;
       }
-      wave_lds_sync();
     }
-    node = next;
   }
-  wave_lds_sync();
-  const int cnt = s_cnt[w];
-  if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
+  if (cnt > 0) flush_pairs(s_buf[w], s_pre[w], cnt, mine, s_wave, pairs, cap, counters + C_LEAF_PAIRS);
 }
 
 void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
