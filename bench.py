@@ -55,37 +55,44 @@ def profiled_traffic(kernel):
     return None
 
 
-def roofline(c, avg, n):
+def roofline(c, avg, n, nv):
     """Roofline of the longest single kernel of the step (HIP-event times)."""
     kern = {"traverse": avg["traverse"], "accd_vf": avg["accd_vf"], "accd_ee": avg["accd_ee"],
             "expand": avg["expand"]}
     dom = max(kern, key=kern.get)
     if dom == "traverse":
         # compulsory bytes: sorted nodes and FP64 leaf boxes once, 8 B per emitted pair
-        byts = 80 * (2 * n - 1) + 144 * n + 8 * c["leaf_pairs"]
+        byts = 64 * (2 * n - 1) + 144 * n + 8 * c["leaf_pairs"]
         achieved = byts / (avg["traverse"] * 1e-3) / 1e9
         return {"kernel": "k_traverse<3>", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": profiled_traffic("k_traverse<3>"),
                 "algorithmic_bytes": byts, "ms": round(avg["traverse"], 3),
-                "note": "algorithmic bytes = 80*(2n-1) nodes + 144*n FP64 leaf boxes + 8*leaf_pairs; traffic = "
+                "note": "algorithmic bytes = 64*(2n-1) nodes + 144*n FP64 leaf boxes + 8*leaf_pairs; traffic = "
                         "rocprof FETCH_SIZE+WRITE_SIZE per launch (calibrated on a 1 GiB copy)"}
     if dom == "expand":
-        byts = 16 * c["leaf_pairs"] + 128 * n + 16 * c["candidates"] + 8 * c["discrete"]
+        byts = 16 * c["leaf_pairs"] + 24 * n + 32 * nv + 16 * c["candidates"] + 8 * c["discrete"]
         achieved = byts / (avg["expand"] * 1e-3) / 1e9
         return {"kernel": "k_expand<3> (+ count pass and scans)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": profiled_traffic("k_expand<3>"), "algorithmic_bytes": byts,
-                "note": "algorithmic bytes = pairs read twice (8+8) + 128 B face record per leaf + 16 B per "
+                "note": "algorithmic bytes = pairs read twice (8+8) + 24 B face arrays per leaf + 32 B V1 per vertex + 16 B per "
                         "stencil + 8 B per discrete hit"}
-    flops = FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]
-    ms = avg["accd_vf"] + avg["accd_ee"]
+    # one ACCD kernel: the vertex-face or the edge-edge stencils
+    vf = dom == "accd_vf"
+    cand = c["vf_candidates"] if vf else c["candidates"] - c["vf_candidates"]
+    evals = c["dist_evals"] - c["ee_dist_evals"] if vf else c["ee_dist_evals"]
+    flops = FLOPS_ACCD_SETUP * cand + FLOPS_PER_EVAL * evals
+    ms = avg[dom]
     achieved = flops / (ms * 1e-3) / 1e12
-    return {"kernel": "k_accd<3,VF> + k_accd<3,EE> (FP64 ACCD)", "bound": "valu", "achieved": round(achieved, 3),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
-            "traffic": profiled_traffic("k_accd<3, false>"),
-            "note": f"algorithmic flops = {FLOPS_ACCD_SETUP}*candidates + {FLOPS_PER_EVAL}*distance evaluations, "
-                    "div/sqrt = 1 flop; non-FMA FP64 peak is 39.3 T ops/s"}
+    kernel = "k_accd<3, true>" if vf else "k_accd<3, false>"
+    return {"kernel": f"{kernel} (FP64 ACCD, {'VF' if vf else 'EE'} stencils)", "bound": "valu",
+            "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kernel),
+            "algorithmic_flops": flops, "stencils": cand, "dist_evals": evals, "ms": round(ms, 3),
+            "note": f"algorithmic flops = {FLOPS_ACCD_SETUP}*stencils + {FLOPS_PER_EVAL}*distance evaluations, "
+                    "div/sqrt = 1 flop; FP64 vector peak 78.6 TFLOP/s counts FMA as 2 and parity forbids "
+                    "contraction, so the usable rate is 39.3 T ops/s"}
 
 
 def _cores():
@@ -268,7 +275,7 @@ def main():
     print(f"[bench] counts {c}", file=sys.stderr, flush=True)
     print(f"[bench] phase ms {avg}", file=sys.stderr, flush=True)
 
-    roof = roofline(c, avg, len(F))
+    roof = roofline(c, avg, len(F), len(V0))
     # the other large kernels, for context
     others = {"traverse_ms": round(avg["traverse"], 3), "expand_ms": round(avg["expand"], 3),
               "accd_vf_ms": round(avg["accd_vf"], 3), "accd_ee_ms": round(avg["accd_ee"], 3),

@@ -97,6 +97,8 @@ typedef struct mclg_counts {
   int64_t hits;
   int64_t discrete;
   int64_t dist_evals; /* ACCD pair_distance evaluations performed */
+  int64_t vf_candidates;  /* of candidates: vertex-face stencils (the rest are edge-edge) */
+  int64_t ee_dist_evals;  /* of dist_evals: those of edge-edge stencils */
 } mclg_counts;
 int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
 

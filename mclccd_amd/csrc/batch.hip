@@ -158,7 +158,7 @@ static void batch_enqueue(Ctx* c) {
               c->d_rchild, c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
   const FaceArrays F = face_arrays(c->d_face, (size_t)np, c->d_v1);
   launch_face_records(dim, c->d_order, np, b->d_P, c->d_flags, nullptr, F, s);
-  launch_traverse(dim, c->d_nodes, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters, b->d_khi, b->d_root,
+  launch_traverse(dim, c->d_nodes, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters, b->d_khi, b->d_prim_off,
                   c->tr_waves, s);
   if (!(o.continuous || o.discrete)) return;
   ExpandLaunch E;
@@ -240,6 +240,8 @@ static int batch_frame(Ctx* c) {
   c->counts.hits = (int64_t)h[C_HITS];
   c->counts.discrete = (int64_t)h[C_DISCRETE];
   c->counts.dist_evals = (int64_t)h[C_DIST_EVALS];
+  c->counts.vf_candidates = (int64_t)h[C_CAND_VF];
+  c->counts.ee_dist_evals = (int64_t)h[C_EE_EVALS];
   c->keep_cand = false;
   return MCLG_OK;
 }

@@ -627,7 +627,6 @@ __device__ __forceinline__ uint64_t split_key(const uint32_t* lo, const uint32_t
 template <int D> __device__ __forceinline__ void merge_node(NodeF<D>& o, const NodeF<D>& a, const NodeF<D>& b) {
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    o.sw_lo[k] = fminf(a.sw_lo[k], b.sw_lo[k]); o.sw_hi[k] = fmaxf(a.sw_hi[k], b.sw_hi[k]);
     o.t0_lo[k] = fminf(a.t0_lo[k], b.t0_lo[k]); o.t0_hi[k] = fmaxf(a.t0_hi[k], b.t0_hi[k]);
     o.t1_lo[k] = fminf(a.t1_lo[k], b.t1_lo[k]); o.t1_hi[k] = fmaxf(a.t1_hi[k], b.t1_hi[k]);
   }
@@ -652,10 +651,11 @@ template <int D> __device__ __forceinline__ NodeF<D> leaf_node(const LeafBox<D>&
   for (int k = 0; k < D; ++k) emp |= (b.t0_lo[k] > b.t0_hi[k]) | (b.t1_lo[k] > b.t1_hi[k]);
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    L.sw_lo[k] = f32_down(b.sw_lo[k]); L.sw_hi[k] = f32_up(b.sw_hi[k]);
-    L.t0_lo[k] = emp ? L.sw_lo[k] : f32_down(b.t0_lo[k]); L.t0_hi[k] = emp ? L.sw_hi[k] : f32_up(b.t0_hi[k]);
-    L.t1_lo[k] = emp ? L.sw_lo[k] : f32_down(b.t1_lo[k]); L.t1_hi[k] = emp ? L.sw_hi[k] : f32_up(b.t1_hi[k]);
+    const float sl = f32_down(b.sw_lo[k]), sh = f32_up(b.sw_hi[k]);
+    L.t0_lo[k] = emp ? sl : f32_down(b.t0_lo[k]); L.t0_hi[k] = emp ? sh : f32_up(b.t0_hi[k]);
+    L.t1_lo[k] = emp ? sl : f32_down(b.t1_lo[k]); L.t1_hi[k] = emp ? sh : f32_up(b.t1_hi[k]);
   }
+  L.first = L.last = 0;
   L.child = emp ? 1 : 0;
   L.meta = active ? 0x80000000u : 0u;
   return L;
@@ -691,7 +691,8 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
     const uint32_t prim = order[s];
     const LeafBox<D> lb = boxes[prim];
     sboxes[s] = lb;
-    const NodeF<D> L = leaf_node<D>(lb, act[prim] != 0);
+    NodeF<D> L = leaf_node<D>(lb, act[prim] != 0);
+    L.first = L.last = s;
     nodes[(n - 1) + s] = L;
     s_node[t] = L;
     if (S.root) {
@@ -724,6 +725,8 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
       const NodeF<D>& b = rc < n - 1 ? s_node[LB_T + (rc - B)] : s_node[rc - (n - 1) - B];
       merge_node<D>(cur, a, b);
       cur.child = lc;
+      cur.first = l;
+      cur.last = r;
       s_node[LB_T + q] = cur;
       nodes[p] = cur;
       node = p;
@@ -767,6 +770,8 @@ __global__ __launch_bounds__(256) void k_lbvh_cross(const uint32_t* __restrict__
     NodeF<D> o;
     merge_node<D>(o, a, b);
     o.child = lc;
+    o.first = l;
+    o.last = r;
     nodes[p] = o;
     node = p;
     note_scene_root(S, hi, l, r, node);

@@ -267,7 +267,7 @@ int Ctx::read_counters() {
 
 // BVHTree.cpp:129-159 (+ collide / get_candidates / narrow phase / discrete)
 int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned flags) {
-  counts = mclg_counts{0, 0, 0, 0, 0};
+  counts = mclg_counts{0, 0, 0, 0, 0, 0, 0};
   keep_cand = (flags & MCLG_KEEP_CANDIDATES) != 0;
   keep_pairs = (flags & MCLG_KEEP_LEAF_PAIRS) != 0;
   if (!have_tree || np <= 1) return MCLG_OK;  // BVHTree.cpp:131-132
@@ -338,6 +338,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     n_vf = (long long)h_counters[C_CAND_VF];
     n_ee = (long long)h_counters[C_CAND_EE];
     counts.candidates = n_vf + n_ee;
+    counts.vf_candidates = n_vf;
     counts.discrete = (int64_t)h_counters[C_DISCRETE];
     // FP64 ACCD over the two queues (BVHTree.cpp:428-461)
     if (opt.continuous && n_vf + n_ee > 0) {
@@ -383,6 +384,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       }
       counts.hits = (int64_t)h_counters[C_HITS];
       counts.dist_evals = (int64_t)h_counters[C_DIST_EVALS];
+      counts.ee_dist_evals = (int64_t)h_counters[C_EE_EVALS];
     }
   }
   if (keep_pairs && npairs > 0) {  // sorted positions -> primitive pairs, lower rank first

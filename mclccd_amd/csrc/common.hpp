@@ -20,23 +20,30 @@ template <> struct VRec<2> { static constexpr int stride = 4; static constexpr i
 // mcl::BoundingBox (BVHLeaf.hpp:12-29) flattened.
 template <int D> struct LeafBox { double sw_lo[D], sw_hi[D], t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D]; };
 
-// Traversal node: the three boxes rounded outward to FP32 (conservative for
-// the culling test), and meta = (escape+1) | active<<31 where escape 0 means
-// "end of traversal".  child is the first child of an internal node; of a
-// leaf it is 1 if its FP64 t0 or t1 box is empty (then both FP32 boxes hold
-// the swept box, so that no culling test separates it at t0 and t1), else 0.
+// Traversal node (64 B, one aligned half line): the t0 and t1 boxes rounded
+// outward to FP32 (conservative for the culling test), and meta = (escape+1) |
+// active<<31 where escape 0 means "end of traversal".  The swept box is their
+// union: the reference builds it from the same points with the same padding
+// (BVHTree.cpp:94-117), and min/max and monotone rounding commute, so the union
+// equals the FP32-rounded FP64 swept box exactly.  child is the first child of
+// an internal node; of a leaf it is 1 if its FP64 t0 or t1 box is empty (then
+// both FP32 boxes hold the swept box, so that no culling test separates it at
+// t0 and t1), else 0.
 template <int D> struct alignas(16) NodeF;
 template <> struct alignas(16) NodeF<3> {
-  float sw_lo[3], sw_hi[3], t0_lo[3], t0_hi[3], t1_lo[3], t1_hi[3];
+  float t0_lo[3], t0_hi[3], t1_lo[3], t1_hi[3];
   int child;
   uint32_t meta;
+  int first, last;  // sorted leaves covered: [first, last]
 };
 template <> struct alignas(16) NodeF<2> {
-  float sw_lo[2], sw_hi[2], t0_lo[2], t0_hi[2], t1_lo[2], t1_hi[2];
+  float t0_lo[2], t0_hi[2], t1_lo[2], t1_hi[2];
   int child;
   uint32_t meta;
-  int pad[2];
+  int first, last;
 };
+
+static_assert(sizeof(NodeF<3>) == 64 && sizeof(NodeF<2>) == 48, "node layout");
 
 // Face data of the candidate kernels, which gather two faces per leaf pair at
 // random: compact arrays in sorted (Morton) order, rebuilt by every traverse,
@@ -73,7 +80,7 @@ enum Counter : int {
   C_DISCRETE,
   C_DIST_EVALS,
   C_ERRORS,     // kernels that detected an inconsistent state
-  C_WORK_HEAD,  // work-queue head of a persistent kernel
+  C_EE_EVALS,   // distance evaluations of edge-edge stencils (also counted in C_DIST_EVALS)
   C_COUNT = 8
 };
 

@@ -51,9 +51,11 @@ int lbvh_frontier_cap(int n);
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
                  const uint8_t* act, void* nodes, void* sboxes, int* lchild, int* rchild, int* rng_l, int* rng_r,
                  int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s);
-// Leaf pairs as sorted positions (s, k), s < k.
+// Leaf pairs as sorted positions (s, k), s < k.  Several scenes: scene_of_sorted
+// gives each sorted leaf's scene and scene_off the scenes' sorted-leaf offsets
+// (n+1 entries); pairs never cross scenes.
 void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
-                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_root, int waves,
+                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_off, int waves,
                      hipStream_t s);
 // The leaf arrays of faces (head, rank, prim) for sorted positions (prim = order[s]); rank may be null.
 void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,

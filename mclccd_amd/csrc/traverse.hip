@@ -32,10 +32,24 @@ __device__ __forceinline__ void wave_lds_sync() {
 // The tests below are branch-free: every field is used unconditionally, so a
 // node's or box's loads are all issued at once instead of one dependent load
 // per early exit.
-template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const NodeF<D>& b) {
-  bool ok = ((q.meta | b.meta) & 0x80000000u) != 0;
+// A node's FP32 swept box: the union of its t0 and t1 boxes.
+template <int D> struct Swept {
+  float lo[D], hi[D];
+};
+template <int D> __device__ __forceinline__ Swept<D> swept(const NodeF<D>& b) {
+  Swept<D> s;
 #pragma unroll
-  for (int k = 0; k < D; ++k) ok &= (q.sw_lo[k] <= b.sw_hi[k]) & (b.sw_lo[k] <= q.sw_hi[k]);
+  for (int k = 0; k < D; ++k) {
+    s.lo[k] = fminf(b.t0_lo[k], b.t1_lo[k]);
+    s.hi[k] = fmaxf(b.t0_hi[k], b.t1_hi[k]);
+  }
+  return s;
+}
+template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const Swept<D>& qs, const NodeF<D>& b) {
+  bool ok = ((q.meta | b.meta) & 0x80000000u) != 0;
+  const Swept<D> bs = swept<D>(b);
+#pragma unroll
+  for (int k = 0; k < D; ++k) ok &= (qs.lo[k] <= bs.hi[k]) & (bs.lo[k] <= qs.hi[k]);
 #pragma unroll
   for (int k = 0; k < D; ++k)
     ok &= !(((q.t0_lo[k] > b.t0_hi[k]) & (q.t1_lo[k] > b.t1_hi[k])) |
@@ -67,6 +81,24 @@ __device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const Le
   return ov & (emp | !sep);
 }
 
+// The same test one axis at a time, reading the FP64 boxes in place: the rare
+// fallback of the FP32 test below, kept small in registers so that it does not
+// set the register budget of the traversal loop.
+template <int D>
+__device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bool la, const LeafBox<D>* r, bool ra) {
+  bool ov = la | ra, emp = false, sep = false;
+#pragma nounroll
+  for (int k = 0; k < D; ++k) {
+    const double lsl = l->sw_lo[k], lsh = l->sw_hi[k], rsl = r->sw_lo[k], rsh = r->sw_hi[k];
+    ov &= (lsl <= rsh) & (rsl <= lsh);
+    const double l0l = l->t0_lo[k], l0h = l->t0_hi[k], l1l = l->t1_lo[k], l1h = l->t1_hi[k];
+    const double r0l = r->t0_lo[k], r0h = r->t0_hi[k], r1l = r->t1_lo[k], r1h = r->t1_hi[k];
+    emp |= (l0l > l0h) | (r0l > r0h) | (l1l > l1h) | (r1l > r1h);
+    sep |= ((l0l > r0h) & (l1l > r1h)) | ((l0h < r0l) & (l1h < r1l));
+  }
+  return ov & (emp | !sep);
+}
+
 // The same test decided from the FP32 node boxes when it can be.  A node's
 // lower bound a = f32_down(lo) brackets the FP64 value as lo in [a, a+) and an
 // upper bound b = f32_up(hi) as hi in (b-, b] (a+ and b- one float step up and
@@ -85,11 +117,11 @@ __device__ __forceinline__ Tri tri_or(Tri x, Tri y) { return {bool(x.t | y.t), b
 template <int D> struct QSteps {
   float sw_lo_p[D], sw_hi_m[D], t0_lo_p[D], t0_hi_m[D], t1_lo_p[D], t1_hi_m[D];
 };
-template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>& q) {
+template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>& q, const Swept<D>& qs) {
   QSteps<D> p;
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    p.sw_lo_p[k] = step_up(q.sw_lo[k]); p.sw_hi_m[k] = step_down(q.sw_hi[k]);
+    p.sw_lo_p[k] = step_up(qs.lo[k]); p.sw_hi_m[k] = step_down(qs.hi[k]);
     p.t0_lo_p[k] = step_up(q.t0_lo[k]); p.t0_hi_m[k] = step_down(q.t0_hi[k]);
     p.t1_lo_p[k] = step_up(q.t1_lo[k]); p.t1_hi_m[k] = step_down(q.t1_hi[k]);
   }
@@ -100,15 +132,16 @@ template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>
 // query's t0 or t1 box is empty.  A pair that cull_f32 would reject is
 // certainly false here, so leaves need no separate cull.
 template <int D>
-__device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const QSteps<D>& qp, bool qa, bool qemp,
-                                             const NodeF<D>& b, bool ba) {
+__device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const Swept<D>& qs, const QSteps<D>& qp, bool qa,
+                                             bool qemp, const NodeF<D>& b, bool ba) {
+  const Swept<D> bs = swept<D>(b);
   const bool act = qa | ba;
   Tri ov = {act, !act};
   Tri sep = {false, true};
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < b.sw_hi[k], q.sw_lo[k] > b.sw_hi[k]});  // q.sw_lo <= b.sw_hi
-    ov = tri_and(ov, Tri{b.sw_lo[k] < qp.sw_hi_m[k], b.sw_lo[k] > q.sw_hi[k]});  // b.sw_lo <= q.sw_hi
+    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < bs.hi[k], qs.lo[k] > bs.hi[k]});  // q.sw_lo <= b.sw_hi
+    ov = tri_and(ov, Tri{bs.lo[k] < qp.sw_hi_m[k], bs.lo[k] > qs.hi[k]});  // b.sw_lo <= q.sw_hi
     // q above b at t0 / t1, b above q at t0 / t1 (lo > hi: certain iff a > b, excluded iff a+ < b)
     const Tri a0 = {q.t0_lo[k] > b.t0_hi[k], qp.t0_lo_p[k] < b.t0_hi[k]};
     const Tri a1 = {q.t1_lo[k] > b.t1_hi[k], qp.t1_lo_p[k] < b.t1_hi[k]};
@@ -140,116 +173,123 @@ template <typename T, bool PIN = false> __device__ __forceinline__ T load16(cons
 }
 
 // Pairs found by a wave are staged in LDS and written out in batches: one
-// global atomic per flush instead of one per emitting step.  A staged entry is
-// (lane | r << 8, k): the r-th pair of that lane since the last flush.  The
-// flush writes each lane's pairs contiguously (a query leaf's pairs stay
-// together, so the candidate kernels find its face data in cache).
-constexpr int TR_MAX_WAVES = 8, TR_BUF = 512;
+// global atomic per flush instead of one per emitting step.
+constexpr int TR_MAX_WAVES = 4, TR_BUF = 1024;
 
-// Called by the whole wave (converged); mine = this lane's staged pairs.
-__device__ __forceinline__ void flush_pairs(const int2* buf, int* pre, int cnt, uint32_t mine, int s_wave,
-                                            int2* __restrict__ pairs, unsigned long long cap,
+// Called by the whole wave (converged).
+__device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,
                                             unsigned long long* counter) {
   const int lane = lane_id();
   wave_lds_sync();
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
   base = __shfl(base, 0);
-  pre[lane] = (int)wave_prefix_u32(mine);
-  wave_lds_sync();
-  for (int i = lane; i < cnt; i += kWave) {
-    const int2 e = buf[i];
-    const int L = e.x & 255;
-    const unsigned long long pos = base + (unsigned long long)(pre[L] + (e.x >> 8));
-    if (pos < cap) pairs[pos] = make_int2(s_wave + L, e.y);
-  }
+  for (int i = lane; i < cnt; i += kWave)  // streamed: the walk's nodes stay in cache
+    if (base + i < cap) {
+      const int2 v = buf[i];
+      __builtin_nontemporal_store(((long long)(unsigned)v.y << 32) | (unsigned)v.x,
+                                  reinterpret_cast<long long*>(pairs + base + i));
+    }
   wave_lds_sync();
 }
 
-// At most 128 VGPRs: four waves per SIMD, the occupancy the LDS staging allows.
+// Packet traversal: the 64 lanes of a wave hold 64 consecutive sorted leaves
+// (queries) and walk ONE node sequence together: the stackless pre-order from
+// escape(first leaf of the wave), descending into a subtree when any lane's
+// query may meet it.  Every step reads one node at a wave-uniform address (a
+// scalar load) instead of 64 scattered ones, and each lane tests its own
+// query against it.  Lane j takes part only where the node covers a sorted
+// leaf in (s_j, last_j] (last_j: the last leaf of its scene, n-1 for one
+// scene), so each unordered leaf pair is still found once, by its lower leaf.
+// The walk ends when its nodes start past every lane's last leaf.
 template <int D>
-__global__ __launch_bounds__(TR_MAX_WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__(TR_MAX_WAVES * kWave)
 void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict__ sboxes, int n,
                 int2* __restrict__ pairs, unsigned long long cap, unsigned long long* counters,
-                const uint32_t* __restrict__ scene_of_sorted, const int* __restrict__ scene_root) {
+                const uint32_t* __restrict__ scene_of_sorted, const int* __restrict__ scene_off) {
   __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
-  __shared__ int s_pre[TR_MAX_WAVES][kWave];
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int s = xcd_block() * blockDim.x + threadIdx.x;
-  const int s_wave = s - lane;
   const bool valid = s < n - 1;
+  if (__ballot(valid) == 0) return;  // wave-uniform: the wave starts at or after the last leaf
   NodeF<D> q;
+  Swept<D> qs;
   QSteps<D> qp;
   bool qa = false, qemp = false;
-  int node = -1, end = -1;
+  int last = -1, start = -1;
   if (valid) {
     q = load16(nodes + (n - 1) + s);
     qa = (q.meta >> 31) != 0;
     qemp = q.child != 0;
-    qp = query_steps<D>(q);
-    node = (int)(q.meta & 0x7fffffffu) - 1;
-    // several scenes: the walk ends where the pre-order leaves this leaf's scene subtree
-    if (scene_root) end = (int)(nodes[scene_root[scene_of_sorted[s]]].meta & 0x7fffffffu) - 1;
-    if (node == end) node = -1;
+    qs = swept<D>(q);
+    qp = query_steps<D>(q, qs);
+    last = scene_off ? scene_off[scene_of_sorted[s] + 1] - 1 : n - 1;
+    start = (int)(q.meta & 0x7fffffffu) - 1;
   }
+  int wave_last = last;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const int v = __shfl_xor(wave_last, o);
+    wave_last = v > wave_last ? v : wave_last;
+  }
+  wave_last = __builtin_amdgcn_readfirstlane(wave_last);
+  int node = __builtin_amdgcn_readfirstlane(__shfl(start, 0));  // lane 0 is valid whenever any lane is
   // a pre-order walk visits each of the 2n-1 nodes at most once: more steps
   // than that means a malformed tree, which is reported instead of looping
   int budget = 2 * n;
-  int cnt = 0;        // staged pairs of the wave (uniform)
-  uint32_t mine = 0;  // staged pairs of this lane
-  // the wave steps while any lane walks; appends and flushes run converged
-  while (__ballot(node >= 0)) {
+  int cnt = 0;  // staged pairs (wave-uniform)
+  while (node >= 0) {
+    if (--budget < 0) {
+      if (lane == 0) atomicAdd(counters + C_ERRORS, 1ull);
+      break;
+    }
+    const NodeF<D> nd = nodes[node];
+    if (nd.first > wave_last) break;
+    const int esc = (int)(nd.meta & 0x7fffffffu) - 1;
     bool emit = false;
     int k = 0;
-    if (node >= 0) {
-      if (--budget < 0) {
-        atomicAdd(counters + C_ERRORS, 1ull);
-        node = -1;
-      } else {
-        const NodeF<D> nd = load16<NodeF<D>, true>(nodes + node);
-        int next = (int)(nd.meta & 0x7fffffffu) - 1;
-        if (node >= n - 1) {
-          k = node - (n - 1);
-          const bool ba = (nd.meta >> 31) != 0;
-          const Tri r = leaf_test_f32<D>(q, qp, qa, qemp, nd, ba);
-          if (r.t | r.f) {
-            emit = r.t;
-          } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
-            emit = leaf_test<D>(load16(sboxes + s), qa, load16(sboxes + k), ba);
-          }
-        } else if (cull_f32<D>(q, nd)) {
-          next = nd.child;
+    if (node >= n - 1) {
+      k = node - (n - 1);
+      if (valid && k > s && k <= last) {
+        const bool ba = (nd.meta >> 31) != 0;
+        const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, nd, ba);
+        if (r.t | r.f) {
+          emit = r.t;
+        } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
+          emit = leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
         }
-        node = next == end ? -1 : next;
       }
+      node = esc;
+    } else {
+      const bool want = valid && nd.last > s && nd.first <= last && cull_f32<D>(q, qs, nd);
+      node = __ballot(want) ? nd.child : esc;
     }
+    node = __builtin_amdgcn_readfirstlane(node);
     const uint64_t m = __ballot(emit);
     if (m) {
-      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(lane | (int)(mine << 8), k);
-      mine += emit ? 1u : 0u;
+      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(s, k);
       cnt += __popcll(m);
       if (cnt > TR_BUF - kWave) {
-        flush_pairs(s_buf[w], s_pre[w], cnt, mine, s_wave, pairs, cap, counters + C_LEAF_PAIRS);
+        flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
         cnt = 0;
-        mine = 0;
       }
     }
   }
-  if (cnt > 0) flush_pairs(s_buf[w], s_pre[w], cnt, mine, s_wave, pairs, cap, counters + C_LEAF_PAIRS);
+  if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
 }
 
 void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
-                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_root, int waves,
+                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_off, int waves,
                      hipStream_t s) {
   if (n < 2) return;
   waves = waves < 1 ? 4 : (waves > TR_MAX_WAVES ? TR_MAX_WAVES : waves);
   const int g = xcd_grid(n - 1, waves * kWave);
   if (dim == 3)
     k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)sboxes, n, pairs, cap,
-                                                 counters, scene_of_sorted, scene_root);
+                                                 counters, scene_of_sorted, scene_off);
   else
     k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)sboxes, n, pairs, cap,
-                                                 counters, scene_of_sorted, scene_root);
+                                                 counters, scene_of_sorted, scene_off);
 }
 
 // ---------------------------------------------------------------------------
@@ -608,7 +648,10 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
     if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
   }
   for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
-  if (lane == 0 && evals) atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
+  if (lane == 0 && evals) {
+    atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
+    if (!VF) atomicAdd(A.counters + C_EE_EVALS, (unsigned long long)evals);
+  }
 }
 
 __global__ void k_hits_total(const unsigned long long* off, long long last, unsigned long long* counters) {

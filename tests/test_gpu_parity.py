@@ -63,6 +63,11 @@ def check_parity(t, o, strict):
     gs, gv, gt = t.candidates()
     os_, ov = o.candidates()
     assert len(gs) == len(os_)
+    c = t.counts()
+    assert c["vf_candidates"] == int(np.count_nonzero(ov))
+    assert 0 <= c["ee_dist_evals"] <= c["dist_evals"]
+    if int(np.count_nonzero(ov)) == len(ov):
+        assert c["ee_dist_evals"] == 0
     if strict:
         assert oriented_stencils(gs, gv) == oriented_stencils(os_, ov)
     else:
@@ -70,7 +75,6 @@ def check_parity(t, o, strict):
     # hits
     hs, hv, ht = t.hits()
     rs, rv, rt = o.hits()
-    c = t.counts()
     assert c["hits"] == len(rs)
     gm, rm = hit_map(hs, hv, ht, canonical=not strict), hit_map(rs, rv, rt, canonical=not strict)
     if strict:
@@ -167,7 +171,8 @@ def test_tiny(gpu, nprim):
     t.run(V0, V1, F, keep_candidates=True, keep_leaf_pairs=True)
     c = t.counts()
     if nprim <= 1:
-        assert c == {"leaf_pairs": 0, "candidates": 0, "hits": 0, "discrete": 0, "dist_evals": 0}
+        assert c == {"leaf_pairs": 0, "candidates": 0, "hits": 0, "discrete": 0, "dist_evals": 0,
+                     "vf_candidates": 0, "ee_dist_evals": 0}
     else:
         o = run_scene(V0, V1, F, 1e-2)
         check_parity(t, o, strict=False)
