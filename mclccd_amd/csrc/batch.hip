@@ -1,8 +1,9 @@
 // Batched scenes: many independent BVHTree problems solved in one pass.
 //
 // All scenes' vertices and primitives are concatenated; the LBVH is keyed by
-// (scene, Morton code), so every scene is one subtree and a query leaf's walk
-// ends at its scene subtree's escape link (no cross-scene pairs).  One frame is
+// (scene, Morton code), so every scene's leaves are one contiguous sorted range
+// and a query leaf only pairs with leaves of its own range (no cross-scene
+// pairs).  One frame is
 // a fixed sequence of kernels with no host synchronisation inside — queue
 // lengths are read from device counters — so it is captured once as a hipGraph
 // and replayed.  Per-scene gaps (box/vf/ee eta) are honoured; the boolean
@@ -44,7 +45,6 @@ struct BatchState {
   int* d_prim_off = nullptr; size_t cap_prim_off = 0;
   double* d_eta = nullptr; size_t cap_eta = 0;  // [box | vf | ee] x nscenes
   unsigned long long* d_sbounds = nullptr; size_t cap_sbounds = 0;
-  int* d_root = nullptr; size_t cap_root = 0;
   uint32_t* d_khi = nullptr; size_t cap_khi = 0;
   uint32_t* d_mort = nullptr; size_t cap_mort = 0;
   double* d_V0 = nullptr; size_t cap_V0 = 0;
@@ -76,7 +76,6 @@ static SceneArgs scene_args(BatchState* b) {
   S.vf_eta = b->d_eta + b->nscenes;
   S.ee_eta = b->d_eta + 2 * b->nscenes;
   S.bounds = b->d_sbounds;
-  S.root = b->d_root;
   return S;
 }
 
@@ -99,6 +98,8 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&b->d_khi, &b->cap_khi, sizeof(uint32_t) * (size_t)np, "scene keys"));
   CHECK(c->grow((void**)&b->d_mort, &b->cap_mort, sizeof(uint32_t) * (size_t)np, "morton"));
   CHECK(c->grow((void**)&c->d_nodes, &c->cap_nodes, node * (size_t)(2 * np - 1), "nodes"));
+  CHECK(c->grow(&c->d_cp, &c->cap_cp, child_pair_bytes(dim) * (size_t)std::max(np - 1, 1), "child pairs"));
+  CHECK(c->grow((void**)&c->d_root, &c->cap_root, sizeof(int), "root"));
   CHECK(c->grow(&c->d_sboxes, &c->cap_sboxes, box * (size_t)np, "sorted leaf boxes"));
   CHECK(c->grow(&c->d_face, &c->cap_face, face_bytes((size_t)np), "face arrays"));
   CHECK(c->grow((void**)&c->d_v1, &c->cap_v1, sizeof(double) * v1_stride(dim) * (size_t)nv, "V1"));
@@ -155,11 +156,12 @@ static void batch_enqueue(Ctx* c) {
   launch_radix_sort(b->d_khi, c->d_order, c->d_ktmp, c->d_vtmp, c->d_hist, np, b->scene_bits, s);
   launch_gather_u32(c->d_order, np, b->d_mort, c->d_keys, s);
   launch_lbvh(dim, c->d_keys, b->d_khi, c->d_order, np, c->d_boxes, c->d_act, c->d_nodes, c->d_sboxes, c->d_lchild,
-              c->d_rchild, c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s);
+              c->d_rchild, c->d_rngl, c->d_rngr, c->d_visits, c->d_frontier, c->d_nfrontier, S, s, c->d_cp,
+              c->d_root);
   const FaceArrays F = face_arrays(c->d_face, (size_t)np, c->d_v1);
   launch_face_records(dim, c->d_order, np, b->d_P, c->d_flags, nullptr, F, s);
-  launch_traverse(dim, c->d_nodes, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters, b->d_khi, b->d_prim_off,
-                  c->tr_waves, s);
+  launch_traverse(dim, c->d_nodes, c->d_cp, c->d_root, c->d_sboxes, np, c->d_pairs, c->pair_cap, c->d_counters,
+                  b->d_khi, b->d_prim_off, c->tr_waves, s);
   if (!(o.continuous || o.discrete)) return;
   ExpandLaunch E;
   E.pairs = c->d_pairs; E.npairs_dev = c->d_counters + C_LEAF_PAIRS; E.pair_cap = c->pair_cap; E.npairs_hint = -1;
@@ -278,7 +280,7 @@ void mclg_batch_destroy(mclg_batch* h) {
   if (b) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(b);
-    void* bufs[] = {b->d_P, b->d_prim_scene, b->d_vert_scene, b->d_prim_off, b->d_eta, b->d_sbounds, b->d_root,
+    void* bufs[] = {b->d_P, b->d_prim_scene, b->d_vert_scene, b->d_prim_off, b->d_eta, b->d_sbounds,
                     b->d_khi, b->d_mort, b->d_V0, b->d_V1, b->d_scene_counts};
     for (void* p : bufs)
       if (p) (void)hipFree(p);
@@ -347,7 +349,6 @@ int mclg_batch_set_scenes(mclg_batch* h, int nscenes, const int* voff, const int
   CHECK(c->grow((void**)&b->d_prim_off, &b->cap_prim_off, sizeof(int) * (nscenes + 1), "batch"));
   CHECK(c->grow((void**)&b->d_eta, &b->cap_eta, sizeof(double) * eta.size(), "batch"));
   CHECK(c->grow((void**)&b->d_sbounds, &b->cap_sbounds, sizeof(unsigned long long) * 2 * dim * nscenes, "batch"));
-  CHECK(c->grow((void**)&b->d_root, &b->cap_root, sizeof(int) * nscenes, "batch"));
   CHECK(c->grow((void**)&b->d_V0, &b->cap_V0, sizeof(double) * dim * std::max<size_t>(nv, 1), "batch V0"));
   CHECK(c->grow((void**)&b->d_V1, &b->cap_V1, sizeof(double) * dim * std::max<size_t>(nv, 1), "batch V1"));
   HIPCHECK(c, hipMemcpy(b->d_P, Pg.data(), sizeof(int) * Pg.size(), hipMemcpyHostToDevice));

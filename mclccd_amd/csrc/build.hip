@@ -1,6 +1,6 @@
 // BVH build kernels for gfx950: vertex packing, representative-triangle flags,
 // leaf boxes, Morton codes, LSD radix sort, bottom-up LBVH (Apetrei 2014) with
-// FP32 outward-rounded node boxes, and escape links for stackless traversal.
+// FP32 outward-rounded node boxes, and the child-pair records the traversal reads.
 #include "kernels.hpp"
 
 #pragma clang fp contract(off)
@@ -661,12 +661,6 @@ template <int D> __device__ __forceinline__ NodeF<D> leaf_node(const LeafBox<D>&
   return L;
 }
 
-__device__ __forceinline__ void note_scene_root(const SceneArgs& S, const uint32_t* hi, int l, int r, int node) {
-  if (!S.root) return;
-  const int sc = (int)hi[l];
-  if (l == S.prim_off[sc] && r == S.prim_off[sc + 1] - 1) S.root[sc] = node;
-}
-
 // Phase A: each block of LB_T threads owns sorted leaves [B, E) and builds every
 // node whose split lies in [B, E-2] and whose children it completes, with LDS
 // visit counters (workgroup scope only).  A completed node whose parent split
@@ -695,10 +689,6 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
     L.first = L.last = s;
     nodes[(n - 1) + s] = L;
     s_node[t] = L;
-    if (S.root) {
-      const int sc = (int)hi[s];
-      if (S.prim_off[sc + 1] - S.prim_off[sc] == 1) S.root[sc] = (n - 1) + s;
-    }
   }
   __syncthreads();
   if (s < n && n > 1) {
@@ -730,7 +720,6 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
       s_node[LB_T + q] = cur;
       nodes[p] = cur;
       node = p;
-      note_scene_root(S, hi, l, r, node);
       if (l == 0 && r == n - 1) break;  // root of the whole tree
     }
   }
@@ -774,45 +763,54 @@ __global__ __launch_bounds__(256) void k_lbvh_cross(const uint32_t* __restrict__
     o.last = r;
     nodes[p] = o;
     node = p;
-    note_scene_root(S, hi, l, r, node);
     if (l == 0 && r == n - 1) return;
   }
 }
 
-// escape(node with range [l,r]) = right child of internal node r, or end.
+// Child-pair records of the internal nodes, and the index of the root (the
+// internal node covering every leaf).
 template <int D>
-__global__ void k_lbvh_ropes(int n, NodeF<D>* nodes, const int* __restrict__ rchild, const int* __restrict__ rng_r) {
-  const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= 2 * n - 1) return;
-  const int r = id < n - 1 ? rng_r[id] : id - (n - 1);
-  const int esc = r < n - 1 ? rchild[r] : -1;
-  nodes[id].meta = (nodes[id].meta & 0x80000000u) | (uint32_t)(esc + 1);
+__global__ void k_child_pairs(const NodeF<D>* __restrict__ nodes, int n, const int* __restrict__ lchild,
+                              const int* __restrict__ rchild, ChildPair<D>* __restrict__ cp, int* root) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n - 1) return;
+  const int lc = lchild[p], rc = rchild[p];
+  NodeF<D> a = nodes[lc], b = nodes[rc];
+  a.meta = (a.meta & 0x80000000u) | (uint32_t)(lc + 1);
+  b.meta = (b.meta & 0x80000000u) | (uint32_t)(rc + 1);
+  cp[p].c[0] = a;
+  cp[p].c[1] = b;
+  if (a.first == 0 && b.last == n - 1) *root = p;
 }
 
 int lbvh_frontier_cap(int n) { return 2 * ((n + LB_T - 1) / LB_T) + 2; }
 
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
                  const uint8_t* act, void* nodes, void* sboxes, int* lchild, int* rchild, int* rng_l, int* rng_r,
-                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s) {
+                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s,
+                 void* cp, int* root) {
   if (n < 1) return;
   fill<unsigned long long>(nfrontier, 1, 0ull, s);
   const int g = (n + LB_T - 1) / LB_T;
   const int gf = (lbvh_frontier_cap(n) + 255) / 256;
-  const int g2 = (2 * n - 1 + 255) / 256;
   if (dim == 3) {
     k_lbvh_local<3><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes,
                                        (LeafBox<3>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
                                        nfrontier, S);
     if (n > 1) k_lbvh_cross<3><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<3>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);
-    k_lbvh_ropes<3><<<g2, 256, 0, s>>>(n, (NodeF<3>*)nodes, rchild, rng_r);
+    if (cp && n > 1)
+      k_child_pairs<3><<<(n - 1 + 255) / 256, 256, 0, s>>>((const NodeF<3>*)nodes, n, lchild, rchild,
+                                                          (ChildPair<3>*)cp, root);
   } else {
     k_lbvh_local<2><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes,
                                        (LeafBox<2>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
                                        nfrontier, S);
     if (n > 1) k_lbvh_cross<2><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<2>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);
-    k_lbvh_ropes<2><<<g2, 256, 0, s>>>(n, (NodeF<2>*)nodes, rchild, rng_r);
+    if (cp && n > 1)
+      k_child_pairs<2><<<(n - 1 + 255) / 256, 256, 0, s>>>((const NodeF<2>*)nodes, n, lchild, rchild,
+                                                          (ChildPair<2>*)cp, root);
   }
 }
 

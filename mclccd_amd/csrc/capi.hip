@@ -245,8 +245,10 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_frontier, &cap_frontier, sizeof(int4) * (size_t)lbvh_frontier_cap(np), "frontier"));
   CHECK(grow((void**)&d_nfrontier, &cap_nfrontier, sizeof(unsigned long long), "frontier"));
   CHECK(grow(&d_sboxes, &cap_sboxes, ldbox_bytes(dim) * (size_t)np, "sorted leaf boxes"));
+  CHECK(grow(&d_cp, &cap_cp, child_pair_bytes(dim) * ni, "child pairs"));
+  CHECK(grow((void**)&d_root, &cap_root, sizeof(int), "root"));
   launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_sboxes, d_lchild, d_rchild, d_rngl, d_rngr,
-              d_visits, d_frontier, d_nfrontier, SceneArgs(), stream);
+              d_visits, d_frontier, d_nfrontier, SceneArgs(), stream, d_cp, d_root);
   mark(6);
   HIPCHECK(this, hipGetLastError());
   HIPCHECK(this, hipStreamSynchronize(stream));
@@ -289,7 +291,8 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   CHECK(grow((void**)&d_pairs, &cap_pairs_bytes, sizeof(int2) * pair_cap, "leaf pairs"));
   for (;;) {
     mark(8);
-    launch_traverse(dim, d_nodes, d_sboxes, np, d_pairs, pair_cap, d_counters, nullptr, nullptr, tr_waves, stream);
+    launch_traverse(dim, d_nodes, d_cp, d_root, d_sboxes, np, d_pairs, pair_cap, d_counters, nullptr, nullptr, tr_waves,
+                    stream);
     mark(9);
     HIPCHECK(this, hipGetLastError());
     CHECK(read_counters());
@@ -417,7 +420,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
 Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
-                  d_vtmp, d_hist, d_sboxes, d_face, d_v1, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
+                  d_vtmp, d_hist, d_sboxes, d_face, d_v1, d_cp, d_root, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
                   d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)

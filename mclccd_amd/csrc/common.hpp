@@ -20,9 +20,10 @@ template <> struct VRec<2> { static constexpr int stride = 4; static constexpr i
 // mcl::BoundingBox (BVHLeaf.hpp:12-29) flattened.
 template <int D> struct LeafBox { double sw_lo[D], sw_hi[D], t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D]; };
 
-// Traversal node (64 B, one aligned half line): the t0 and t1 boxes rounded
-// outward to FP32 (conservative for the culling test), and meta = (escape+1) |
-// active<<31 where escape 0 means "end of traversal".  The swept box is their
+// Tree node (64 B, one aligned half line): the t0 and t1 boxes rounded outward
+// to FP32 (conservative for the culling test), meta = active<<31 (the child-pair
+// copies below put the node's index in the low bits), and the range of sorted
+// leaves the node covers.  The swept box is their
 // union: the reference builds it from the same points with the same padding
 // (BVHTree.cpp:94-117), and min/max and monotone rounding commute, so the union
 // equals the FP32-rounded FP64 swept box exactly.  child is the first child of
@@ -44,6 +45,14 @@ template <> struct alignas(16) NodeF<2> {
 };
 
 static_assert(sizeof(NodeF<3>) == 64 && sizeof(NodeF<2>) == 48, "node layout");
+
+// Both children of internal node p side by side (one 128 B line in 3D), as the
+// traversal reads them: copies of their NodeF with meta = (index+1) |
+// active<<31, the child's own node index in place of its escape link.
+template <int D> struct ChildPair {
+  NodeF<D> c[2];
+};
+inline size_t child_pair_bytes(int dim) { return dim == 3 ? sizeof(ChildPair<3>) : sizeof(ChildPair<2>); }
 
 // Face data of the candidate kernels, which gather two faces per leaf pair at
 // random: compact arrays in sorted (Morton) order, rebuilt by every traverse,

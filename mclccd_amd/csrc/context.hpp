@@ -59,6 +59,8 @@ struct Ctx {
   uint32_t* d_vtmp = nullptr; size_t cap_vtmp = 0;
   uint32_t* d_hist = nullptr; size_t cap_hist = 0;
   void* d_nodes = nullptr; size_t cap_nodes = 0;
+  void* d_cp = nullptr; size_t cap_cp = 0;          // child pairs of the internal nodes
+  int* d_root = nullptr; size_t cap_root = 0;       // root node index
   int* d_lchild = nullptr; size_t cap_lchild = 0;
   int* d_rchild = nullptr; size_t cap_rchild = 0;
   int* d_rngl = nullptr; size_t cap_rngl = 0;

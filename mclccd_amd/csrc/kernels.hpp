@@ -17,7 +17,6 @@ struct SceneArgs {
   const double* vf_eta = nullptr;      // per scene
   const double* ee_eta = nullptr;      // per scene
   unsigned long long* bounds = nullptr;  // per scene centre bounds, ordered encoding [lo(dim), hi(dim)]
-  int* root = nullptr;                 // per scene subtree root node
 };
 
 // rec: per-vertex records (both time steps); v1 (may be null): V1 per vertex, v1_stride(dim) doubles each.
@@ -47,16 +46,19 @@ void launch_scan_u64(unsigned long long* data, long long n, unsigned long long* 
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
 int lbvh_frontier_cap(int n);
-// Also writes sboxes[s] = boxes[order[s]] (FP64 leaf boxes in sorted order).
+// Also writes sboxes[s] = boxes[order[s]] (FP64 leaf boxes in sorted order)
+// and, when cp is given and n > 1, the child-pair records (ChildPair<dim> per
+// internal node) and *root.
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,
                  const uint8_t* act, void* nodes, void* sboxes, int* lchild, int* rchild, int* rng_l, int* rng_r,
-                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s);
+                 int* visits, int4* frontier, unsigned long long* nfrontier, const SceneArgs& S, hipStream_t s,
+                 void* cp, int* root);
 // Leaf pairs as sorted positions (s, k), s < k.  Several scenes: scene_of_sorted
 // gives each sorted leaf's scene and scene_off the scenes' sorted-leaf offsets
 // (n+1 entries); pairs never cross scenes.
-void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
-                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_off, int waves,
-                     hipStream_t s);
+void launch_traverse(int dim, const void* nodes, const void* cp, const int* root, const void* sboxes, int n,
+                     int2* pairs, unsigned long long cap, unsigned long long* counters,
+                     const uint32_t* scene_of_sorted, const int* scene_off, int waves, hipStream_t s);
 // The leaf arrays of faces (head, rank, prim) for sorted positions (prim = order[s]); rank may be null.
 void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
                          const FaceArrays& faces, hipStream_t s);

@@ -1,12 +1,13 @@
 // Self-traversal and narrow phase for gfx950.
 //
-// k_traverse: one lane per sorted leaf s; the lane walks the LBVH stacklessly
-//   from escape(leaf s), i.e. over exactly the subtrees covering leaves s+1..n-1,
-//   so every unordered leaf pair is visited once.  Internal nodes are culled with
-//   the FP32 outward-rounded boxes; at a leaf the reference's exact test
-//   (BVHTree.cpp:401-426) is decided on the FP32 bounds when they settle it and
-//   on the FP64 boxes (kept in sorted order) otherwise.  Pairs (sorted
-//   positions) are staged per wave in LDS and appended in batches.
+// k_traverse: one lane per sorted leaf s (the query), 64 consecutive leaves per
+//   wave walking the LBVH together (child-pair records, wave-uniform stack);
+//   lane s pairs only with leaves after it, so every unordered leaf pair is
+//   found once.  Internal nodes are culled with the FP32 outward-rounded boxes;
+//   at a leaf the reference's exact test (BVHTree.cpp:401-426) is decided on the
+//   FP32 bounds when they settle it and on the FP64 boxes (kept in sorted
+//   order) otherwise.  Pairs (sorted positions) are staged per wave in LDS and
+//   appended in batches.
 // k_face_records: per sorted leaf, the compact head (P, reptri flags), rank
 //   and primitive index.
 // k_expand: one lane per leaf pair: orientation, the discrete test and the
@@ -194,20 +195,65 @@ __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __re
 }
 
 // Packet traversal: the 64 lanes of a wave hold 64 consecutive sorted leaves
-// (queries) and walk ONE node sequence together: the stackless pre-order from
-// escape(first leaf of the wave), descending into a subtree when any lane's
-// query may meet it.  Every step reads one node at a wave-uniform address (a
-// scalar load) instead of 64 scattered ones, and each lane tests its own
-// query against it.  Lane j takes part only where the node covers a sorted
-// leaf in (s_j, last_j] (last_j: the last leaf of its scene, n-1 for one
-// scene), so each unordered leaf pair is still found once, by its lower leaf.
-// The walk ends when its nodes start past every lane's last leaf.
+// (queries) and walk the tree together, depth first from the root with a
+// wave-uniform stack.  Each step reads the child-pair record of one internal
+// node (both children, one 128 B line, at a wave-uniform address: scalar
+// loads) and tests each lane's query against both children; a child is
+// entered if any lane wants it.  Lane j takes part only where a child covers
+// a sorted leaf in (s_j, last_j] (last_j: the last leaf of its scene, n-1 for
+// one scene), so each unordered leaf pair is found once, by its lower leaf.
+constexpr int TR_STACK = 96;  // deeper than any LBVH over 32-bit keys and int indices
+
+// One child of the current node: a leaf is tested by every lane that wants it
+// (emit), an internal node is wanted if any lane's query may meet it.
+struct Visit {
+  int idx;
+  bool emit, want;
+};
+template <int D>
+__device__ __forceinline__ Visit visit_child(const NodeF<D>& c, int n, int s, bool valid, int last, int wave_last,
+                                             const NodeF<D>& q, const Swept<D>& qs, const QSteps<D>& qp, bool qa,
+                                             bool qemp, const LeafBox<D>* __restrict__ sboxes) {
+  Visit v = {(int)(c.meta & 0x7fffffffu) - 1, false, false};
+  if (c.first > wave_last) return v;  // past every lane's scene
+  if (v.idx >= n - 1) {
+    const int k = v.idx - (n - 1);
+    if (valid && k > s && k <= last) {
+      const bool ba = (c.meta >> 31) != 0;
+      const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, c, ba);
+      if (r.t | r.f) {
+        v.emit = r.t;
+      } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
+        v.emit = leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
+      }
+    }
+  } else {
+    v.want = valid && c.last > s && c.first <= last && cull_f32<D>(q, qs, c);
+  }
+  return v;
+}
+
+// Stages the lanes' pairs (s, k) of one leaf k; flushes a full buffer.
+__device__ __forceinline__ void stage_pairs(bool emit, int s, int k, int2* buf, int& cnt, int2* __restrict__ pairs,
+                                            unsigned long long cap, unsigned long long* counter) {
+  const uint64_t m = __ballot(emit);
+  if (m == 0) return;
+  if (emit) buf[cnt + popc_below(m)] = make_int2(s, k);
+  cnt += __popcll(m);
+  if (cnt > TR_BUF - kWave) {
+    flush_pairs(buf, cnt, pairs, cap, counter);
+    cnt = 0;
+  }
+}
+
 template <int D>
 __global__ __launch_bounds__(TR_MAX_WAVES * kWave)
-void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict__ sboxes, int n,
-                int2* __restrict__ pairs, unsigned long long cap, unsigned long long* counters,
-                const uint32_t* __restrict__ scene_of_sorted, const int* __restrict__ scene_off) {
+void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restrict__ cp, const int* __restrict__ root,
+                const LeafBox<D>* __restrict__ sboxes, int n, int2* __restrict__ pairs, unsigned long long cap,
+                unsigned long long* counters, const uint32_t* __restrict__ scene_of_sorted,
+                const int* __restrict__ scene_off) {
   __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
+  __shared__ int s_stack[TR_MAX_WAVES][TR_STACK];
   const int w = threadIdx.x / kWave, lane = lane_id();
   const int s = xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = s < n - 1;
@@ -216,7 +262,7 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict
   Swept<D> qs;
   QSteps<D> qp;
   bool qa = false, qemp = false;
-  int last = -1, start = -1;
+  int last = -1;
   if (valid) {
     q = load16(nodes + (n - 1) + s);
     qa = (q.meta >> 31) != 0;
@@ -224,7 +270,6 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict
     qs = swept<D>(q);
     qp = query_steps<D>(q, qs);
     last = scene_off ? scene_off[scene_of_sorted[s] + 1] - 1 : n - 1;
-    start = (int)(q.meta & 0x7fffffffu) - 1;
   }
   int wave_last = last;
 #pragma unroll
@@ -233,63 +278,60 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const LeafBox<D>* __restrict
     wave_last = v > wave_last ? v : wave_last;
   }
   wave_last = __builtin_amdgcn_readfirstlane(wave_last);
-  int node = __builtin_amdgcn_readfirstlane(__shfl(start, 0));  // lane 0 is valid whenever any lane is
-  // a pre-order walk visits each of the 2n-1 nodes at most once: more steps
-  // than that means a malformed tree, which is reported instead of looping
-  int budget = 2 * n;
+  int p = *root;  // wave-uniform
+  int sp = 0;     // stack depth (uniform)
+  // each internal node is entered at most once: more steps means a malformed tree
+  int budget = n;
   int cnt = 0;  // staged pairs (wave-uniform)
-  while (node >= 0) {
+  for (;;) {
     if (--budget < 0) {
       if (lane == 0) atomicAdd(counters + C_ERRORS, 1ull);
       break;
     }
-    const NodeF<D> nd = nodes[node];
-    if (nd.first > wave_last) break;
-    const int esc = (int)(nd.meta & 0x7fffffffu) - 1;
-    bool emit = false;
-    int k = 0;
-    if (node >= n - 1) {
-      k = node - (n - 1);
-      if (valid && k > s && k <= last) {
-        const bool ba = (nd.meta >> 31) != 0;
-        const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, nd, ba);
-        if (r.t | r.f) {
-          emit = r.t;
-        } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
-          emit = leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
+    const ChildPair<D> pr = cp[p];
+    const Visit va = visit_child<D>(pr.c[0], n, s, valid, last, wave_last, q, qs, qp, qa, qemp, sboxes);
+    const Visit vb = visit_child<D>(pr.c[1], n, s, valid, last, wave_last, q, qs, qp, qa, qemp, sboxes);
+    // pairs of leaf children (leaf k = index - (n-1))
+    stage_pairs(va.emit, s, va.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
+    stage_pairs(vb.emit, s, vb.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
+    const int ia = va.idx, ib = vb.idx;
+    const bool ga = __ballot(va.want) != 0, gb = __ballot(vb.want) != 0;
+    if (ga) {
+      if (gb) {  // enter the first child, the second waits on the stack
+        if (sp == TR_STACK) {
+          if (lane == 0) atomicAdd(counters + C_ERRORS, 1ull);
+          break;
         }
+        if (lane == 0) s_stack[w][sp] = ib;
+        ++sp;
       }
-      node = esc;
+      p = ia;
+    } else if (gb) {
+      p = ib;
     } else {
-      const bool want = valid && nd.last > s && nd.first <= last && cull_f32<D>(q, qs, nd);
-      node = __ballot(want) ? nd.child : esc;
-    }
-    node = __builtin_amdgcn_readfirstlane(node);
-    const uint64_t m = __ballot(emit);
-    if (m) {
-      if (emit) s_buf[w][cnt + popc_below(m)] = make_int2(s, k);
-      cnt += __popcll(m);
-      if (cnt > TR_BUF - kWave) {
-        flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
-        cnt = 0;
-      }
+      if (sp == 0) break;
+      --sp;
+      wave_lds_sync();
+      p = __builtin_amdgcn_readfirstlane(s_stack[w][sp]);
     }
   }
   if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
 }
 
-void launch_traverse(int dim, const void* nodes, const void* sboxes, int n, int2* pairs, unsigned long long cap,
-                     unsigned long long* counters, const uint32_t* scene_of_sorted, const int* scene_off, int waves,
-                     hipStream_t s) {
+void launch_traverse(int dim, const void* nodes, const void* cp, const int* root, const void* sboxes, int n,
+                     int2* pairs, unsigned long long cap, unsigned long long* counters,
+                     const uint32_t* scene_of_sorted, const int* scene_off, int waves, hipStream_t s) {
   if (n < 2) return;
   waves = waves < 1 ? 4 : (waves > TR_MAX_WAVES ? TR_MAX_WAVES : waves);
   const int g = xcd_grid(n - 1, waves * kWave);
   if (dim == 3)
-    k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const LeafBox<3>*)sboxes, n, pairs, cap,
-                                                 counters, scene_of_sorted, scene_off);
+    k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const ChildPair<3>*)cp, root,
+                                                 (const LeafBox<3>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
+                                                 scene_off);
   else
-    k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const LeafBox<2>*)sboxes, n, pairs, cap,
-                                                 counters, scene_of_sorted, scene_off);
+    k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const ChildPair<2>*)cp, root,
+                                                 (const LeafBox<2>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
+                                                 scene_off);
 }
 
 // ---------------------------------------------------------------------------
