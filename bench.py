@@ -227,8 +227,8 @@ def main():
                     help="first run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is a design, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path runs on one GPU; "
+              "sharding by ranges of sorted query leaves is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
