@@ -11,7 +11,9 @@
 //     toi (narrow_phase if set, else the GPU's ACCD) and append_pair when
 //     toi >= 0; then append_discrete for each discrete hit until one returns true.
 //   - The tree is the GPU's LBVH: the `tree` member (an Eigen KdBVH) does not
-//     exist.  `leaves` is filled on demand by sync_leaves().
+//     exist.  `leaves` is filled on demand by sync_leaves().  traverse(BVHTraverse*)
+//     walks a host copy of the LBVH with the KdBVH walk's rules, so a visitor
+//     sees other internal volumes, in another order, than on the reference.
 //   - vf_one_sided (the CTCD narrow phase) has no GPU path: update/traverse
 //     throw std::runtime_error for it.  GPU failures throw std::runtime_error.
 //   - Only T = double is provided (the FP64 path).
@@ -23,6 +25,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -33,6 +36,7 @@
 #if defined(__has_include)
 #if __has_include(<Eigen/Core>)
 #include <Eigen/Core>
+#include <Eigen/Geometry>
 #define MCLG_HAVE_EIGEN 1
 #endif
 #endif
@@ -53,13 +57,85 @@ struct Vector4i {
 };
 #endif
 
-// BVHLeaf.hpp:12-52 without Eigen: the three boxes flattened, active, idx and
-// the representative-triangle flags.
+#ifdef MCLG_HAVE_EIGEN
+template <typename T, int DIM> using AlignedBox = Eigen::AlignedBox<T, DIM>;
+#else
+// The part of Eigen::AlignedBox the leaves and visitors use: an empty box
+// (min = +max, max = lowest) grown by extend with Eigen's min/max forms.
+template <typename T, int DIM> class AlignedBox {
+ public:
+  AlignedBox() { setEmpty(); }
+  void setEmpty() {
+    for (int k = 0; k < DIM; ++k) { lo_[k] = std::numeric_limits<T>::max(); hi_[k] = std::numeric_limits<T>::lowest(); }
+  }
+  const T* min() const { return lo_; }
+  const T* max() const { return hi_; }
+  T* min() { return lo_; }
+  T* max() { return hi_; }
+  bool isEmpty() const {
+    for (int k = 0; k < DIM; ++k)
+      if (lo_[k] > hi_[k]) return true;
+    return false;
+  }
+  bool contains(const T* p) const {
+    for (int k = 0; k < DIM; ++k)
+      if (!(lo_[k] <= p[k] && p[k] <= hi_[k])) return false;
+    return true;
+  }
+  AlignedBox& extend(const T* p) {
+    for (int k = 0; k < DIM; ++k) { lo_[k] = p[k] < lo_[k] ? p[k] : lo_[k]; hi_[k] = hi_[k] < p[k] ? p[k] : hi_[k]; }
+    return *this;
+  }
+  AlignedBox& extend(const AlignedBox& b) {
+    for (int k = 0; k < DIM; ++k) {
+      lo_[k] = b.lo_[k] < lo_[k] ? b.lo_[k] : lo_[k];
+      hi_[k] = hi_[k] < b.hi_[k] ? b.hi_[k] : hi_[k];
+    }
+    return *this;
+  }
+
+ private:
+  T lo_[DIM], hi_[DIM];
+};
+#endif
+
+// BVHLeaf.hpp:12-29: the swept box with its t0 and t1 boxes and the active flag.
+template <typename T, int DIM> class BoundingBox : public AlignedBox<T, DIM> {
+ public:
+  bool active = true;
+  AlignedBox<T, DIM> t0, t1;
+  // BVHLeaf.cpp:9-23
+  BoundingBox merged(const BoundingBox& b) const {
+    BoundingBox m;
+    m.active = this->active || b.active;
+    m.extend(this->min());
+    m.extend(this->max());
+    m.extend(b.min());
+    m.extend(b.max());
+    m.t0.extend(this->t0);
+    m.t0.extend(b.t0);
+    m.t1.extend(this->t1);
+    m.t1.extend(b.t1);
+    return m;
+  }
+};
+
+// BVHLeaf.hpp:31-52: primitive index, representative-triangle flags, boxes.
 template <typename T, int DIM> struct BVHLeaf {
+  typedef BoundingBox<T, DIM> BoxType;
   int idx = -1;
   int v[3] = {1, 1, 1}, e[3] = {1, 1, 1};
-  T sw_min[DIM], sw_max[DIM], t0_min[DIM], t0_max[DIM], t1_min[DIM], t1_max[DIM];
-  bool active = true;
+  BoxType box;
+};
+
+// BVHTree.hpp:17-27: a user traversal (point-in-element queries and the like).
+template <typename T, int DIM> class BVHTraverse {
+ public:
+  typedef BVHLeaf<T, DIM> ObjectType;
+  typedef typename BVHLeaf<T, DIM>::BoxType VolumeType;
+  virtual ~BVHTraverse() {}
+  virtual bool intersectVolume(const VolumeType&) = 0;  // true: go down this branch
+  virtual bool intersectObject(const ObjectType&) = 0;  // true: stop the traversal
 };
 
 template <typename T, int DIM> class BVHTree;
@@ -110,6 +186,7 @@ template <int DIM> class BVHTree<double, DIM> {
     for (long long i = 0; i < (long long)np * pdim; ++i) nv = std::max(nv, P[i] + 1);
     check(mclg_update(ctx_.get(), V0, V1, nv, P, np, pdim, active));
     np_ = np;
+    mirror_ = false;
     pull_options();
   }
 
@@ -158,6 +235,7 @@ template <int DIM> class BVHTree<double, DIM> {
     push_options();
     check(mclg_update(ctx_.get(), a.data(), b.data(), (int)a.rows(), p.data(), (int)p.rows(), (int)p.cols(), act));
     np_ = (int)p.rows();
+    mirror_ = false;
     pull_options();
   }
   template <typename DerivedV, typename DerivedP>
@@ -184,12 +262,38 @@ template <int DIM> class BVHTree<double, DIM> {
       L.idx = (int)i;
       const double* x = b.data() + i * 6 * DIM;
       for (int k = 0; k < DIM; ++k) {
-        L.sw_min[k] = x[k]; L.sw_max[k] = x[DIM + k];
-        L.t0_min[k] = x[2 * DIM + k]; L.t0_max[k] = x[3 * DIM + k];
-        L.t1_min[k] = x[4 * DIM + k]; L.t1_max[k] = x[5 * DIM + k];
+        L.box.min()[k] = x[k]; L.box.max()[k] = x[DIM + k];
+        L.box.t0.min()[k] = x[2 * DIM + k]; L.box.t0.max()[k] = x[3 * DIM + k];
+        L.box.t1.min()[k] = x[4 * DIM + k]; L.box.t1.max()[k] = x[5 * DIM + k];
       }
-      L.active = a[(size_t)i] != 0;
+      L.box.active = a[(size_t)i] != 0;
       for (int j = 0; j < 3; ++j) { L.v[j] = (f[(size_t)i] >> j) & 1; L.e[j] = (f[(size_t)i] >> (3 + j)) & 1; }
+    }
+  }
+
+  // BVHTree.hpp:96, BVHTree.cpp:162-166 (Eigen::BVIntersect over the KdBVH):
+  // from the root, the child volumes of each visited node are offered to
+  // intersectVolume (true queues the child), then its child objects to
+  // intersectObject (true ends the walk); queued nodes are taken last in,
+  // first out.  Walks a host copy of the last update's tree (LBVH), whose
+  // internal volumes are merges of their children's boxes as in the reference.
+  void traverse(BVHTraverse<T, DIM>* trav) const {
+    const_cast<BVHTree*>(this)->sync_mirror();
+    if (root_ < 0 || !trav) return;
+    const int ni = np_ - 1;  // internal nodes; leaf s is node ni + s
+    if (np_ == 1) {
+      trav->intersectObject(leaves[(size_t)order_[0]]);
+      return;
+    }
+    std::vector<int> todo(1, root_);
+    while (!todo.empty()) {
+      const int p = todo.back();
+      todo.pop_back();
+      const int ch[2] = {lchild_[(size_t)p], rchild_[(size_t)p]};
+      for (int c : ch)
+        if (c < ni && trav->intersectVolume(volumes_[(size_t)c])) todo.push_back(c);
+      for (int c : ch)
+        if (c >= ni && trav->intersectObject(leaves[(size_t)order_[(size_t)(c - ni)]])) return;
     }
   }
 
@@ -204,6 +308,42 @@ template <int DIM> class BVHTree<double, DIM> {
  private:
   std::shared_ptr<mclg_ctx> ctx_;
   int np_ = 0;
+  // host copy of the tree for visitor traversals (valid until the next update)
+  bool mirror_ = false;
+  int root_ = -1;
+  std::vector<int> lchild_, rchild_, order_;
+  std::vector<typename LeafType::BoxType> volumes_;  // internal nodes
+
+  void sync_mirror() {
+    if (mirror_) return;
+    sync_leaves();
+    const int n = np_, ni = std::max(n - 1, 0);
+    lchild_.assign((size_t)ni, -1);
+    rchild_.assign((size_t)ni, -1);
+    order_.assign((size_t)n, 0);
+    check(mclg_copy_tree(ctx_.get(), lchild_.data(), rchild_.data(), order_.data(), &root_, n));
+    // internal volumes, children first (an explicit post-order)
+    volumes_.assign((size_t)ni, typename LeafType::BoxType());
+    auto box = [&](int c) -> const typename LeafType::BoxType& {
+      return c < ni ? volumes_[(size_t)c] : leaves[(size_t)order_[(size_t)(c - ni)]].box;
+    };
+    if (n > 1) {
+      std::vector<std::pair<int, bool>> st(1, std::make_pair(root_, false));
+      while (!st.empty()) {
+        const std::pair<int, bool> t = st.back();
+        st.pop_back();
+        const int p = t.first, l = lchild_[(size_t)p], r = rchild_[(size_t)p];
+        if (t.second) {
+          volumes_[(size_t)p] = box(l).merged(box(r));
+        } else {
+          st.emplace_back(p, true);
+          if (l < ni) st.emplace_back(l, false);
+          if (r < ni) st.emplace_back(r, false);
+        }
+      }
+    }
+    mirror_ = true;
+  }
 
   static void check(int rc, mclg_ctx* c) {
     if (rc != MCLG_OK)

@@ -130,6 +130,17 @@ int mclg_copy_leaf_pairs(mclg_ctx* ctx, int* pairs2, int64_t cap);
  * (bits 0..2 = v[0..2], bits 3..5 = e[0..2]).  Any output may be NULL. */
 int mclg_copy_leaves(mclg_ctx* ctx, double* boxes, unsigned char* active, unsigned char* flags, int64_t cap);
 
+/* The tree of the last update, for visitor traversals on the host (replaces
+ * the KdBVH the reference walks in traverse(BVHTraverse*), BVHTree.cpp:162-166).
+ * Node ids: internal node k (0 <= k < np-1) is k, the leaf at sorted position
+ * s is np-1+s.  lchild / rchild: np-1 entries each (the internal nodes'
+ * children); order[s]: the primitive at sorted position s (np entries); *root:
+ * the root node id (0 for a single leaf, -1 for an empty tree).  A leaf
+ * primitive's boxes are those of mclg_copy_leaves; an internal node's volume
+ * is the merge of its children's (BVHLeaf.cpp:9-23).  Any output may be NULL;
+ * cap must be at least np. */
+int mclg_copy_tree(mclg_ctx* ctx, int* lchild, int* rchild, int* order, int* root, int64_t cap);
+
 /* Device pointers to the last traverse's results (valid until the next traverse).
  * Candidates live in two queues, VF then EE; their per-candidate toi arrays are
  * set only when the traverse kept candidates, leaf_pairs2 only when it kept

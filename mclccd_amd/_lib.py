@@ -75,6 +75,7 @@ def _load():
         "mclg_copy_leaf_pairs": (I, [P, P, I64]),
         "mclg_get_device_results": (I, [P, P]),
         "mclg_copy_leaves": (I, [P, P, P, P, I64]),
+        "mclg_copy_tree": (I, [P, P, P, P, P, I64]),
         "mclg_toi_grad": (I, [P, P, P, P, I]),
         "mclg_batch_create": (I, [I, P]),
         "mclg_batch_destroy": (None, [P]),

@@ -563,6 +563,21 @@ int mclg_copy_leaves(mclg_ctx* h, double* boxes, unsigned char* active, unsigned
   return MCLG_OK;
 }
 
+int mclg_copy_tree(mclg_ctx* h, int* lchild, int* rchild, int* order, int* root, int64_t cap) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  const int64_t n = c->have_tree ? c->np : 0;
+  if (cap < n) return c->fail(MCLG_ERR_CAPACITY, "copy_tree: cap < np");
+  if (root) {
+    if (n <= 1) *root = n == 1 ? 0 : -1;
+    else CHECK(copy_out(c, root, c->d_root, sizeof(int)));
+  }
+  if (n > 1 && lchild) CHECK(copy_out(c, lchild, c->d_lchild, sizeof(int) * (size_t)(n - 1)));
+  if (n > 1 && rchild) CHECK(copy_out(c, rchild, c->d_rchild, sizeof(int) * (size_t)(n - 1)));
+  if (n > 0 && order) CHECK(copy_out(c, order, c->d_order, sizeof(int) * (size_t)n));
+  return MCLG_OK;
+}
+
 int mclg_get_device_results(const mclg_ctx* h, mclg_device_results* out) {
   const Ctx* c = reinterpret_cast<const Ctx*>(h);
   if (!c || !out) return MCLG_ERR_ARG;

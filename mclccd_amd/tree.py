@@ -17,6 +17,10 @@ workers, so any serial order is one it can produce):
   2. discrete: append_discrete(p0, p1) per discrete hit; a True return stops
      the replay (the reference's stop flag, BVHTree.cpp:236-239,280-281).
 
+``traverse(visitor)`` (BVHTree.hpp:96) walks a host copy of the tree with a
+``BVHTraverse``-like object: ``intersect_volume(box) -> bool`` (True: go down)
+and ``intersect_object(leaf) -> bool`` (True: stop), see ``visit``.
+
 Inputs may be numpy arrays (copied to the GPU) or CUDA torch tensors (used in
 place: float64 vertices, int32 indices, contiguous).
 """
@@ -105,6 +109,7 @@ class BVHTree:
         self.nv = 0
         self.np = 0
         self.serial = 0  # counts traversals (the adjoint refers to the last one)
+        self._mirror = None  # host copy of the tree for visitor walks (per update)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -157,9 +162,13 @@ class BVHTree:
         self._check(rc)
         self._pull_options()
         self.nv, self.np = nv, npr
+        self._mirror = None
 
-    def traverse(self, V0, V1, P):
-        """BVHTree::traverse (BVHTree.hpp:86-93) with serial callback replay."""
+    def traverse(self, V0, V1=None, P=None):
+        """BVHTree::traverse (BVHTree.hpp:86-93) with serial callback replay;
+        traverse(visitor) is BVHTree::traverse(BVHTraverse*), see ``visit``."""
+        if V1 is None and P is None and hasattr(V0, "intersect_volume"):
+            return self.visit(V0)
         if len(V0) != len(V1):
             return
         need_cand = self.narrow_phase is not None or self.filter_pair is not None
@@ -261,6 +270,35 @@ class BVHTree:
         self._check(lib.mclg_copy_leaves(self._h, _ptr(b), _ptr(a), _ptr(f), n))
         return b, a, f
 
+    def tree(self):
+        """The last update's tree: (lchild, rchild, order, root).  Internal node k
+        (< np-1) has children lchild[k], rchild[k]; node np-1+s is the leaf at
+        sorted position s, primitive order[s]."""
+        n = self.np
+        lc = np.empty(max(n - 1, 0), np.int32)
+        rc = np.empty(max(n - 1, 0), np.int32)
+        order = np.empty(n, np.int32)
+        root = C.c_int(-1)
+        self._check(lib.mclg_copy_tree(self._h, _ptr(lc), _ptr(rc), _ptr(order), C.byref(root), n))
+        return lc, rc, order, root.value
+
+    def visit(self, visitor):
+        """BVHTree::traverse(BVHTraverse*) (BVHTree.cpp:162-166, the KdBVH walk of
+        Eigen::BVIntersect): from the root, each visited node's child volumes go
+        to ``visitor.intersect_volume(box)`` (True queues the child), then its
+        child objects to ``visitor.intersect_object(leaf)`` (True ends the walk);
+        queued nodes are taken last in, first out.  ``box`` is a ``Box``: swept
+        ``lo``/``hi``, ``t0``/``t1`` as (lo, hi) and ``active``; an internal
+        volume merges its children's boxes (BVHLeaf.cpp:9-23).  ``leaf`` is a
+        ``Leaf`` (idx, box).  The tree is the GPU's LBVH, so the volumes and the
+        visiting order differ from the reference's KdBVH; the objects reached by
+        a query that never stops early do not."""
+        if self.np == 0:
+            return
+        if self._mirror is None:
+            self._mirror = _HostTree(self)
+        self._mirror.walk(visitor)
+
     def set_rank(self, rank):
         """Orient leaf pairs by rank (the reference's KdBVH in-order position)."""
         if rank is None:
@@ -276,3 +314,82 @@ class BVHTree:
 
     def set_stream(self, stream_ptr: int | None):
         self._check(lib.mclg_set_stream(self._h, C.c_void_p(stream_ptr) if stream_ptr else None))
+
+
+class Box:
+    """BoundingBox (BVHLeaf.hpp:12-29): swept box lo/hi, t0 and t1 as (lo, hi), active."""
+
+    __slots__ = ("lo", "hi", "t0", "t1", "active")
+
+    def __init__(self, lo, hi, t0, t1, active):
+        self.lo, self.hi, self.t0, self.t1, self.active = lo, hi, t0, t1, active
+
+    def contains(self, p) -> bool:
+        p = np.asarray(p, np.float64)
+        return bool(np.all(self.lo <= p) and np.all(p <= self.hi))
+
+
+class Leaf:
+    """BVHLeaf (BVHLeaf.hpp:31-52): primitive index and its box."""
+
+    __slots__ = ("idx", "box")
+
+    def __init__(self, idx, box):
+        self.idx, self.box = idx, box
+
+
+class _HostTree:
+    """Host copy of a BVHTree's last update for visitor walks: leaf boxes and the
+    internal volumes, merged children first (BVHLeaf.cpp:9-23: union of the swept,
+    t0 and t1 boxes, active if either is)."""
+
+    def __init__(self, t):
+        b, a, _ = t.leaves()
+        lc, rc, order, root = t.tree()
+        d, n = t.dim, t.np
+        self.n, self.root, self.lc, self.rc, self.order = n, root, lc, rc, order
+        # boxes of nodes in node-id order: internal 0..n-2, then leaves by sorted position
+        lo = np.empty((2 * n - 1, 3, d))  # [swept, t0, t1] lower bounds
+        hi = np.empty((2 * n - 1, 3, d))
+        act = np.empty(2 * n - 1, bool)
+        lb = b[order].reshape(n, 6, d)
+        lo[n - 1:], hi[n - 1:] = lb[:, 0::2], lb[:, 1::2]
+        act[n - 1:] = a[order] != 0
+        if n > 1:
+            post = []  # post-order of the internal nodes
+            st = [(root, False)]
+            while st:
+                p, done = st.pop()
+                if done:
+                    post.append(p)
+                    continue
+                st.append((p, True))
+                for c in (lc[p], rc[p]):
+                    if c < n - 1:
+                        st.append((c, False))
+            for p in post:  # the reference's extend: lo = (b < a) ? b : a, hi = (a < b) ? b : a
+                l, r = lc[p], rc[p]
+                lo[p] = np.where(lo[r] < lo[l], lo[r], lo[l])
+                hi[p] = np.where(hi[l] < hi[r], hi[r], hi[l])
+                act[p] = act[l] or act[r]
+        self.lo, self.hi, self.act = lo, hi, act
+
+    def _box(self, i):
+        lo, hi = self.lo[i], self.hi[i]
+        return Box(lo[0], hi[0], (lo[1], hi[1]), (lo[2], hi[2]), bool(self.act[i]))
+
+    def walk(self, visitor):
+        n = self.n
+        if n == 1:
+            visitor.intersect_object(Leaf(int(self.order[0]), self._box(0)))
+            return
+        todo = [self.root]
+        while todo:
+            p = todo.pop()
+            ch = (int(self.lc[p]), int(self.rc[p]))
+            for c in ch:
+                if c < n - 1 and visitor.intersect_volume(self._box(c)):
+                    todo.append(c)
+            for c in ch:
+                if c >= n - 1 and visitor.intersect_object(Leaf(int(self.order[c - (n - 1)]), self._box(c))):
+                    return

@@ -6,7 +6,40 @@
 #include <MCL/BVHTree.hpp>
 
 #include <cstdio>
+#include <set>
 #include <vector>
+
+// Visitors for traverse(BVHTraverse*) (BVHTree.hpp:17-27, examples/traverser.cpp):
+// every volume and object, points in leaf boxes, and a stop at the first object.
+struct VisitAll : mcl::BVHTraverse<double, 3> {
+  std::vector<int> seen;
+  long volumes = 0;
+  bool intersectVolume(const VolumeType&) override {
+    ++volumes;
+    return true;
+  }
+  bool intersectObject(const ObjectType& o) override {
+    seen.push_back(o.idx);
+    return false;
+  }
+};
+struct PointInBox : mcl::BVHTraverse<double, 3> {
+  double pt[3];
+  std::set<int> in;
+  bool intersectVolume(const VolumeType& v) override { return v.contains(pt); }
+  bool intersectObject(const ObjectType& o) override {
+    if (o.box.contains(pt)) in.insert(o.idx);
+    return false;
+  }
+};
+struct StopFirst : mcl::BVHTraverse<double, 3> {
+  long objects = 0;
+  bool intersectVolume(const VolumeType&) override { return true; }
+  bool intersectObject(const ObjectType&) override {
+    ++objects;
+    return true;
+  }
+};
 
 int main(int argc, char** argv) {
   if (argc < 3) return 2;
@@ -91,6 +124,35 @@ int main(int argc, char** argv) {
   long owned_v = 0;
   for (const auto& L : tree.leaves) owned_v += L.v[0] + L.v[1] + L.v[2];
   std::fprintf(out, "leaves %zu %ld\n", tree.leaves.size(), owned_v);
+
+  // 6. visitor traversal over the host copy of the tree
+  VisitAll all;
+  tree.traverse(&all);
+  std::vector<int> count((size_t)nf, 0);
+  long bad_idx = 0;
+  for (int i : all.seen) {
+    if (i < 0 || i >= nf) ++bad_idx;
+    else ++count[(size_t)i];
+  }
+  long once = 0;
+  for (int c : count) once += c == 1;
+  std::fprintf(out, "visit_all %zu %ld %ld %ld\n", all.seen.size(), once, bad_idx, all.volumes);
+  long points = 0, mismatches = 0, found = 0;
+  for (int v = 0; v < nv; v += 7) {
+    PointInBox q;
+    for (int k = 0; k < 3; ++k) q.pt[k] = V0[(size_t)v * 3 + k];
+    tree.traverse(&q);
+    std::set<int> brute;
+    for (const auto& L : tree.leaves)
+      if (L.box.contains(q.pt)) brute.insert(L.idx);
+    ++points;
+    found += (long)q.in.size();
+    mismatches += q.in != brute;
+  }
+  std::fprintf(out, "visit_points %ld %ld %ld\n", points, mismatches, found);
+  StopFirst sf;
+  tree.traverse(&sf);
+  std::fprintf(out, "visit_stop %ld\n", sf.objects);
   std::fclose(out);
   return 0;
 }

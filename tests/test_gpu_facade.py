@@ -63,3 +63,10 @@ def test_facade_callbacks(gpu, tmp_path):
     # 5. leaves mirror: every vertex owned once
     nleaves, owned = rec["leaves"]
     assert nleaves == len(F) and owned == len(np.unique(F))
+    # visitor traversal: every object once, point queries equal brute force over
+    # the leaf boxes, a true from intersectObject stops the walk
+    nseen, once, bad_idx, nvol = rec["visit_all"]
+    assert nseen == once == len(F) and bad_idx == 0 and nvol == len(F) - 2
+    npts, mism, found = rec["visit_points"]
+    assert npts > 0 and mism == 0 and found >= npts  # each point lies in its own triangles' boxes
+    assert rec["visit_stop"][0] == 1
