@@ -160,18 +160,22 @@ template <bool VF> struct Stencil<3, VF> {
       pb.y = pb.y + w * p[i].y;
       pb.z = pb.z + w * p[i].z;
     }
+    // l_p = max |p| over the first group + max |p| over the second.  The
+    // maxima are taken over the squared norms and rooted once per group: a
+    // correctly rounded square root is monotone, so sqrt(max) = max(sqrt)
+    // bit for bit (NaNs are skipped by smax either way).
     double lf = 0.0, ls = 0.0, sum = 0.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       p[i] = sub(p[i], pb);
       sum += (p[i].x + p[i].y) + p[i].z;
-      const double n = __builtin_sqrt(dot(p[i], p[i]));
+      const double n2 = dot(p[i], p[i]);
       const bool first = VF ? (i == 0) : (i <= 1);
-      if (first) lf = smax(lf, n);
-      else ls = smax(ls, n);
+      if (first) lf = smax(lf, n2);
+      else ls = smax(ls, n2);
     }
     fin = __builtin_isfinite(sum);
-    return lf + ls;
+    return __builtin_sqrt(lf) + __builtin_sqrt(ls);
   }
 };
 
@@ -195,17 +199,17 @@ template <> struct Stencil<2, true> {
       pb.x = pb.x + w * p[i].x;
       pb.y = pb.y + w * p[i].y;
     }
-    double lf = 0.0, ls = 0.0, sum = 0.0;
+    double lf = 0.0, ls = 0.0, sum = 0.0;  // squared norms, rooted once per group (as in 3D)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       p[i] = sub(p[i], pb);
       sum += p[i].x + p[i].y;
-      const double n = __builtin_sqrt(dot(p[i], p[i]));
-      if (i == 0) lf = smax(lf, n);
-      else ls = smax(ls, n);
+      const double n2 = dot(p[i], p[i]);
+      if (i == 0) lf = smax(lf, n2);
+      else ls = smax(ls, n2);
     }
     fin = __builtin_isfinite(sum);
-    return lf + ls;
+    return __builtin_sqrt(lf) + __builtin_sqrt(ls);
   }
 };
 
