@@ -21,6 +21,9 @@
 
 namespace mclg {
 
+// int4 as a clang vector: the operand type of the non-temporal builtins
+typedef int nt_int4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -585,8 +588,11 @@ __device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a
     stage[p++] = st;
   }
   wave_lds_sync();
-  for (uint32_t j = lane_id(); j < total; j += kWave)
-    if (base + j < cap) dst[base + j] = stage[j];
+  for (uint32_t j = lane_id(); j < total; j += kWave)  // streamed: read back by the ACCD kernels from HBM
+    if (base + j < cap) {
+      const int4 v = stage[j];
+      __builtin_nontemporal_store(nt_int4{v.x, v.y, v.z, v.w}, reinterpret_cast<nt_int4*>(dst + base + j));
+    }
   wave_lds_sync();
 }
 
@@ -683,7 +689,7 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
       double toi = -1.0;
       int ev = 0;
       hit = accd<D, VF>(v0, v1, eta, toi, ev);
-      A.cand_toi[i] = hit ? toi : -1.0;
+      __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
       evals += ev;
     }
     const uint64_t m = __ballot(hit != 0);
@@ -702,7 +708,8 @@ __global__ void k_hits_total(const unsigned long long* off, long long last, unsi
 
 // Each chunk's hits in lane order at off[ch].  A wave takes HC_U consecutive
 // 64-stencil chunks per step and issues all their loads before any store, so
-// each wave keeps several independent loads in flight.
+// each wave keeps several independent loads in flight.  (Plain accesses:
+// non-temporal loads or stores made this kernel slower.)
 constexpr int HC_U = 4;
 
 template <bool VF>
