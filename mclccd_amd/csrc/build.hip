@@ -43,9 +43,14 @@ void launch_pack(int dim, const double* V0, const double* V1, int nv, double* re
 }
 
 // ---------------------------------------------------------------------------
-// Index validation: min and max of an int array (one block per 64K items).
+// Index validation: min and max of an int array.  Each block reduces in LDS
+// and issues one atomic pair (same-address atomics serialise, so there are at
+// most MM_BLOCKS of them).
 // ---------------------------------------------------------------------------
-__global__ void k_minmax_i32(const int* __restrict__ a, long long n, int* out) {
+constexpr int MM_BLOCKS = 256, MM_T = 256;
+
+__global__ __launch_bounds__(MM_T) void k_minmax_i32(const int* __restrict__ a, long long n, int* out) {
+  __shared__ int s_lo[MM_T / kWave], s_hi[MM_T / kWave];
   int lo = 0x7fffffff, hi = -0x7fffffff - 1;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const int v = a[i];
@@ -57,14 +62,24 @@ __global__ void k_minmax_i32(const int* __restrict__ a, long long n, int* out) {
     lo = l2 < lo ? l2 : lo;
     hi = h2 > hi ? h2 : hi;
   }
-  if (lane_id() == 0) { atomicMin(out, lo); atomicMax(out + 1, hi); }
+  const int w = threadIdx.x / kWave;
+  if (lane_id() == 0) { s_lo[w] = lo; s_hi[w] = hi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < MM_T / kWave; ++k) {
+      lo = s_lo[k] < lo ? s_lo[k] : lo;
+      hi = s_hi[k] > hi ? s_hi[k] : hi;
+    }
+    atomicMin(out, lo);
+    atomicMax(out + 1, hi);
+  }
 }
 
 void launch_minmax_i32(const int* a, long long n, int* out2, hipStream_t s) {
   if (n <= 0) return;
-  long long g = (n + 255) / 256;
-  if (g > 2048) g = 2048;
-  k_minmax_i32<<<(int)g, 256, 0, s>>>(a, n, out2);
+  long long g = (n + MM_T - 1) / MM_T;
+  if (g > MM_BLOCKS) g = MM_BLOCKS;
+  k_minmax_i32<<<(int)g, MM_T, 0, s>>>(a, n, out2);
 }
 
 // ---------------------------------------------------------------------------

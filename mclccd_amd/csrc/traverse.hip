@@ -695,10 +695,17 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
     const uint64_t m = __ballot(hit != 0);
     if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
   }
+  // one atomic per block (same-address atomics serialise)
+  __shared__ long long s_ev[AC_WAVES];
   for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
-  if (lane == 0 && evals) {
-    atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
-    if (!VF) atomicAdd(A.counters + C_EE_EVALS, (unsigned long long)evals);
+  if (lane == 0) s_ev[w] = evals;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < AC_WAVES; ++k) evals += s_ev[k];
+    if (evals) {
+      atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
+      if (!VF) atomicAdd(A.counters + C_EE_EVALS, (unsigned long long)evals);
+    }
   }
 }
 
