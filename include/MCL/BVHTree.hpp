@@ -14,8 +14,9 @@
 //     exist.  `leaves` is filled on demand by sync_leaves().  traverse(BVHTraverse*)
 //     walks a host copy of the LBVH with the KdBVH walk's rules, so a visitor
 //     sees other internal volumes, in another order, than on the reference.
-//   - vf_one_sided (the CTCD narrow phase) has no GPU path: update/traverse
-//     throw std::runtime_error for it.  GPU failures throw std::runtime_error.
+//   - vf_one_sided runs the CTCD narrow phase with its wrong-side test on the
+//     GPU for vertex-face stencils (EE stays ACCD), as BVHTree.cpp:446-447 does.
+//     GPU failures throw std::runtime_error.
 //   - Only T = double is provided (the FP64 path).
 // When Eigen is available the Eigen overloads of update/traverse are provided
 // and the callbacks take Eigen::Vector4i, as in the reference.

@@ -32,7 +32,7 @@ enum {
   MCLG_ERR_HIP = -2,         /* a HIP runtime call failed */
   MCLG_ERR_NOMEM = -3,       /* device allocation failed */
   MCLG_ERR_STATE = -4,       /* call order (e.g. traverse before update) */
-  MCLG_ERR_UNSUPPORTED = -5, /* option with no GPU path (vf_one_sided CTCD) */
+  MCLG_ERR_UNSUPPORTED = -5, /* request with no GPU path (e.g. the TOI adjoint of CTCD hits) */
   MCLG_ERR_CAPACITY = -6     /* caller buffer smaller than the result */
 };
 
@@ -48,7 +48,8 @@ typedef struct mclg_options {
   int threaded;      /* accepted for compatibility; the GPU path is always parallel */
   int discrete;      /* discrete tri-tri (3D) / edge-edge (2D) at V1 (default 1) */
   int continuous;    /* ACCD from V0 to V1 (default 1) */
-  int vf_one_sided;  /* CTCD VF path: no GPU implementation -> MCLG_ERR_UNSUPPORTED */
+  int vf_one_sided;  /* VF stencils through CTCD with the wrong-side test (NarrowPhase.cpp:283-419)
+                      * instead of ACCD; EE stays ACCD (BVHTree.cpp:443-455) */
   int ee_robust;     /* unused on this path (as in the reference) */
 } mclg_options;
 
@@ -188,7 +189,9 @@ int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
  * the forward pass computed (envelope theorem), except in the reference's
  * point_on_triangle quirk branch (Projection.hpp:131), which is differentiated
  * through.  Host pointers, or device pointers when device_ptrs.  Allocates
- * scratch on the first call (about 16 bytes per hit for the visiting order). */
+ * scratch on the first call (about 16 bytes per hit for the visiting order).
+ * MCLG_ERR_UNSUPPORTED when the last traverse ran with vf_one_sided (CTCD
+ * times have no adjoint here). */
 int mclg_toi_grad(mclg_ctx* ctx, const double* w, double* dV0, double* dV1, int device_ptrs);
 
 /* ------------------------------------------------------------------------

@@ -101,14 +101,17 @@ class Batch:
         self._check(lib.mclg_batch_autotune(self._h, cache_path.encode() if cache_path else None, int(reps)))
 
 
-def from_scenes(scenes, dim=3):
+def from_scenes(scenes, dim=3, **options):
     """Builds a Batch from a list of (V0, V1, P, theta) scenes; returns
-    (batch, V0 concatenated, V1 concatenated)."""
+    (batch, V0 concatenated, V1 concatenated).  Keyword options (e.g.
+    vf_one_sided=True) apply to every scene."""
     b = Batch(dim)
     opts = []
     for _, _, _, th in scenes:
         o = Options()
         o.box_eta = o.vf_ccd_eta = o.ee_ccd_eta = th
+        for k, v in options.items():
+            setattr(o, k, v)
         opts.append(o)
     b.set_scenes([s[2] for s in scenes], [len(s[0]) for s in scenes], opts)
     V0 = np.ascontiguousarray(np.concatenate([s[0] for s in scenes]))

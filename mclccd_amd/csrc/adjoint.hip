@@ -514,6 +514,8 @@ using namespace mclg;
 extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* dV1, int device_ptrs) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !w || !dV0 || !dV1) return MCLG_ERR_ARG;
+  if (c->hits_one_sided)
+    return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad: the TOI adjoint covers ACCD only, not the vf_one_sided CTCD path");
   const long long nh = c->counts.hits;
   const size_t gbytes = sizeof(double) * c->dim * (size_t)c->nv;
   double *gw = nullptr, *g0 = nullptr, *g1 = nullptr;

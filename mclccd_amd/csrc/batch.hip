@@ -182,7 +182,9 @@ static void batch_enqueue(Ctx* c) {
   L.counters = c->d_counters; L.num_cus = c->num_cus; L.blocks_per_cu = c->blocks_per_cu; 
   L.q = c->d_vfq; L.n_dev = c->d_counters + C_CAND_VF; L.cap = c->vf_cap; L.n_hint = -1; L.eta = 0;
   L.scene_eta = S.vf_eta; L.cand_toi = c->d_vf_toi; L.chunk_hits = c->d_hit_off;
+  L.one_sided = o.vf_one_sided != 0;
   launch_accd(dim, true, L, s);
+  L.one_sided = 0;
   if (dim == 3) {
     L.q = c->d_eeq; L.n_dev = c->d_counters + C_CAND_EE; L.cap = c->ee_cap; L.scene_eta = S.ee_eta;
     L.cand_toi = c->d_ee_toi; L.chunk_hits = c->d_hit_off + ncv;
@@ -309,9 +311,9 @@ int mclg_batch_set_scenes(mclg_batch* h, int nscenes, const int* voff, const int
   b->opts.assign(nscenes, opts[0]);
   for (int sc = 0; sc < nscenes; ++sc) {
     mclg_options o = opts[nopts == 1 ? 0 : sc];
-    if (o.vf_one_sided) return c->fail(MCLG_ERR_UNSUPPORTED, "vf_one_sided has no GPU implementation");
-    if (o.reptri != opts[0].reptri || o.discrete != opts[0].discrete || o.continuous != opts[0].continuous)
-      return c->fail(MCLG_ERR_UNSUPPORTED, "batch: reptri/discrete/continuous must agree across scenes");
+    if (o.reptri != opts[0].reptri || o.discrete != opts[0].discrete || o.continuous != opts[0].continuous ||
+        o.vf_one_sided != opts[0].vf_one_sided)
+      return c->fail(MCLG_ERR_UNSUPPORTED, "batch: reptri/discrete/continuous/vf_one_sided must agree across scenes");
     o.vf_ccd_eta = std::min(o.vf_ccd_eta, o.box_eta);  // BVHTree.cpp:39-40
     o.ee_ccd_eta = std::min(o.ee_ccd_eta, o.box_eta);
     b->opts[sc] = o;

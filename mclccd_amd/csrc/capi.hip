@@ -170,8 +170,6 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   }
   if (nv_ == 0 || !V0 || !V1 || !P) return fail(MCLG_ERR_ARG, "update: null input");
   if (np_ > (1 << 26)) return fail(MCLG_ERR_ARG, "update: %d primitives exceeds the 2^26 limit", np_);
-  if (opt.vf_one_sided)
-    return fail(MCLG_ERR_UNSUPPORTED, "vf_one_sided (CTCD narrow phase) has no GPU implementation");
   // BVHTree.cpp:39-40
   opt.vf_ccd_eta = std::min(opt.vf_ccd_eta, opt.box_eta);
   opt.ee_ccd_eta = std::min(opt.ee_ccd_eta, opt.box_eta);
@@ -270,13 +268,12 @@ int Ctx::read_counters() {
 // BVHTree.cpp:129-159 (+ collide / get_candidates / narrow phase / discrete)
 int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned flags) {
   counts = mclg_counts{0, 0, 0, 0, 0, 0, 0};
+  hits_one_sided = opt.vf_one_sided != 0;
   keep_cand = (flags & MCLG_KEEP_CANDIDATES) != 0;
   keep_pairs = (flags & MCLG_KEEP_LEAF_PAIRS) != 0;
   if (!have_tree || np <= 1) return MCLG_OK;  // BVHTree.cpp:131-132
   if (pdim != dim) return fail(MCLG_ERR_ARG, "traverse: primitives must have pdim == dim (%d != %d)", pdim, dim);
   if (!V0 || !V1 || !P) return fail(MCLG_ERR_ARG, "traverse: null input");
-  if (opt.vf_one_sided)
-    return fail(MCLG_ERR_UNSUPPORTED, "vf_one_sided (CTCD narrow phase) has no GPU implementation");
   timing = true;
   mark(7);
   CHECK(check_indices(P, (long long)np * dim, nv, "traverse: P"));
@@ -361,7 +358,9 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       mark(12);
       L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
       L.eta = opt.vf_ccd_eta; L.cand_toi = d_vf_toi; L.chunk_hits = d_hit_off;
+      L.one_sided = opt.vf_one_sided != 0;
       launch_accd(dim, true, L, stream);
+      L.one_sided = 0;
       mark(16);
       L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
       L.eta = opt.ee_ccd_eta; L.cand_toi = d_ee_toi; L.chunk_hits = d_hit_off + ncv;

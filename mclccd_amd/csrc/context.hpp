@@ -33,6 +33,7 @@ struct Ctx {
   bool have_tree = false;
   bool rank_set = false;
   bool keep_cand = false, keep_pairs = false;
+  bool hits_one_sided = false;  // the last traverse used CTCD for VF (no TOI adjoint for it)
   mclg_counts counts{0, 0, 0, 0, 0};
 
   // device buffers (grown on demand; cap in bytes)

@@ -106,6 +106,7 @@ struct AccdLaunch {
   unsigned long long* counters;
   int num_cus;
   int blocks_per_cu;
+  int one_sided = 0;                 // VF: CTCD with the wrong-side test instead of ACCD
 };
 void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
 long long accd_chunks(unsigned long long cap);

@@ -14,6 +14,9 @@
 //   representative-triangle candidates (BVHTree.cpp:252-398) into the VF / EE
 //   stencil queues.
 // k_accd: one lane per stencil, FP64 ACCD (NarrowPhase.cpp:634-711).
+#include <cmath>
+
+#include "ctcd.hpp"
 #include "geom.hpp"
 #include "kernels.hpp"
 
@@ -709,6 +712,34 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
   }
 }
 
+// CTCD vertex-face narrow phase (options.vf_one_sided; BVHTree.cpp:446-447):
+// the VF queue, outputs and per-chunk hit counts of the VF ACCD kernel.  cosr,
+// sinr: the root finder's shift rotation, cos/sin(94 * 0.017453293).
+template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(AccdArgs A, double cosr, double sinr) {
+  using V = typename VLoad<D>::V;
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
+  const long long nchunks = (n + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * AC_WAVES;
+  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
+    const long long i = ch * kWave + lane;
+    int hit = 0;
+    if (i < n) {
+      const int4 sv = A.q[i];
+      const int id[4] = {sv.x, sv.y, sv.z, sv.w};
+      V v0[D + 1], v1[D + 1];
+#pragma unroll
+      for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
+      const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
+      double toi = -1.0;
+      hit = ctcd::Query<D>::run(v0, v1, eta, toi, cosr, sinr) == 1;
+      __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
+    }
+    const uint64_t m = __ballot(hit != 0);
+    if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+  }
+}
+
 __global__ void k_hits_total(const unsigned long long* off, long long last, unsigned long long* counters) {
   counters[C_HITS] = off[last];
 }
@@ -798,6 +829,13 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);
+  if (vf && L.one_sided) {
+    // the host libm's values, as the reference computes them (rpoly.h:74-77)
+    static const double rot = 94.0 * 0.017453293, cosr = std::cos(rot), sinr = std::sin(rot);
+    if (dim == 3) k_ctcd<3><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    else k_ctcd<2><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    return;
+  }
   if (dim == 3) {
     if (vf) k_accd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
     else k_accd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A);

@@ -20,6 +20,7 @@
 //   projections       include/MCL/ccd_internal/Projection.hpp:60-145,193-204,241-302
 //   tri-tri           include/MCL/ccd_internal/tt_isect.hpp:71-270,499-585
 //   2D edge-edge      include/MCL/NarrowPhase.cpp:229-257
+//   CTCD (vf_one_sided)  ctcd_oracle.hpp
 //
 // Arithmetic follows the reference's evaluation order as Eigen 3.4 evaluates
 // it for fixed-size vectors on x86-64 without FMA: dot(a,b) = (a0*b0 + a1*b1)
@@ -55,6 +56,8 @@
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#include "ctcd_oracle.hpp"
 
 namespace mclo {
 
@@ -546,6 +549,7 @@ inline bool discrete_edge_edge2(const double* p0, const double* p1, const double
 struct Options {
   double box_eta = 1e-6, vf_ccd_eta = 1e-6, ee_ccd_eta = 1e-6;
   bool reptri = true, threaded = true, discrete = true, continuous = true;
+  bool vf_one_sided = false;  // CTCD for vertex-face stencils (ctcd_oracle.hpp)
 };
 
 struct Stencil {
@@ -816,13 +820,14 @@ template <int D> class Tree {
     return n;
   }
 
-  // BVHTree.cpp:428-461 (the ACCD branch; vf_one_sided is not part of the checker)
+  // BVHTree.cpp:428-461
   double narrow_phase(const double* V0, const double* V1, const Stencil& s, AccdStats* st) const {
     double a[D + 1][D], b[D + 1][D];
     for (int i = 0; i < D + 1; ++i)
       for (int k = 0; k < D; ++k) { a[i][k] = V0[s.v[i] * D + k]; b[i][k] = V1[s.v[i] * D + k]; }
     double toi = -1;
     const double eta = s.is_vf ? options.vf_ccd_eta : options.ee_ccd_eta;
+    if (s.is_vf && options.vf_one_sided) return ctcd::query_vf<D>(a, b, eta, toi) == 1 ? toi : -1;
     if (additive_ccd<D>(a, b, eta, s.is_vf, toi, st)) return toi;
     return -1;
   }

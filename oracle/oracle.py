@@ -18,7 +18,8 @@ _lib = None
 
 class OracleOptions(C.Structure):
     _fields_ = [("box_eta", C.c_double), ("vf_ccd_eta", C.c_double), ("ee_ccd_eta", C.c_double),
-                ("reptri", C.c_int), ("threaded", C.c_int), ("discrete", C.c_int), ("continuous", C.c_int)]
+                ("reptri", C.c_int), ("threaded", C.c_int), ("discrete", C.c_int), ("continuous", C.c_int),
+                ("vf_one_sided", C.c_int)]
 
 
 def build() -> str:
@@ -64,6 +65,13 @@ def lib():
         L.oracle_accd_grad.argtypes = [I, P, P, D, I, P, P, P]
         L.oracle_toi_list.argtypes = [I, P, P, P, P, LL, D, D, P, P]
         L.oracle_toi_grad_list.argtypes = [I, P, P, I, P, P, P, LL, D, D, P, P]
+        L.oracle_ctcd_vf.argtypes = [I, P, P, D, P]
+        L.oracle_rpoly.argtypes = [P, I, P, P]
+        L.oracle_find_intervals.argtypes = [P, I, I, P, P, I]
+        L.oracle_ctcd_subtest.argtypes = [I, P, P, D, P, I, P]
+        L.oracle_wrong_side.argtypes = [I, P, P, D]
+        L.oracle_ray_box.argtypes = [P, P, P, P]
+        L.oracle_ctcd_list.argtypes = [I, P, P, P, LL, D, P]
         _lib = L
     return _lib
 
@@ -96,9 +104,9 @@ class Oracle:
             self._h = None
 
     def set_options(self, box_eta=1e-6, vf_ccd_eta=1e-6, ee_ccd_eta=1e-6, reptri=True, threaded=True,
-                    discrete=True, continuous=True, nthreads=0):
+                    discrete=True, continuous=True, vf_one_sided=False, nthreads=0):
         o = OracleOptions(box_eta, vf_ccd_eta, ee_ccd_eta, int(reptri), int(threaded), int(discrete),
-                          int(continuous))
+                          int(continuous), int(vf_one_sided))
         lib().oracle_set_options(self._h, C.byref(o), int(nthreads))
 
     def etas(self):
@@ -253,6 +261,60 @@ def toi_grad_list(V0, V1, sten, is_vf, w, eta_vf, eta_ee):
     lib().oracle_toi_grad_list(V0.shape[1], _p(V0), _p(V1), len(V0), _p(s), _p(v), _p(w), len(s), float(eta_vf),
                                float(eta_ee), _p(g0), _p(g1))
     return g0, g1
+
+
+def ctcd_vf(dim, verts0, verts1, eta):
+    """CTCD vertex-face query with the wrong-side test: (code, toi); code 1 hit,
+    0 no hit, -1 only wrong-side hits (toi -1 unless code is 1)."""
+    v0, v1 = _f64(verts0), _f64(verts1)
+    toi = C.c_double(-1.0)
+    r = lib().oracle_ctcd_vf(dim, _p(v0), _p(v1), float(eta), C.byref(toi))
+    return r, toi.value
+
+
+def rpoly(coeffs):
+    """Jenkins-Traub zeros of a polynomial (decreasing powers): (count, real, imag)."""
+    op = _f64(coeffs)
+    deg = len(op) - 1
+    zr, zi = np.zeros(max(deg, 1)), np.zeros(max(deg, 1))
+    n = lib().oracle_rpoly(_p(op), deg, _p(zr), _p(zi))
+    return n, zr, zi
+
+
+def find_intervals(coeffs, pos):
+    """CTCD's sign intervals of a polynomial over [0, 1] as a list of (l, u)."""
+    op = _f64(coeffs)
+    lo, hi = np.zeros(8), np.zeros(8)
+    n = lib().oracle_find_intervals(_p(op), len(op) - 1, int(pos), _p(lo), _p(hi), 8)
+    return [(lo[i], hi[i]) for i in range(n)]
+
+
+def ctcd_subtest(kind, x0, x1, eta):
+    """One 3D sub-test: kind 0 vertex-face, 1 vertex-edge, 2 vertex-vertex.  Returns (hit, times)."""
+    a, b = _f64(x0).reshape(-1), _f64(x1).reshape(-1)
+    out = np.zeros(512)
+    nt = C.c_int(0)
+    h = lib().oracle_ctcd_subtest(int(kind), _p(a), _p(b), float(eta), _p(out), 512, C.byref(nt))
+    return bool(h), out[:min(nt.value, 512)].copy()
+
+
+def wrong_side(dim, verts0, verts1, t):
+    v0, v1 = _f64(verts0), _f64(verts1)
+    return bool(lib().oracle_wrong_side(dim, _p(v0), _p(v1), float(t)))
+
+
+def ray_box(p0, p1, bmin, bmax):
+    p0, p1, bmin, bmax = (_f64(x) for x in (p0, p1, bmin, bmax))
+    return bool(lib().oracle_ray_box(_p(p0), _p(p1), _p(bmin), _p(bmax)))
+
+
+def ctcd_list(V0, V1, sten, eta):
+    """CTCD TOI per vertex-face stencil (-1: no hit)."""
+    V0, V1 = _f64(V0), _f64(V1)
+    s = _i32(sten).reshape(-1, 4)
+    out = np.zeros(len(s))
+    lib().oracle_ctcd_list(V0.shape[1], _p(V0), _p(V1), _p(s), len(s), float(eta), _p(out))
+    return out
 
 
 def run_scene(V0, V1, P, theta, dim=None, leaf_pairs=True, candidates=True, nthreads=0, **opts):

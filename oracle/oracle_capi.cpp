@@ -28,6 +28,7 @@ double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::
 struct oracle_options {
   double box_eta, vf_ccd_eta, ee_ccd_eta;
   int reptri, threaded, discrete, continuous;
+  int vf_one_sided;
 };
 
 template <int D> static void set_opts(OracleCtx* c, const oracle_options* o, int nthreads) {
@@ -39,6 +40,7 @@ template <int D> static void set_opts(OracleCtx* c, const oracle_options* o, int
   t.options.threaded = o->threaded != 0;
   t.options.discrete = o->discrete != 0;
   t.options.continuous = o->continuous != 0;
+  t.options.vf_one_sided = o->vf_one_sided != 0;
   t.nthreads = nthreads;
 }
 
@@ -247,6 +249,56 @@ int oracle_edge_edge2(const double* e0, const double* e1) {
   return mclo::discrete_edge_edge2(e0, e0 + 2, e1, e1 + 2) ? 1 : 0;
 }
 
+// ---- CTCD (vf_one_sided) ----------------------------------------------------
+// query_ccd_vf with the wrong-side test: 1 hit (toi written), 0 none, -1 only wrong-side hits.
+int oracle_ctcd_vf(int dim, const double* verts0, const double* verts1, double eta, double* toi) {
+  double t = -1;
+  int r;
+  if (dim == 3) r = mclo::ctcd::query_vf<3>(reinterpret_cast<const double(*)[3]>(verts0), reinterpret_cast<const double(*)[3]>(verts1), eta, t);
+  else r = mclo::ctcd::query_vf<2>(reinterpret_cast<const double(*)[2]>(verts0), reinterpret_cast<const double(*)[2]>(verts1), eta, t);
+  if (toi) *toi = r == 1 ? t : -1.0;
+  return r;
+}
+
+// Jenkins-Traub: number of zeros (or -1); real and imaginary parts in zr, zi.
+int oracle_rpoly(const double* op, int degree, double* zr, double* zi) {
+  mclo::ctcd::Rpoly rf;
+  return rf.solve(op, degree, zr, zi);
+}
+
+// CTCD intervals of a polynomial (op is copied; the reference normalises in place).
+int oracle_find_intervals(const double* op, int n, int pos, double* lo, double* hi, int cap) {
+  double c[7];
+  for (int i = 0; i <= n && i < 7; ++i) c[i] = op[i];
+  std::vector<mclo::ctcd::Interval> v;
+  mclo::ctcd::find_intervals(c, n, v, pos != 0);
+  for (int i = 0; i < (int)v.size() && i < cap; ++i) { lo[i] = v[i].l; hi[i] = v[i].u; }
+  return (int)v.size();
+}
+
+// Sub-tests of the 3D query (kind 0 vertex-face, 1 vertex-edge, 2 vertex-vertex),
+// verts = the sub-test's points at t0 then t1 (4, 3 or 2 points of 3 doubles).
+// Returns the hit flag; the candidate times go to out (count in *nt).
+int oracle_ctcd_subtest(int kind, const double* x0, const double* x1, double eta, double* out, int cap, int* nt) {
+  std::vector<double> all_t;
+  bool hit = false;
+  if (kind == 0) hit = mclo::ctcd::vertex_face(x0, x0 + 3, x0 + 6, x0 + 9, x1, x1 + 3, x1 + 6, x1 + 9, eta, all_t);
+  else if (kind == 1) hit = mclo::ctcd::vertex_edge3(x0, x0 + 3, x0 + 6, x1, x1 + 3, x1 + 6, eta, all_t);
+  else hit = mclo::ctcd::vertex_vertex3(x0, x0 + 3, x1, x1 + 3, eta, all_t);
+  for (int i = 0; i < (int)all_t.size() && i < cap; ++i) out[i] = all_t[i];
+  *nt = (int)all_t.size();
+  return hit ? 1 : 0;
+}
+
+int oracle_wrong_side(int dim, const double* verts0, const double* verts1, double t) {
+  if (dim == 3) return mclo::ctcd::wrong_side3(reinterpret_cast<const double(*)[3]>(verts0), reinterpret_cast<const double(*)[3]>(verts1), t);
+  return mclo::ctcd::wrong_side2(reinterpret_cast<const double(*)[2]>(verts0), reinterpret_cast<const double(*)[2]>(verts1), t);
+}
+
+int oracle_ray_box(const double* p0, const double* p1, const double* bmin, const double* bmax) {
+  return mclo::ctcd::ray_box(p0, p1, bmin, bmax) ? 1 : 0;
+}
+
 }  // extern "C"
 
 // ---- TOI and its reference gradient ---------------------------------------
@@ -304,6 +356,18 @@ void oracle_toi_list(int dim, const double* V0, const double* V1, const int* ste
     toi[i] = hit ? t : -1.0;
     if (sig) sig[i] = word;
     mclo::branch_sig = nullptr;
+  }
+}
+
+// CTCD TOI per vertex-face stencil (-1: no hit, or only wrong-side hits).
+void oracle_ctcd_list(int dim, const double* V0, const double* V1, const int* sten, long long n, double eta,
+                      double* toi) {
+#pragma omp parallel for schedule(dynamic, 256)
+  for (long long i = 0; i < n; ++i) {
+    double a[12], b[12];
+    for (int j = 0; j <= dim; ++j)
+      for (int k = 0; k < dim; ++k) { a[j * dim + k] = V0[(size_t)sten[4 * i + j] * dim + k]; b[j * dim + k] = V1[(size_t)sten[4 * i + j] * dim + k]; }
+    oracle_ctcd_vf(dim, a, b, eta, toi + i);
   }
 }
 

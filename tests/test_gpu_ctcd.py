@@ -1,0 +1,136 @@
+"""The CTCD vertex-face narrow phase (options.vf_one_sided) on the GPU vs the
+CPU oracle, through the C ABI.
+
+Bar: every vertex-face candidate's CTCD time of impact (or -1) equal to the
+oracle's, and the full hit, candidate and discrete sets equal to the oracle
+tree's with vf_one_sided set (EE stencils stay on ACCD).  The root finder's
+exp/log come from different math libraries on the two sides (device library
+vs the host's libm), so a bound estimate can differ in its last bit; the
+tests count such TOI differences and bound them (at most one in a thousand
+candidates, within 1e-12 relative), while hit decisions must agree exactly.
+"""
+import numpy as np
+import pytest
+
+from mclccd_amd import scenes
+from oracle import oracle as O
+from tests.parity import hit_map
+from tests.test_gpu_parity import check_parity, run_both
+
+pytestmark = pytest.mark.gpu
+
+TOI_RTOL = 1e-12
+
+
+def _vf_candidates(t):
+    s, v, toi = t.candidates()
+    m = v.astype(bool)
+    return s[m], toi[m]
+
+
+def _compare_toi(got, want):
+    assert got.shape == want.shape
+    assert np.array_equal(got >= 0, want >= 0), "hit decisions differ"
+    h = want >= 0
+    diff = got[h] != want[h]
+    rel = np.abs(got[h] - want[h]) / np.maximum(np.abs(want[h]), 1e-300)
+    assert np.all(rel <= TOI_RTOL), float(rel.max())
+    assert int(diff.sum()) <= max(1, len(got) // 1000), int(diff.sum())
+    return int(diff.sum())
+
+
+@pytest.mark.parametrize("scene", ["cloth_fold", "soup0", "soup1", "layers"])
+def test_ctcd_candidates_match_oracle(gpu, scene):
+    if scene == "cloth_fold":
+        V0, V1, F, th = scenes.cloth_fold(n=32)
+    elif scene == "layers":
+        V0, V1, F, th = scenes.stacked_layers(n=12)
+    else:
+        V0, V1, F, th = scenes.random_soup(400, seed=int(scene[-1]), share=True, motion=0.25)
+        th = 1e-2
+    from mclccd_amd.tree import BVHTree
+
+    t = BVHTree(3)
+    t.options.box_eta = t.options.vf_ccd_eta = t.options.ee_ccd_eta = th
+    t.options.vf_one_sided = True
+    t.update(V0, V1, F)
+    t.run(V0, V1, F, keep_candidates=True)
+    sten, toi = _vf_candidates(t)
+    assert len(sten) > 100
+    want = O.ctcd_list(V0, V1, sten, th)
+    ndiff = _compare_toi(toi, want)
+    print(f"[ctcd] {scene}: {len(sten)} VF candidates, {int((want >= 0).sum())} hits, {ndiff} TOIs differ in the last bits")
+
+
+@pytest.mark.parametrize("strict", [True, False])
+def test_ctcd_tree_parity(gpu, strict):
+    V0, V1, F, th = scenes.random_soup(300, seed=4, share=True, motion=0.25)
+    t, o = run_both(V0, V1, F, 1e-2, strict, vf_one_sided=True)
+    c = check_parity(t, o, strict)
+    assert c["hits"] > 0
+
+
+def test_ctcd_2d(gpu):
+    V0, V1, E, th = scenes.disk2d(npts=3000, edges="all")
+    t, o = run_both(V0, V1, E, th, strict=True, vf_one_sided=True)
+    check_parity(t, o, strict=True)
+    sten, toi = _vf_candidates(t)
+    assert np.array_equal(toi, O.ctcd_list(V0, V1, sten, th))
+
+
+def test_ctcd_falling_vertex_closed_form(gpu):
+    from mclccd_amd.tree import BVHTree
+
+    V0 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.2, 0.2, 0.5], [1.2, 0.2, 0.5], [0.2, 1.2, 0.5]])
+    V1 = V0.copy()
+    V1[3:, 2] = -0.5
+    F = np.array([[0, 1, 2], [3, 4, 5]], np.int32)
+    eta = 1e-3
+    t = BVHTree(3)
+    t.options.box_eta = t.options.vf_ccd_eta = t.options.ee_ccd_eta = eta
+    t.options.vf_one_sided = True
+    t.update(V0, V1, F)
+    t.run(V0, V1, F)
+    s, v, toi = t.hits()
+    got = {tuple(x): y for x, y, w in zip(s.tolist(), toi.tolist(), v.tolist()) if w}
+    assert got[(3, 0, 1, 2)] == pytest.approx(0.5 - eta, abs=1e-12)
+    # the same fall from below the face is a wrong-side contact: no VF hit from vertex 3
+    t.update(V1, V0, F)
+    t.run(V1, V0, F)
+    s, v, _ = t.hits()
+    assert (3, 0, 1, 2) not in {tuple(x) for x, w in zip(s.tolist(), v.tolist()) if w}
+
+
+def test_ctcd_batch(gpu):
+    from mclccd_amd.batch import from_scenes
+    from oracle.oracle import run_scene
+
+    sc_list = scenes.batch_sheets(count=8, nmin=6, nmax=16, seed=9)
+    b, V0, V1 = from_scenes(sc_list, vf_one_sided=True)
+    b.run(V0, V1)
+    sc_h, s_h, v_h, t_h = b.hits()
+    for k, (W0, W1, F, th) in enumerate(sc_list):
+        o = run_scene(W0, W1, F, th, vf_one_sided=True)
+        rs, rv, rt = o.hits()
+        m = sc_h == k
+        gm, rm = hit_map(s_h[m], v_h[m], t_h[m]), hit_map(rs, rv, rt)
+        assert gm.keys() == rm.keys(), f"scene {k}"
+        for key in gm:
+            tol = TOI_RTOL if key[0] == 1 else 1e-9
+            assert np.allclose(gm[key], rm[key], rtol=tol, atol=0), f"scene {k}"
+
+
+def test_ctcd_has_no_toi_adjoint(gpu):
+    from mclccd_amd import _lib
+    from mclccd_amd.autograd import toi_grad
+    from mclccd_amd.tree import BVHTree
+
+    V0, V1, F, th = scenes.random_soup(100, seed=2, share=True, motion=0.25)
+    t = BVHTree(3)
+    t.options.box_eta = t.options.vf_ccd_eta = t.options.ee_ccd_eta = 1e-2
+    t.options.vf_one_sided = True
+    t.update(V0, V1, F)
+    t.run(V0, V1, F)
+    with pytest.raises(_lib.MclgError) as e:
+        toi_grad(t, np.ones(t.counts()["hits"]))
+    assert e.value.code == _lib.ERR_UNSUPPORTED
