@@ -1,4 +1,4 @@
-// Drop-in replacement of mcl::BVHTree<double,DIM> (mattoverby/mclccd,
+// Drop-in replacement of mcl::BVHTree<T,DIM> (mattoverby/mclccd,
 // include/MCL/BVHTree.hpp:30-148) running on an MI355X through the C ABI of
 // mclg.h.  Header-only; link against libmclg.so.
 //
@@ -17,7 +17,15 @@
 //   - vf_one_sided runs the CTCD narrow phase with its wrong-side test on the
 //     GPU for vertex-face stencils (EE stays ACCD), as BVHTree.cpp:446-447 does.
 //     GPU failures throw std::runtime_error.
-//   - Only T = double is provided (the FP64 path).
+//   - T = double and T = float (BVHTree.cpp:504-507).  The GPU path computes in
+//     FP64: a float tree widens its vertices (exactly) to double, so its
+//     candidates, hits and discrete pairs are those of BVHTree<double,DIM> on
+//     the same coordinates, and TOIs are rounded to float on output.  The
+//     reference's float tree pads boxes and runs ACCD in float arithmetic, so a
+//     pair within float rounding of a threshold can differ; its CTCD and
+//     discrete tests widen to double as here.  Leaf boxes handed back (leaves,
+//     visitor volumes) equal the reference's float boxes: a float minus a float
+//     computed in double and rounded once to float is the float difference.
 // When Eigen is available the Eigen overloads of update/traverse are provided
 // and the callbacks take Eigen::Vector4i, as in the reference.
 #ifndef MCL_BVHTREE_HPP
@@ -30,6 +38,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../mclg.h"
@@ -141,11 +150,11 @@ template <typename T, int DIM> class BVHTraverse {
 
 template <typename T, int DIM> class BVHTree;
 
-template <int DIM> class BVHTree<double, DIM> {
+template <typename T, int DIM> class BVHTree {
   static_assert(DIM == 2 || DIM == 3, "DIM must be 2 or 3");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value, "T must be double or float");
 
  public:
-  typedef double T;
   typedef BVHLeaf<T, DIM> LeafType;
 
   // BVHTree.hpp:42-64
@@ -185,8 +194,19 @@ template <int DIM> class BVHTree<double, DIM> {
     push_options();
     int nv = 0;
     for (long long i = 0; i < (long long)np * pdim; ++i) nv = std::max(nv, P[i] + 1);
-    check(mclg_update(ctx_.get(), V0, V1, nv, P, np, pdim, active));
+    check(mclg_update(ctx_.get(), widen(V0, nv, w0_), widen(V1, nv, w1_), nv, P, np, pdim, active));
     np_ = np;
+    nv_ = nv;
+    mirror_ = false;
+    pull_options();
+  }
+
+  // Refit-only update (no reference counterpart): same arguments as update for
+  // the same mesh; keeps the last update's hierarchy and recomputes the boxes
+  // (mclg_update_refit).  Results equal a rebuild's.
+  void refit(const T* V0, const T* V1, const int* P, int np, int pdim, const int* active = nullptr) {
+    push_options();
+    check(mclg_update_refit(ctx_.get(), widen(V0, nv_, w0_), widen(V1, nv_, w1_), nv_, P, np, pdim, active));
     mirror_ = false;
     pull_options();
   }
@@ -196,7 +216,9 @@ template <int DIM> class BVHTree<double, DIM> {
     const bool need_cand = (bool)filter_pair || (bool)narrow_phase;
     const_cast<BVHTree*>(this)->push_options();
     if (np_ <= 1) return;  // BVHTree.cpp:131-132
-    check(mclg_traverse(ctx_.get(), V0, V1, P, need_cand ? MCLG_KEEP_CANDIDATES : 0u));
+    BVHTree* self = const_cast<BVHTree*>(this);
+    check(mclg_traverse(ctx_.get(), widen(V0, nv_, self->w0_), widen(V1, nv_, self->w1_), P,
+                        need_cand ? MCLG_KEEP_CANDIDATES : 0u));
     mclg_counts c;
     check(mclg_get_counts(ctx_.get(), &c));
     if (options.continuous && (append_pair || need_cand)) {
@@ -210,7 +232,7 @@ template <int DIM> class BVHTree<double, DIM> {
         const Vector4i s(sten[4 * i], sten[4 * i + 1], sten[4 * i + 2], sten[4 * i + 3]);
         const bool is_vf = vf[i] != 0;
         if (filter_pair && filter_pair(s, is_vf)) continue;
-        const T t = narrow_phase ? narrow_phase(s, is_vf) : toi[i];
+        const T t = narrow_phase ? narrow_phase(s, is_vf) : (T)toi[i];
         if (t >= 0 && append_pair) append_pair(s, is_vf, t);
       }
     }
@@ -234,8 +256,10 @@ template <int DIM> class BVHTree<double, DIM> {
     RV a = V0.template cast<T>(), b = V1.template cast<T>();
     RI p = P.template cast<int>();
     push_options();
-    check(mclg_update(ctx_.get(), a.data(), b.data(), (int)a.rows(), p.data(), (int)p.rows(), (int)p.cols(), act));
+    check(mclg_update(ctx_.get(), widen(a.data(), (int)a.rows(), w0_), widen(b.data(), (int)b.rows(), w1_),
+                      (int)a.rows(), p.data(), (int)p.rows(), (int)p.cols(), act));
     np_ = (int)p.rows();
+    nv_ = (int)a.rows();
     mirror_ = false;
     pull_options();
   }
@@ -263,9 +287,9 @@ template <int DIM> class BVHTree<double, DIM> {
       L.idx = (int)i;
       const double* x = b.data() + i * 6 * DIM;
       for (int k = 0; k < DIM; ++k) {
-        L.box.min()[k] = x[k]; L.box.max()[k] = x[DIM + k];
-        L.box.t0.min()[k] = x[2 * DIM + k]; L.box.t0.max()[k] = x[3 * DIM + k];
-        L.box.t1.min()[k] = x[4 * DIM + k]; L.box.t1.max()[k] = x[5 * DIM + k];
+        L.box.min()[k] = (T)x[k]; L.box.max()[k] = (T)x[DIM + k];
+        L.box.t0.min()[k] = (T)x[2 * DIM + k]; L.box.t0.max()[k] = (T)x[3 * DIM + k];
+        L.box.t1.min()[k] = (T)x[4 * DIM + k]; L.box.t1.max()[k] = (T)x[5 * DIM + k];
       }
       L.box.active = a[(size_t)i] != 0;
       for (int j = 0; j < 3; ++j) { L.v[j] = (f[(size_t)i] >> j) & 1; L.e[j] = (f[(size_t)i] >> (3 + j)) & 1; }
@@ -308,7 +332,16 @@ template <int DIM> class BVHTree<double, DIM> {
 
  private:
   std::shared_ptr<mclg_ctx> ctx_;
-  int np_ = 0;
+  int np_ = 0, nv_ = 0;
+  std::vector<double> w0_, w1_;  // FP64 copies of float vertices
+
+  // The device path reads FP64 vertices: double input is passed through,
+  // float input widened (exactly) into buf.
+  static const double* widen(const double* V, int, std::vector<double>&) { return V; }
+  static const double* widen(const float* V, int nv, std::vector<double>& buf) {
+    buf.assign(V, V + (size_t)nv * DIM);
+    return buf.data();
+  }
   // host copy of the tree for visitor traversals (valid until the next update)
   bool mirror_ = false;
   int root_ = -1;
@@ -368,8 +401,8 @@ template <int DIM> class BVHTree<double, DIM> {
   void pull_options() {
     mclg_options o;
     check(mclg_get_options(ctx_.get(), &o));
-    options.vf_ccd_eta = o.vf_ccd_eta;  // clamped by update (BVHTree.cpp:39-40)
-    options.ee_ccd_eta = o.ee_ccd_eta;
+    options.vf_ccd_eta = (T)o.vf_ccd_eta;  // clamped by update (BVHTree.cpp:39-40)
+    options.ee_ccd_eta = (T)o.ee_ccd_eta;
   }
 };
 

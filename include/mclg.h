@@ -64,6 +64,7 @@ void mclg_destroy(mclg_ctx* ctx);
 int mclg_set_options(mclg_ctx* ctx, const mclg_options* o);
 int mclg_get_options(const mclg_ctx* ctx, mclg_options* o);
 
+/* ctx == NULL: why the calling thread's last mclg_create failed. */
 const char* mclg_last_error(const mclg_ctx* ctx);
 
 /* Runs the context's work on a caller-provided hipStream_t (NULL: the context's
@@ -79,6 +80,22 @@ int mclg_update(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const
                 const int* active);
 int mclg_update_device(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
                        const int* active);
+
+/* Refit-only update.  No reference counterpart (the reference rebuilds its
+ * tree on every update, BVHTree.cpp:124).  Same arguments and results as
+ * mclg_update for the same mesh: recomputes the leaf boxes (and active flags)
+ * from V0/V1 and the node volumes bottom-up, but keeps the previous full
+ * update's leaf order and hierarchy, skipping the Morton codes and the sort.
+ * Leaf pairs, candidates, hits and discrete pairs equal a rebuild's (the sets
+ * do not depend on the hierarchy); pairs are oriented by the kept leaf order.
+ * Traversal slows as the kept hierarchy ages under large motion: call
+ * mclg_update again then.  P must be the last update's; nv, np and pdim must
+ * match it, else MCLG_ERR_STATE.  Representative-triangle flags are kept, as
+ * the reference keeps them when the leaf count is unchanged (BVHTree.cpp:44-48). */
+int mclg_update_refit(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
+                      const int* active);
+int mclg_update_refit_device(mclg_ctx* ctx, const double* V0, const double* V1, int nv, const int* P, int np,
+                             int pdim, const int* active);
 
 /* BVHTree<T,DIM>::traverse(const T* V0, const T* V1, const int* P) const —
  * include/MCL/BVHTree.hpp:86, BVHTree.cpp:129-159 (+ collide, get_candidates,

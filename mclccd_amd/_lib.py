@@ -51,9 +51,22 @@ class MclgError(RuntimeError):
         self.code = code
 
 
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm ships its own HIP runtime.  Two HIP runtimes in one process
+    cannot both use the GPU (the first to initialise holds it), so when torch
+    is importable it is imported before the library: libmclg.so's libamdhip64
+    dependency then resolves (same SONAME) to the runtime torch already loaded,
+    whichever of torch and this package the caller uses first."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run `python -m mclccd_amd.build`")
+    _share_torch_hip_runtime()
     L = C.CDLL(LIB_PATH)
     P, I, U, I64 = C.c_void_p, C.c_int, C.c_uint, C.c_int64
     sig = {
@@ -66,6 +79,8 @@ def _load():
         "mclg_set_stream": (I, [P, P]),
         "mclg_update": (I, [P, P, P, I, P, I, I, P]),
         "mclg_update_device": (I, [P, P, P, I, P, I, I, P]),
+        "mclg_update_refit": (I, [P, P, P, I, P, I, I, P]),
+        "mclg_update_refit_device": (I, [P, P, P, I, P, I, I, P]),
         "mclg_traverse": (I, [P, P, P, P, U]),
         "mclg_traverse_device": (I, [P, P, P, P, U]),
         "mclg_get_counts": (I, [P, P]),

@@ -86,6 +86,9 @@ void mclg_default_options(mclg_options* o) {
   o->ee_robust = 1;
 }
 
+// Why the last mclg_create failed (mclg_last_error(NULL)).
+static thread_local std::string g_create_error = "no error";
+
 int mclg_create(int dim, mclg_ctx** out) {
   if (!out) return MCLG_ERR_ARG;
   *out = nullptr;
@@ -93,19 +96,24 @@ int mclg_create(int dim, mclg_ctx** out) {
   Ctx* c = new Ctx();
   c->dim = dim;
   mclg_default_options(&c->opt);
-  if (hipGetDevice(&c->device) != hipSuccess) { delete c; return MCLG_ERR_HIP; }
-  if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) { delete c; return MCLG_ERR_HIP; }
+  auto failed = [&](hipError_t e, const char* what) {
+    g_create_error = std::string("mclg_create: ") + what + ": " + hipGetErrorString(e);
+    delete c;
+    return MCLG_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipGetDevice(&c->device)) != hipSuccess) return failed(e, "hipGetDevice");
+  if ((e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking)) != hipSuccess)
+    return failed(e, "hipStreamCreateWithFlags");
   c->stream = c->own_stream;
   for (int i = 0; i < kNumEvents; ++i)
-    if (hipEventCreate(&c->ev[i]) != hipSuccess) { delete c; return MCLG_ERR_HIP; }
+    if ((e = hipEventCreate(&c->ev[i])) != hipSuccess) return failed(e, "hipEventCreate");
   int cus = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device);
   c->num_cus = cus > 0 ? cus : 256;
-  if (hipHostMalloc((void**)&c->h_counters, sizeof(unsigned long long) * C_COUNT, 0) != hipSuccess) {
-    delete c;
-    return MCLG_ERR_HIP;
-  }
-  if (hipHostMalloc((void**)&c->h_minmax, sizeof(int) * 2, 0) != hipSuccess) { delete c; return MCLG_ERR_HIP; }
+  if ((e = hipHostMalloc((void**)&c->h_counters, sizeof(unsigned long long) * C_COUNT, 0)) != hipSuccess)
+    return failed(e, "hipHostMalloc");
+  if ((e = hipHostMalloc((void**)&c->h_minmax, sizeof(int) * 2, 0)) != hipSuccess) return failed(e, "hipHostMalloc");
   *out = reinterpret_cast<mclg_ctx*>(c);
   return MCLG_OK;
 }
@@ -131,7 +139,7 @@ int mclg_get_options(const mclg_ctx* h, mclg_options* o) {
 
 const char* mclg_last_error(const mclg_ctx* h) {
   const Ctx* c = reinterpret_cast<const Ctx*>(h);
-  return c ? c->err.c_str() : "null context";
+  return c ? c->err.c_str() : g_create_error.c_str();
 }
 
 int mclg_set_stream(mclg_ctx* h, void* s) {
@@ -160,8 +168,12 @@ int Ctx::check_indices(const int* P, long long n, int nv, const char* what) {
 }
 
 // BVHTree.cpp:27-126
-int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int np_, int pdim_, const int* active) {
+int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int np_, int pdim_, const int* active,
+                bool refit) {
   if (np_ < 0 || pdim_ < 1 || nv_ < 0) return fail(MCLG_ERR_ARG, "update: bad sizes nv=%d np=%d pdim=%d", nv_, np_, pdim_);
+  if (refit && !(have_tree && np_ == np && nv_ == nv && pdim_ == pdim))
+    return fail(MCLG_ERR_STATE, "refit: needs a previous update with the same nv, np and pdim (have %d/%d/%d, got %d/%d/%d)",
+                have_tree ? nv : 0, have_tree ? np : 0, have_tree ? pdim : 0, nv_, np_, pdim_);
   if (np_ == 0) {  // BVHTree.cpp:29-34
     np = 0;
     have_tree = false;
@@ -228,9 +240,9 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_ktmp, &cap_ktmp, sizeof(uint32_t) * (size_t)np, "keys tmp"));
   CHECK(grow((void**)&d_vtmp, &cap_vtmp, sizeof(uint32_t) * (size_t)np, "vals tmp"));
   CHECK(grow((void**)&d_hist, &cap_hist, sizeof(uint32_t) * radix_hist_words(np), "histogram"));
-  launch_morton(dim, d_boxes, np, d_cbounds, SceneArgs(), d_keys, d_order, nullptr, stream);
+  if (!refit) launch_morton(dim, d_boxes, np, d_cbounds, SceneArgs(), d_keys, d_order, nullptr, stream);
   mark(4);
-  launch_radix_sort(d_keys, d_order, d_ktmp, d_vtmp, d_hist, np, 32, stream);
+  if (!refit) launch_radix_sort(d_keys, d_order, d_ktmp, d_vtmp, d_hist, np, 32, stream);
   mark(5);
   // LBVH
   CHECK(grow((void**)&d_nodes, &cap_nodes, node_bytes(dim) * (size_t)(2 * np - 1), "nodes"));
@@ -466,6 +478,22 @@ int mclg_update(mclg_ctx* h, const double* V0, const double* V1, int nv, const i
   if (!V0 || !V1 || !P || nv == 0) return c->fail(MCLG_ERR_ARG, "update: null input");
   CHECK(c->stage(V0, V1, nv, P, (long long)np * pdim, active));
   return c->update(c->d_stage_v0, c->d_stage_v1, nv, c->d_stage_p, np, pdim, active ? c->d_stage_act : nullptr);
+}
+
+int mclg_update_refit(mclg_ctx* h, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
+                      const int* active) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  if (np <= 0 || pdim < 1 || nv <= 0 || !V0 || !V1 || !P) return c->fail(MCLG_ERR_ARG, "refit: bad input");
+  CHECK(c->stage(V0, V1, nv, P, (long long)np * pdim, active));
+  return c->update(c->d_stage_v0, c->d_stage_v1, nv, c->d_stage_p, np, pdim, active ? c->d_stage_act : nullptr, true);
+}
+
+int mclg_update_refit_device(mclg_ctx* h, const double* V0, const double* V1, int nv, const int* P, int np, int pdim,
+                             const int* active) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  return c->update(V0, V1, nv, P, np, pdim, active, true);
 }
 
 int mclg_traverse_device(mclg_ctx* h, const double* V0, const double* V1, const int* P, unsigned flags) {

@@ -113,7 +113,10 @@ struct Ctx {
   int check_indices(const int* P, long long n, int nv, const char* what);
   int read_counters();
   int stage(const double* V0, const double* V1, int nv, const int* P, long long nidx, const int* active);
-  int update(const double* V0, const double* V1, int nv, const int* P, int np, int pdim, const int* active);
+  // refit: keep the last build's leaf order and Morton keys (so its hierarchy)
+  // and recompute only the boxes; needs the same nv, np and pdim.
+  int update(const double* V0, const double* V1, int nv, const int* P, int np, int pdim, const int* active,
+             bool refit = false);
   int traverse(const double* V0, const double* V1, const int* P, unsigned flags);
   FaceArrays faces() { return face_arrays(d_face, (size_t)np, d_v1); }
 };

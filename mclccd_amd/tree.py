@@ -104,7 +104,7 @@ class BVHTree:
         h = C.c_void_p()
         rc = lib.mclg_create(dim, C.byref(h))
         if rc != _lib.OK:
-            raise _lib.MclgError(rc, "mclg_create failed (no HIP device?)")
+            raise _lib.MclgError(rc, lib.mclg_last_error(None).decode())
         self._h = h
         self.nv = 0
         self.np = 0
@@ -132,10 +132,15 @@ class BVHTree:
         self.options.from_c(o)
 
     # -- reference API ------------------------------------------------------
-    def update(self, V0, V1, P, active=None):
+    def update(self, V0, V1, P, active=None, refit=False):
         """BVHTree::update (BVHTree.hpp:73-82): builds the BVH of P's primitives
         over the swept boxes of V0 -> V1.  Mismatched row counts return silently
-        as in the reference (BVHTree.hpp:160)."""
+        as in the reference (BVHTree.hpp:160).
+
+        refit=True (no reference counterpart; mclg_update_refit): keeps the last
+        full update's leaf order and hierarchy and recomputes only the boxes;
+        same P, vertex and primitive counts required.  Results equal a rebuild's;
+        the kept hierarchy only changes how fast traverse runs."""
         if len(V0) != len(V1):
             return
         self._push_options()
@@ -148,7 +153,8 @@ class BVHTree:
                 if len(active) != len(V0):
                     active = None
             nv, npr, pdim = int(V0.shape[0]), int(P.shape[0]), int(P.shape[1]) if P.ndim == 2 else 1
-            rc = lib.mclg_update_device(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
+            fn = lib.mclg_update_refit_device if refit else lib.mclg_update_device
+            rc = fn(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
         else:
             V0, V1, P = _host_f64(V0), _host_f64(V1), _host_i32(P)
             if P.ndim == 1:
@@ -158,7 +164,8 @@ class BVHTree:
                 if len(active) != len(V0):
                     active = None
             nv, npr, pdim = V0.shape[0], P.shape[0], P.shape[1]
-            rc = lib.mclg_update(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
+            fn = lib.mclg_update_refit if refit else lib.mclg_update
+            rc = fn(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
         self._check(rc)
         self._pull_options()
         self.nv, self.np = nv, npr

@@ -5,7 +5,9 @@
 //   in.bin: int32 nv, int32 nf, double eta, V0[nv*3], V1[nv*3], int32 F[nf*3]
 #include <MCL/BVHTree.hpp>
 
+#include <algorithm>
 #include <cstdio>
+#include <map>
 #include <set>
 #include <vector>
 
@@ -153,6 +155,50 @@ int main(int argc, char** argv) {
   StopFirst sf;
   tree.traverse(&sf);
   std::fprintf(out, "visit_stop %ld\n", sf.objects);
+  // 7. the float instantiation: vertices rounded to float, then the float tree
+  // vs the double tree on the same (widened) coordinates
+  {
+    std::vector<float> F0(V0.begin(), V0.end()), F1(V1.begin(), V1.end());
+    std::vector<double> D0(F0.begin(), F0.end()), D1(F1.begin(), F1.end());
+    mcl::BVHTree<float, 3> tf;
+    mcl::BVHTree<double, 3> td;
+    tf.options.box_eta = tf.options.vf_ccd_eta = tf.options.ee_ccd_eta = (float)eta;
+    td.options.box_eta = td.options.vf_ccd_eta = td.options.ee_ccd_eta = (double)(float)eta;
+    std::map<std::vector<int>, std::vector<double>> hf, hd;
+    tf.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const float& toi) {
+      hf[{s[0], s[1], s[2], s[3], (int)is_vf}].push_back(toi);
+    };
+    td.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const double& toi) {
+      hd[{s[0], s[1], s[2], s[3], (int)is_vf}].push_back((double)(float)toi);
+    };
+    tf.update(F0.data(), F1.data(), F.data(), nf, 3);
+    td.update(D0.data(), D1.data(), F.data(), nf, 3);
+    tf.traverse(F0.data(), F1.data(), F.data());
+    td.traverse(D0.data(), D1.data(), F.data());
+    for (auto& kv : hf) std::sort(kv.second.begin(), kv.second.end());
+    for (auto& kv : hd) std::sort(kv.second.begin(), kv.second.end());
+    long nf_hits = 0;
+    for (auto& kv : hf) nf_hits += (long)kv.second.size();
+    // leaf boxes equal the float arithmetic of the reference's float tree
+    // (BVHTree.cpp:83-124 with T = float): min/max of float vertices, padded in float
+    tf.sync_leaves();
+    long box_bad = 0;
+    const float ef = (float)eta;
+    for (size_t i = 0; i < tf.leaves.size(); ++i)
+      for (int k = 0; k < 3; ++k) {
+        float lo = 3.0e38f, hi = -3.0e38f, hi1 = -3.0e38f;
+        for (int j = 0; j < 3; ++j) {
+          const int v = F[i * 3 + j];
+          lo = std::min(lo, std::min(F0[(size_t)v * 3 + k], F1[(size_t)v * 3 + k]));
+          hi = std::max(hi, std::max(F0[(size_t)v * 3 + k], F1[(size_t)v * 3 + k]));
+          hi1 = std::max(hi1, F1[(size_t)v * 3 + k]);
+        }
+        box_bad += tf.leaves[i].box.min()[k] != lo - ef;
+        box_bad += tf.leaves[i].box.max()[k] != hi + ef;
+        box_bad += tf.leaves[i].box.t1.max()[k] != hi1 + ef;
+      }
+    std::fprintf(out, "float %ld %d %ld %g\n", nf_hits, (int)(hf == hd), box_bad, (double)tf.options.vf_ccd_eta);
+  }
   std::fclose(out);
   return 0;
 }

@@ -73,3 +73,20 @@ def test_create_without_gpu_fails_cleanly():
     rc = _lib.lib.mclg_create(3, C.byref(h))
     assert rc != 0 and not h.value
     assert _lib.lib.mclg_create(4, C.byref(h)) == _lib.ERR_ARG
+
+
+def test_one_hip_runtime_per_process():
+    """Loading the library must not bring a second HIP runtime next to torch's
+    (the first runtime to initialise would hold the GPU and the other would
+    see no device).  Checked in a fresh interpreter that loads the library
+    before anything imports torch."""
+    import subprocess
+    import sys
+
+    code = ("from mclccd_amd import _lib\n"
+            "import torch\n"
+            "libs = {l.split()[-1] for l in open('/proc/self/maps') if 'libamdhip64' in l}\n"
+            "print(len(libs))\n")
+    r = subprocess.run([sys.executable, "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines()[-1] == "1"

@@ -70,3 +70,7 @@ def test_facade_callbacks(gpu, tmp_path):
     npts, mism, found = rec["visit_points"]
     assert npts > 0 and mism == 0 and found >= npts  # each point lies in its own triangles' boxes
     assert rec["visit_stop"][0] == 1
+    # the float tree: the double tree's hits on the same (float) coordinates,
+    # TOIs rounded to float; leaf boxes equal to float arithmetic
+    n_float_hits, same, box_bad, eta_f = rec["float"]
+    assert n_float_hits > 0 and same == 1 and box_bad == 0 and eta_f == np.float32(eta)

@@ -54,3 +54,27 @@ def test_cathead_uv_fold(gpu):
     t, o = run_both(U0, U1, E, 1e-3, True)
     assert o.count(2) > 100
     check_parity(t, o, True)
+
+
+def test_cli_hand(gpu, tmp_path):
+    """The headless example (python -m mclccd_amd.cli) on hand.obj: the
+    reference example's report (dillo.cpp:32-80) against the oracle."""
+    import io
+
+    from mclccd_amd.cli import run
+    from oracle.oracle import run_scene
+
+    buf = io.StringIO()
+    r = run(os.path.join(GOLD, "hand.obj.gz"), out=str(tmp_path / "hand"), stream=buf)
+    V, F = hand()
+    eps = float(np.finfo(np.float32).eps)
+    o = run_scene(V, V.copy(), F, eps)
+    faces = np.unique(o.discrete().reshape(-1))
+    assert r["faces"] == len(F) and r["verts"] == len(V)
+    assert r["ccd_hits"] == 0 and r["discrete_pairs"] == o.count(3) > 0
+    assert r["discrete_faces"] == len(faces)
+    assert np.array_equal(np.loadtxt(tmp_path / "hand_discrete.txt", dtype=np.int64), faces)
+    # one box per internal volume below the root, 12 edges each
+    assert r["box_edges"] == 12 * (len(F) - 2)
+    text = buf.getvalue()
+    assert f"Num discrete isect: {len(faces)}" in text and "Num ccd isect: 0" in text
