@@ -55,6 +55,40 @@ def profiled_traffic(kernel):
     return None
 
 
+# FP64 VALU counters per launch (rocprofv3 --pmc SQ_INSTS_VALU_*_F64 ...) on
+# this workload; the summary (with the command) is the file below.
+FP64_FILE = os.path.join(ROOT, "profiles", "stacked_layers_1M_pmc_fp64_valu.md")
+CUS, CLOCK_HZ = 256, 2.4e9  # MI355X: one wave64 FP64 VALU instruction per CU per cycle at peak
+
+
+def profiled_fp64(kernel, ms):
+    """FP64 VALU use of one ACCD launch from the committed PMC summary, scaled by
+    this run's kernel time: VALU issue fraction, FP64 TFLOP/s (FMA = 2), lane use."""
+    try:
+        with open(FP64_FILE) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if not line.startswith("fp64_valu ") or ":" not in line:
+                    continue
+                k, rest = line[len("fp64_valu "):].rsplit(":", 1)
+                if k.strip() != kernel:
+                    continue
+                v = dict(kv.split("=") for kv in rest.split())
+                v = {a: float(b) for a, b in v.items()}
+                t = ms * 1e-3
+                tf = v["f64_flop_insts"] * 64 * v["lane_use"] / t / 1e12
+                return {"valu_issue_frac": round(v["valu_insts"] / (CUS * CLOCK_HZ * t), 3),
+                        "fp64_share_of_valu": round(v["f64_insts"] / v["valu_insts"], 3),
+                        "lane_use": v["lane_use"], "fp64_tflops": round(tf, 2),
+                        "fp64_frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                        "note": "hardware FP64 instruction counts (division and sqrt sequences included) x 64 x "
+                                "lane use / this run's kernel time; issue fraction against one wave64 VALU "
+                                "instruction per CU per cycle"}
+    except (OSError, ValueError, KeyError):
+        pass
+    return None
+
+
 def roofline(c, avg, n, nv):
     """Roofline of the longest single kernel of the step (HIP-event times)."""
     kern = {"traverse": avg["traverse"], "accd_vf": avg["accd_vf"], "accd_ee": avg["accd_ee"],
@@ -90,6 +124,7 @@ def roofline(c, avg, n, nv):
             "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kernel),
             "algorithmic_flops": flops, "stencils": cand, "dist_evals": evals, "ms": round(ms, 3),
+            "pmc": profiled_fp64(kernel, ms),
             "note": f"algorithmic flops = {FLOPS_ACCD_SETUP}*stencils + {FLOPS_PER_EVAL}*distance evaluations, "
                     "div/sqrt = 1 flop; FP64 vector peak 78.6 TFLOP/s counts FMA as 2 and parity forbids "
                     "contraction, so the usable rate is 39.3 T ops/s"}
