@@ -19,6 +19,12 @@
 // is computed on the host (kernel argument) so it is the host libm's value.
 // Two loops of the root-finder's bound estimate stop after a cap that only
 // non-finite coefficients reach (the reference would not terminate there).
+//
+// GPU structure: a lane walks its stencil's sub-tests one polynomial per pass
+// of a single loop, so lanes of a wave in different sub-tests still share the
+// root-finder call; the root finder's arrays are indexed only by compile-time
+// constants (guarded loops over the maximum degree, select chains for a
+// runtime index), so its state is kept in registers rather than scratch.
 #pragma once
 #include "geom.hpp"
 
@@ -32,15 +38,31 @@ __device__ __forceinline__ d3 cross3(const d3& a, const d3& b) {
 }
 __device__ __forceinline__ d3 add3(const d3& a, const d3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 
-// Sign intervals of one polynomial over [0, 1] (TimeInterval, CTCD.hpp:10-28).
+// Small per-lane arrays stay in registers only if every access has a
+// compile-time index: loops run over the fixed maximum size with a guard, and
+// a runtime index goes through a select chain.
+template <int N> __device__ __forceinline__ double at(const double (&x)[N], int i) {
+  double r = x[0];
+#pragma unroll
+  for (int j = 1; j < N; ++j) r = (i == j) ? x[j] : r;
+  return r;
+}
+template <int N> __device__ __forceinline__ void put(double (&x)[N], int i, double v) {
+#pragma unroll
+  for (int j = 0; j < N; ++j)
+    if (i == j) x[j] = v;
+}
+
+// Sign intervals of one polynomial over [0, 1] (TimeInterval, CTCD.hpp:10-28):
+// at most degree+1 <= 7.
 struct Ivals {
   double l[7], u[7];
   int n;
-  __device__ void push(double t1, double t2) {
+  __device__ __forceinline__ void push(double t1, double t2) {
     double a = t1, b = t2;
     if (a > b) { const double s = a; a = b; b = s; }
-    l[n] = smax(a, 0.0);
-    u[n] = smin(b, 1.0);
+    put(l, n, smax(a, 0.0));
+    put(u, n, smin(b, 1.0));
     ++n;
   }
 };
@@ -75,10 +97,38 @@ __device__ __forceinline__ void insertion_sort(double* a, int n) {
   }
 }
 
+// The same sort on the first n (<= 6) of six register values.
+__device__ __forceinline__ void insertion_sort6(double (&a)[6], int n) {
+#pragma unroll
+  for (int i = 1; i < 6; ++i) {
+    if (i >= n) break;
+    const double val = a[i];
+    if (val < a[0]) {
+#pragma unroll
+      for (int j = i; j > 0; --j) a[j] = a[j - 1];
+      a[0] = val;
+    } else {
+      // unguarded: stops at j = 1 at the latest, since !(val < a[0])
+      bool moving = true;
+#pragma unroll
+      for (int j = i; j >= 1; --j) {
+        if (!moving) break;
+        if (val < a[j - 1]) {
+          a[j] = a[j - 1];
+        } else {
+          a[j] = val;
+          moving = false;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Jenkins-Traub (TOMS 493) over degree <= 6, same steps and arithmetic as the
 // reference's C translation (rpoly.h); the goto structure of the fixed-shift
-// stage is a three-state loop.
+// stage is a three-state loop, and every array loop runs to the maximum size
+// under a guard so the state stays in registers.
 // ---------------------------------------------------------------------------
 constexpr double kEta = 2.22e-16, kAre = kEta, kMre = kEta;
 
@@ -88,21 +138,26 @@ struct JT {
   int n;
 
   // rpoly.h:687-700 — p / (1 + u z + v z^2): quotient q, remainder (ra, rb)
-  __device__ static void quadsd(int nn, double uu, double vv, const double* pp, double* q, double& ra, double& rb) {
+  __device__ __forceinline__ static void quadsd(int nn, double uu, double vv, const double (&pp)[7], double (&q)[7],
+                                                double& ra, double& rb) {
     rb = pp[0];
     q[0] = rb;
     ra = pp[1] - rb * uu;
     q[1] = ra;
-    for (int i = 2; i <= nn; i++) {
-      const double cc = pp[i] - ra * uu - rb * vv;
-      q[i] = cc;
-      rb = ra;
-      ra = cc;
+#pragma unroll
+    for (int i = 2; i < 7; i++) {
+      if (i <= nn) {
+        const double cc = pp[i] - ra * uu - rb * vv;
+        q[i] = cc;
+        rb = ra;
+        ra = cc;
+      }
     }
   }
 
   // rpoly.h:706-760
-  __device__ static void quad(double a_, double b1, double c_, double& sr, double& si, double& lr, double& li) {
+  __device__ __forceinline__ static void quad(double a_, double b1, double c_, double& sr, double& si, double& lr,
+                                              double& li) {
     if (a_ == 0.0) {
       sr = (b1 != 0.0) ? -c_ / b1 : 0.0;
       lr = 0.0;
@@ -143,9 +198,9 @@ struct JT {
   }
 
   // rpoly.h:573-603; returns the type
-  __device__ int calcsc() {
+  __device__ __forceinline__ int calcsc() {
     quadsd(n - 1, u, v, k, qk, c, d);
-    if (!(fabs(c) > fabs(k[n - 1] * 100.0 * kEta)) && !(fabs(d) > fabs(k[n - 2] * 100.0 * kEta))) return 3;
+    if (!(fabs(c) > fabs(at(k, n - 1) * 100.0 * kEta)) && !(fabs(d) > fabs(at(k, n - 2) * 100.0 * kEta))) return 3;
     if (fabs(d) < fabs(c)) {
       e = a / c;
       f = d / c;
@@ -167,29 +222,35 @@ struct JT {
   }
 
   // rpoly.h:608-641
-  __device__ void nextk(int type) {
+  __device__ __forceinline__ void nextk(int type) {
     if (type == 3) {
       k[0] = 0.0;
       k[1] = 0.0;
-      for (int i = 2; i < n; i++) k[i] = qk[i - 2];
+#pragma unroll
+      for (int i = 2; i < 7; i++)
+        if (i < n) k[i] = qk[i - 2];
       return;
     }
     const double temp = (type == 1) ? b : a;
     if (fabs(a1) <= fabs(temp) * kEta * 10.0) {
       k[0] = 0.0;
       k[1] = -a7 * qp[0];
-      for (int i = 2; i < n; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
+#pragma unroll
+      for (int i = 2; i < 7; i++)
+        if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1];
       return;
     }
     a7 /= a1;
     a3 /= a1;
     k[0] = qp[0];
     k[1] = qp[1] - a7 * qp[0];
-    for (int i = 2; i < n; i++) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
+#pragma unroll
+    for (int i = 2; i < 7; i++)
+      if (i < n) k[i] = a3 * qk[i - 2] - a7 * qp[i - 1] + qp[i];
   }
 
   // rpoly.h:647-682
-  __device__ void newest(int type, double& uu, double& vv) const {
+  __device__ __forceinline__ void newest(int type, double& uu, double& vv) const {
     if (type == 3) {
       uu = 0.0;
       vv = 0.0;
@@ -203,8 +264,9 @@ struct JT {
       a4 = a + u * b + h * f;
       a5 = c + (u + v * f) * d;
     }
-    const double b1 = -k[n - 1] / p[n];
-    const double b2 = -(k[n - 2] + b1 * p[n - 1]) / p[n];
+    const double pn = at(p, n);
+    const double b1 = -at(k, n - 1) / pn;
+    const double b2 = -(at(k, n - 2) + b1 * at(p, n - 1)) / pn;
     const double c1 = v * b2 * a1;
     const double c2 = b1 * a7;
     const double c3 = b1 * b1 * a3;
@@ -220,7 +282,7 @@ struct JT {
   }
 
   // rpoly.h:397-474; returns the zeros found (0 or 2)
-  __device__ int quadit(double uu, double vv) {
+  __device__ __forceinline__ int quadit(double uu, double vv) {
     double omp = 0, relstp = 0;
     bool tried = false;
     u = uu;
@@ -234,7 +296,9 @@ struct JT {
       const double zm = sqrt(fabs(v));
       double ee = 2.0 * fabs(qp[0]);
       const double t = -szr * b;
-      for (int i = 1; i < n; i++) ee = ee * zm + fabs(qp[i]);
+#pragma unroll
+      for (int i = 1; i < 7; i++)
+        if (i < n) ee = ee * zm + fabs(qp[i]);
       ee = ee * zm + fabs(a + t);
       ee *= (5.0 * kMre + 4.0 * kAre);
       ee = ee - (5.0 * kMre + 2.0 * kAre) * (fabs(a + t) + fabs(b) * zm) + 2.0 * kAre * fabs(t);
@@ -264,7 +328,7 @@ struct JT {
   }
 
   // rpoly.h:480-566; returns the zeros found (0 or 1); iflag: a near-double real zero
-  __device__ int realit(double& sss, int& iflag) {
+  __device__ __forceinline__ int realit(double& sss, int& iflag) {
     double t = 0, omp = 0;
     double s = sss;
     iflag = 0;
@@ -272,14 +336,19 @@ struct JT {
     for (;;) {
       double pv = p[0];
       qp[0] = pv;
-      for (int i = 1; i <= n; i++) {
-        pv = pv * s + p[i];
-        qp[i] = pv;
+#pragma unroll
+      for (int i = 1; i < 7; i++) {
+        if (i <= n) {
+          pv = pv * s + p[i];
+          qp[i] = pv;
+        }
       }
       const double mp = fabs(pv);
       const double ms = fabs(s);
       double ee = (kMre / (kAre + kMre)) * fabs(qp[0]);
-      for (int i = 1; i <= n; i++) ee = ee * ms + fabs(qp[i]);
+#pragma unroll
+      for (int i = 1; i < 7; i++)
+        if (i <= n) ee = ee * ms + fabs(qp[i]);
       if (mp <= 20.0 * ((kAre + kMre) * ee - kMre * mp)) {
         szr = s;
         szi = 0.0;
@@ -295,28 +364,37 @@ struct JT {
       omp = mp;
       double kv = k[0];
       qk[0] = kv;
-      for (int i = 1; i < n; i++) {
-        kv = kv * s + k[i];
-        qk[i] = kv;
+#pragma unroll
+      for (int i = 1; i < 7; i++) {
+        if (i < n) {
+          kv = kv * s + k[i];
+          qk[i] = kv;
+        }
       }
-      if (fabs(kv) <= fabs(k[n - 1]) * 10.0 * kEta) {
+      if (fabs(kv) <= fabs(at(k, n - 1)) * 10.0 * kEta) {
         k[0] = 0.0;
-        for (int i = 1; i < n; i++) k[i] = qk[i - 1];
+#pragma unroll
+        for (int i = 1; i < 7; i++)
+          if (i < n) k[i] = qk[i - 1];
       } else {
         t = -pv / kv;
         k[0] = qp[0];
-        for (int i = 1; i < n; i++) k[i] = t * qk[i - 1] + qp[i];
+#pragma unroll
+        for (int i = 1; i < 7; i++)
+          if (i < n) k[i] = t * qk[i - 1] + qp[i];
       }
       kv = k[0];
-      for (int i = 1; i < n; i++) kv = kv * s + k[i];
+#pragma unroll
+      for (int i = 1; i < 7; i++)
+        if (i < n) kv = kv * s + k[i];
       t = 0.0;
-      if (fabs(kv) > fabs(k[n - 1] * 10.0 * kEta)) t = -pv / kv;
+      if (fabs(kv) > fabs(at(k, n - 1) * 10.0 * kEta)) t = -pv / kv;
       s += t;
     }
   }
 
   // rpoly.h:284-390 with the shift (sr, v); returns the zeros found
-  __device__ int fxshfr(int l2, double sr) {
+  __device__ __forceinline__ int fxshfr(int l2, double sr) {
     double betav = 0.25, betas = 0.25;
     double oss = sr, ovv = v, ots = 0, otv = 0;
     quadsd(n, u, v, p, qp, a, b);
@@ -328,7 +406,8 @@ struct JT {
       newest(type, ui, vi);
       const double vv = vi;
       double ss = 0.0;
-      if (k[n - 1] != 0.0) ss = -p[n] / k[n - 1];
+      const double kn1 = at(k, n - 1);
+      if (kn1 != 0.0) ss = -at(p, n) / kn1;
       double tv = 1.0, ts = 1.0;
       if (!(j == 0 || type == 3)) {
         if (vv != 0.0) tv = fabs((vv - ovv) / vv);
@@ -338,7 +417,9 @@ struct JT {
         const bool vpass = tvv < betav, spass = tss < betas;
         if (spass || vpass) {
           const double svu = u, svv = v;
-          for (int i = 0; i < n; i++) svk[i] = k[i];
+#pragma unroll
+          for (int i = 0; i < 7; i++)
+            if (i < n) svk[i] = k[i];
           double s = ss;
           bool vtry = false, stry = false;
           // 0: quadratic iteration, 1: linear iteration, 2: restore and go on
@@ -350,7 +431,9 @@ struct JT {
               vtry = true;
               betav *= 0.25;
               if (stry || !spass) { st = 2; continue; }
-              for (int i = 0; i < n; i++) k[i] = svk[i];
+#pragma unroll
+              for (int i = 0; i < 7; i++)
+                if (i < n) k[i] = svk[i];
               st = 1;
               continue;
             }
@@ -368,7 +451,9 @@ struct JT {
             }
             u = svu;
             v = svv;
-            for (int i = 0; i < n; i++) k[i] = svk[i];
+#pragma unroll
+            for (int i = 0; i < 7; i++)
+              if (i < n) k[i] = svk[i];
             if (vpass && !vtry) { st = 0; continue; }
             quadsd(n, u, v, p, qp, a, b);
             type = calcsc();
@@ -385,35 +470,42 @@ struct JT {
   }
 
   // rpoly.h:56-268: zeros (real parts in zr) of op[0..degree], or -1
-  __device__ __noinline__ int solve(const double* op, int degree, double* zr, double cosr, double sinr) {
+  __device__ __forceinline__ int solve(const double (&op)[7], int degree, double (&zr)[6], double cosr, double sinr) {
     const double infin = 3.4e38, smalno = 1.2e-38, lo = smalno / kEta;
     double xx = sqrt(0.5), yy = -xx;
     n = degree;
     if (op[0] == 0.0) return -1;
-    while (op[n] == 0.0) {
-      zr[degree - n] = 0.0;
+    while (at(op, n) == 0.0) {
+      put(zr, degree - n, 0.0);
       n--;
     }
     if (n < 1) return -1;
     double temp[7], pt[7];
-    for (int i = 0; i <= n; i++) p[i] = op[i];
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+      if (i <= n) p[i] = op[i];
     for (;;) {
       if (n == 1) {
-        zr[degree - 1] = -p[1] / p[0];
+        put(zr, degree - 1, -p[1] / p[0]);
         n -= 1;
         break;
       }
       if (n == 2) {
-        double si, li;
-        quad(p[0], p[1], p[2], zr[degree - 2], si, zr[degree - 1], li);
+        double sr, si, lr, li;
+        quad(p[0], p[1], p[2], sr, si, lr, li);
+        put(zr, degree - 2, sr);
+        put(zr, degree - 1, lr);
         n -= 2;
         break;
       }
       double mx = 0.0, mnv = infin;
-      for (int i = 0; i <= n; i++) {
-        const double x = fabs(p[i]);
-        if (x > mx) mx = x;
-        if (x != 0.0 && x < mnv) mnv = x;
+#pragma unroll
+      for (int i = 0; i < 7; i++) {
+        if (i <= n) {
+          const double x = fabs(p[i]);
+          if (x > mx) mx = x;
+          if (x != 0.0 && x < mnv) mnv = x;
+        }
       }
       double sc = lo / mnv;
       bool scale = !(sc > 1.0 && infin / sc < mx);
@@ -424,60 +516,73 @@ struct JT {
       if (scale) {
         const int l = (int)(log(sc) / log(2.0) + 0.5);
         const double factor = pow(2.0, (double)l);
-        if (factor != 1.0)
-          for (int i = 0; i <= n; i++) p[i] = factor * p[i];
+        if (factor != 1.0) {
+#pragma unroll
+          for (int i = 0; i < 7; i++)
+            if (i <= n) p[i] = factor * p[i];
+        }
       }
-      for (int i = 0; i <= n; i++) pt[i] = fabs(p[i]);
-      pt[n] = -pt[n];
-      double x = exp((log(-pt[n]) - log(pt[0])) / (double)n);
-      if (pt[n - 1] != 0.0) {
-        const double xm = -pt[n] / pt[n - 1];
+#pragma unroll
+      for (int i = 0; i < 7; i++)
+        if (i <= n) pt[i] = fabs(p[i]);
+      const double ptn = -at(pt, n);
+      put(pt, n, ptn);
+      double x = exp((log(-ptn) - log(pt[0])) / (double)n);
+      const double ptn1 = at(pt, n - 1);
+      if (ptn1 != 0.0) {
+        const double xm = -ptn / ptn1;
         if (xm < x) x = xm;
       }
       for (int it = 0; it < 2000; ++it) {
         const double xm = x * 0.1;
         double ff = pt[0];
-        for (int i = 1; i <= n; i++) ff = ff * xm + pt[i];
+#pragma unroll
+        for (int i = 1; i < 7; i++)
+          if (i <= n) ff = ff * xm + pt[i];
         if (ff <= 0.0) break;
         x = xm;
       }
       double dx = x;
       for (int it = 0; it < 1000 && fabs(dx / x) > 0.005; ++it) {
         double ff = pt[0], df = ff;
-        for (int i = 1; i < n; i++) {
-          ff = ff * x + pt[i];
-          df = df * x + ff;
+#pragma unroll
+        for (int i = 1; i < 7; i++) {
+          if (i < n) {
+            ff = ff * x + pt[i];
+            df = df * x + ff;
+          }
         }
-        ff = ff * x + pt[n];
+        ff = ff * x + ptn;
         dx = ff / df;
         x -= dx;
       }
       const double bnd = x;
-      const int nm1 = n - 1;
-      for (int i = 1; i < n; i++) k[i] = (double)(n - i) * p[i] / (double)n;
+#pragma unroll
+      for (int i = 1; i < 7; i++)
+        if (i < n) k[i] = (double)(n - i) * p[i] / (double)n;
       k[0] = p[0];
-      const double aa = p[n], bb = p[n - 1];
-      bool zerok = (k[n - 1] == 0);
+      const double aa = at(p, n), bb = at(p, n - 1);
+      bool zerok = (at(k, n - 1) == 0);
       for (int jj = 0; jj < 5; jj++) {
-        const double cc = k[n - 1];
+        const double cc = at(k, n - 1);
         if (!zerok) {
           const double t = -aa / cc;
-          for (int i = 0; i < nm1; i++) {
-            const int j = n - i - 1;
-            k[j] = t * k[j - 1] + p[j];
-          }
+#pragma unroll
+          for (int j = 6; j >= 1; --j)
+            if (j <= n - 1) k[j] = t * k[j - 1] + p[j];
           k[0] = p[0];
-          zerok = (fabs(k[n - 1]) <= fabs(bb) * kEta * 10.0);
+          zerok = (fabs(at(k, n - 1)) <= fabs(bb) * kEta * 10.0);
         } else {
-          for (int i = 0; i < nm1; i++) {
-            const int j = n - i - 1;
-            k[j] = k[j - 1];
-          }
+#pragma unroll
+          for (int j = 6; j >= 1; --j)
+            if (j <= n - 1) k[j] = k[j - 1];
           k[0] = 0.0;
-          zerok = (k[n - 1] == 0.0);
+          zerok = (at(k, n - 1) == 0.0);
         }
       }
-      for (int i = 0; i < n; i++) temp[i] = k[i];
+#pragma unroll
+      for (int i = 0; i < 7; i++)
+        if (i < n) temp[i] = k[i];
       bool found = false;
       for (int cnt = 0; cnt < 20; cnt++) {
         const double xxx = cosr * xx - sinr * yy;
@@ -489,14 +594,18 @@ struct JT {
         const int nz = fxshfr(20 * (cnt + 1), sr);
         if (nz != 0) {
           const int j = degree - n;
-          zr[j] = szr;
+          put(zr, j, szr);
           n -= nz;
-          for (int i = 0; i <= n; i++) p[i] = qp[i];
-          if (nz != 1) zr[j + 1] = lzr;
+#pragma unroll
+          for (int i = 0; i < 7; i++)
+            if (i <= n) p[i] = qp[i];
+          if (nz != 1) put(zr, j + 1, lzr);
           found = true;
           break;
         }
-        for (int i = 0; i < n; i++) k[i] = temp[i];
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+          if (i < n) k[i] = temp[i];
       }
       if (!found) break;
     }
@@ -516,8 +625,9 @@ __device__ __forceinline__ int quad_roots(double a, double b, double c, double& 
   return 2;
 }
 
-// CTCD.cpp:66-86
-__device__ __forceinline__ void check_interval(double t1, double t2, const double* op, int degree, Ivals& out,
+// CTCD.cpp:66-86 (op: degree+1 coefficients, decreasing powers)
+template <int N>
+__device__ __forceinline__ void check_interval(double t1, double t2, const double (&op)[N], int degree, Ivals& out,
                                                bool pos) {
   t1 = smax(0.0, t1);
   t2 = smax(0.0, t2);
@@ -525,41 +635,61 @@ __device__ __forceinline__ void check_interval(double t1, double t2, const doubl
   t2 = smin(1.0, t2);
   const double tmid = (t2 + t1) / 2;
   double f = op[0];
-  for (int i = 1; i <= degree; i++) {
-    f *= tmid;
-    f += op[i];
+#pragma unroll
+  for (int i = 1; i < N; i++) {
+    if (i <= degree) {
+      f *= tmid;
+      f += op[i];
+    }
   }
   if (pos ? (f >= 0) : (f <= 0)) out.push(t1, t2);
 }
 
 // CTCD.cpp:88-103
-__device__ __forceinline__ bool could_have_roots(const double* op, int degree, bool pos) {
+__device__ __forceinline__ bool could_have_roots(const double (&op)[7], int degree, bool pos) {
   double result = 0;
   if (pos ? op[0] > 0 : op[0] < 0) result = op[0];
-  for (int i = 1; i < degree; i++) {
-    result *= 1.0;
-    if (pos ? op[i] > 0 : op[i] < 0) result += op[i];
+#pragma unroll
+  for (int i = 1; i < 7; i++) {
+    if (i < degree) {
+      result *= 1.0;
+      if (pos ? op[i] > 0 : op[i] < 0) result += op[i];
+    }
   }
   result *= 1.0;
-  result += op[degree];
+  result += at(op, degree);
   return !(pos ? result < 0 : result > 0);
 }
 
 // CTCD.cpp:105-184; op (n+1 coefficients, decreasing powers) is normalised in place.
-__device__ __noinline__ void find_intervals(double* op, int n, Ivals& out, bool pos, double cosr, double sinr) {
+__device__ __forceinline__ void find_intervals(double (&op)[7], int n, Ivals& out, bool pos, double cosr,
+                                               double sinr) {
   out.n = 0;
   double time[6];
   double maxval = 0;
-  for (int i = 0; i <= n; i++) maxval = smax(maxval, fabs(op[i]));
-  if (maxval != 0)
-    for (int i = 0; i <= n; i++) op[i] /= maxval;
-  int rd = n;
-  for (int i = 0; i < n; i++) {
-    if (op[i] == 0) rd--;
-    else break;
+#pragma unroll
+  for (int i = 0; i < 7; i++)
+    if (i <= n) maxval = smax(maxval, fabs(op[i]));
+  if (maxval != 0) {
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+      if (i <= n) op[i] /= maxval;
   }
-  if (rd < n)
-    for (int i = 0; i <= rd; i++) op[i] = op[i + n - rd];
+  int rd = n;
+  bool lead = true;
+#pragma unroll
+  for (int i = 0; i < 6; i++) {
+    if (i < n && lead) {
+      if (op[i] == 0) rd--;
+      else lead = false;
+    }
+  }
+  if (rd < n) {
+    const int sh = n - rd;
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+      if (i <= rd) op[i] = at(op, i + sh);
+  }
   int roots;
   if (rd > 2) {
     if (!could_have_roots(op, rd, pos)) return;
@@ -575,32 +705,31 @@ __device__ __noinline__ void find_intervals(double* op, int n, Ivals& out, bool 
     return;
   }
   if (roots > 0) {
-    insertion_sort(time, roots);
+    insertion_sort6(time, roots);
     if (time[0] >= 0) check_interval(0, time[0], op, rd, out, pos);
-    for (int i = 0; i < roots - 1; i++)
-      if (!((time[i] < 0 && time[i + 1] < 0) || (time[i] > 1.0 && time[i + 1] > 1.0)))
+#pragma unroll
+    for (int i = 0; i < 5; i++)
+      if (i < roots - 1 && !((time[i] < 0 && time[i + 1] < 0) || (time[i] > 1.0 && time[i + 1] > 1.0)))
         check_interval(time[i], time[i + 1], op, rd, out, pos);
-    if (time[roots - 1] <= 1.0) check_interval(time[roots - 1], 1.0, op, rd, out, pos);
+    const double last = at(time, roots - 1);
+    if (last <= 1.0) check_interval(last, 1.0, op, rd, out, pos);
   } else {
     check_interval(0.0, 1.0, op, rd, out, pos);
   }
 }
 
-// CTCD.cpp:222-236
+// CTCD.cpp:222-236 (op[0..3])
 __device__ __forceinline__ void plane_poly(const d3& x10, const d3& x20, const d3& x30, const d3& v10, const d3& v20,
-                                           const d3& v30, Ivals& out, double cosr, double sinr) {
-  double op[4];
+                                           const d3& v30, double (&op)[7]) {
   op[0] = dot(v10, cross3(v20, v30));
   op[1] = dot(x10, cross3(v20, v30)) + dot(v10, cross3(x20, v30)) + dot(v10, cross3(v20, x30));
   op[2] = dot(x10, cross3(x20, v30)) + dot(x10, cross3(v20, x30)) + dot(v10, cross3(x20, x30));
   op[3] = dot(x10, cross3(x20, x30));
-  find_intervals(op, 3, out, true, cosr, sinr);
 }
 
-// CTCD.cpp:238-264
+// CTCD.cpp:238-264 (op[0..6])
 __device__ __forceinline__ void distance_poly(const d3& x10, const d3& x20, const d3& x30, const d3& v10,
-                                              const d3& v20, const d3& v30, double minD2, Ivals& out, double cosr,
-                                              double sinr) {
+                                              const d3& v20, const d3& v30, double minD2, double (&op)[7]) {
   const double A = dot(v10, cross3(v20, v30));
   const double B = dot(x10, cross3(v20, v30)) + dot(v10, cross3(x20, v30)) + dot(v10, cross3(v20, x30));
   const double C = dot(x10, cross3(x20, v30)) + dot(x10, cross3(v20, x30)) + dot(v10, cross3(x20, x30));
@@ -608,7 +737,6 @@ __device__ __forceinline__ void distance_poly(const d3& x10, const d3& x20, cons
   const d3 E = cross3(x20, x30);
   const d3 F = add3(cross3(x20, v30), cross3(v20, x30));
   const d3 G = cross3(v20, v30);
-  double op[7];
   op[0] = A * A;
   op[1] = 2 * A * B;
   op[2] = B * B + 2 * A * C - dot(G, G) * minD2;
@@ -616,70 +744,31 @@ __device__ __forceinline__ void distance_poly(const d3& x10, const d3& x20, cons
   op[4] = 2 * B * D + C * C - (2 * dot(G, E) + dot(F, F)) * minD2;
   op[5] = 2 * C * D - 2 * dot(F, E) * minD2;
   op[6] = D * D - dot(E, E) * minD2;
-  find_intervals(op, 6, out, false, cosr, sinr);
 }
 
 __device__ __forceinline__ bool overlap(double al, double au, double bl, double bu) { return !(al > bu || bl > au); }
 
-// CTCD.cpp:435-534; q[0] the vertex, q[1..3] the face
-__device__ __noinline__ bool vertex_face(const d3* q0, const d3* q1, double eta, Times& T, double cosr, double sinr) {
-  const double minD = eta * eta;
-  const d3 v0 = sub(q1[0], q0[0]), v1 = sub(q1[1], q0[1]), v2 = sub(q1[2], q0[2]), v3 = sub(q1[3], q0[3]);
-  Ivals co, e1, e2, e3;
-  plane_poly(sub(q0[0], q0[1]), cross3(sub(q0[3], q0[1]), sub(q0[2], q0[1])), sub(q0[3], q0[1]), sub(v0, v1),
-             cross3(sub(v3, v1), sub(v2, v1)), sub(v3, v1), e1, cosr, sinr);
-  if (e1.n == 0) return false;
-  plane_poly(sub(q0[0], q0[2]), cross3(sub(q0[1], q0[2]), sub(q0[3], q0[2])), sub(q0[1], q0[2]), sub(v0, v2),
-             cross3(sub(v1, v2), sub(v3, v2)), sub(v1, v2), e2, cosr, sinr);
-  if (e2.n == 0) return false;
-  plane_poly(sub(q0[0], q0[3]), cross3(sub(q0[2], q0[3]), sub(q0[1], q0[3])), sub(q0[2], q0[3]), sub(v0, v3),
-             cross3(sub(v2, v3), sub(v1, v3)), sub(v2, v3), e3, cosr, sinr);
-  if (e3.n == 0) return false;
-  distance_poly(sub(q0[0], q0[1]), sub(q0[2], q0[1]), sub(q0[3], q0[1]), sub(v0, v1), sub(v2, v1), sub(v3, v1), minD,
-                co, cosr, sinr);
-  if (co.n == 0) return false;
-  bool col = false;
-  for (int i = 0; i < co.n; i++)
-    for (int j = 0; j < e1.n; j++) {
-      if (!overlap(co.l[i], co.u[i], e1.l[j], e1.u[j])) continue;
-      for (int k = 0; k < e2.n; k++) {
-        if (!overlap(co.l[i], co.u[i], e2.l[k], e2.u[k]) || !overlap(e1.l[j], e1.u[j], e2.l[k], e2.u[k])) continue;
-        for (int l = 0; l < e3.n; l++) {
-          if (!overlap(co.l[i], co.u[i], e3.l[l], e3.u[l]) || !overlap(e1.l[j], e1.u[j], e3.l[l], e3.u[l]) ||
-              !overlap(e2.l[k], e2.u[k], e3.l[l], e3.u[l]))
-            continue;
-          // TimeInterval::intersect from (0, 1), in list order
-          double lo = smax(co.l[i], 0.0);
-          lo = smax(e1.l[j], lo);
-          lo = smax(e2.l[k], lo);
-          lo = smax(e3.l[l], lo);
-          T.insert_sorted(lo);
-          col = true;
-        }
-      }
-    }
-  return col;
-}
-
-// CTCD.cpp:536-634; vertex q0, edge (q1, q2)
-__device__ __noinline__ bool vertex_edge3(const d3& q0s, const d3& q1s, const d3& q2s, const d3& q0e, const d3& q1e,
-                                          const d3& q2e, double eta, Times& T, double cosr, double sinr) {
-  double op[5];
-  const double minD = eta * eta;
+// Polynomial `part` of vertex-edge sub-test (vertex q0, edge q1-q2), CTCD.cpp:536-607:
+// 0 and 1 the two projection quadratics (>= 0), 2 the colinearity quartic (<= 0).
+__device__ __forceinline__ int vertex_edge_poly(int part, const d3& q0s, const d3& q1s, const d3& q2s, const d3& q0e,
+                                                const d3& q1e, const d3& q2e, double minD, double (&op)[7], bool& pos) {
   const d3 v0 = sub(q0e, q0s), v1 = sub(q1e, q1s), v2 = sub(q2e, q2s);
   const d3 ab = sub(q2s, q1s), ac = sub(q0s, q1s), cb = sub(q2s, q0s);
   const d3 vab = sub(v2, v1), vac = sub(v0, v1), vcb = sub(v2, v0);
-  Ivals e1, e2, co;
-  op[0] = dot(vab, vac);
-  op[1] = dot(ac, vab) + dot(ab, vac);
-  op[2] = dot(ab, ac);
-  find_intervals(op, 2, e1, true, cosr, sinr);
-  if (e1.n == 0) return false;
-  op[0] = dot(vab, vcb);
-  op[1] = dot(cb, vab) + dot(ab, vcb);
-  op[2] = dot(ab, cb);
-  find_intervals(op, 2, e2, true, cosr, sinr);
-  if (e2.n == 0) return false;
+  if (part == 0) {
+    op[0] = dot(vab, vac);
+    op[1] = dot(ac, vab) + dot(ab, vac);
+    op[2] = dot(ab, ac);
+    pos = true;
+    return 2;
+  }
+  if (part == 1) {
+    op[0] = dot(vab, vcb);
+    op[1] = dot(cb, vab) + dot(ab, vcb);
+    op[2] = dot(ab, cb);
+    pos = true;
+    return 2;
+  }
   const double A = dot(ab, ab), B = 2 * dot(ab, vab), C = dot(vab, vab);
   const double D = dot(ac, ac), E = 2 * dot(ac, vac), F = dot(vac, vac);
   const double G = dot(ac, ab), H = dot(vab, ac) + dot(vac, ab), I = dot(vab, vac);
@@ -688,22 +777,8 @@ __device__ __noinline__ bool vertex_edge3(const d3& q0s, const d3& q1s, const d3
   op[2] = B * E + A * F + C * D - H * H - 2 * G * I - minD * C;
   op[1] = B * F + C * E - 2 * H * I;
   op[0] = C * F - I * I;
-  find_intervals(op, 4, co, false, cosr, sinr);
-  if (co.n == 0) return false;
-  bool col = false;
-  for (int i = 0; i < co.n; i++)
-    for (int j = 0; j < e1.n; j++) {
-      if (!overlap(co.l[i], co.u[i], e1.l[j], e1.u[j])) continue;
-      for (int k = 0; k < e2.n; k++) {
-        if (!overlap(co.l[i], co.u[i], e2.l[k], e2.u[k]) || !overlap(e1.l[j], e1.u[j], e2.l[k], e2.u[k])) continue;
-        double lo = smax(co.l[i], 0.0);
-        lo = smax(e1.l[j], lo);
-        lo = smax(e2.l[k], lo);
-        T.insert_sorted(lo);
-        col = true;
-      }
-    }
-  return col;
+  pos = false;
+  return 4;
 }
 
 // CTCD.cpp:715-817 with the candidate list (its values appended in order;
@@ -925,9 +1000,44 @@ __device__ __forceinline__ bool vertex_edge2(const d2* q0, const d2* q1, Times& 
 template <int D> struct Query;
 
 template <> struct Query<3> {
+  // Combines the sign intervals of a sub-test (CTCD.cpp:506-523 / 608-625): the
+  // start of every overlapping combination, TimeInterval::intersect from (0, 1)
+  // in list order (the distance/colinearity list first).
+  __device__ static bool combine(const Ivals* iv, bool vf, Times& T) {
+    const Ivals& co = iv[vf ? 3 : 2];
+    const Ivals& e1 = iv[0];
+    const Ivals& e2 = iv[1];
+    bool col = false;
+    for (int i = 0; i < co.n; i++)
+      for (int j = 0; j < e1.n; j++) {
+        if (!overlap(co.l[i], co.u[i], e1.l[j], e1.u[j])) continue;
+        for (int k = 0; k < e2.n; k++) {
+          if (!overlap(co.l[i], co.u[i], e2.l[k], e2.u[k]) || !overlap(e1.l[j], e1.u[j], e2.l[k], e2.u[k])) continue;
+          double lo = smax(co.l[i], 0.0);
+          lo = smax(e1.l[j], lo);
+          lo = smax(e2.l[k], lo);
+          if (!vf) {
+            T.insert_sorted(lo);
+            col = true;
+            continue;
+          }
+          const Ivals& e3 = iv[2];
+          for (int l = 0; l < e3.n; l++) {
+            if (!overlap(co.l[i], co.u[i], e3.l[l], e3.u[l]) || !overlap(e1.l[j], e1.u[j], e3.l[l], e3.u[l]) ||
+                !overlap(e2.l[k], e2.u[k], e3.l[l], e3.u[l]))
+              continue;
+            T.insert_sorted(smax(e3.l[l], lo));
+            col = true;
+          }
+        }
+      }
+    return col;
+  }
+
   __device__ static int run(const d3* q0, const d3* q1, double eta, double& toi, double cosr, double sinr) {
     d3 lo = {1.79769313486231570815e+308, 1.79769313486231570815e+308, 1.79769313486231570815e+308};
     d3 hi = {-lo.x, -lo.y, -lo.z};
+#pragma unroll
     for (int i = 1; i <= 3; ++i) {
       const d3 ps[2] = {q0[i], q1[i]};
       for (const d3& q : ps) {
@@ -938,12 +1048,60 @@ template <> struct Query<3> {
     lo = {lo.x - eta, lo.y - eta, lo.z - eta};
     hi = {hi.x + eta, hi.y + eta, hi.z + eta};
     if (!ray_box(q0[0], q1[0], lo, hi)) return 0;
+    const double minD = eta * eta;
     Times T;
     T.n = 0;
-    bool hit = vertex_face(q0, q1, eta, T, cosr, sinr);
-    for (int e = 0; e < 3 && !hit; ++e) {
-      const int a = 1 + e, b = 1 + (e + 1) % 3;
-      hit = vertex_edge3(q0[0], q0[a], q0[b], q1[0], q1[a], q1[b], eta, T, cosr, sinr);
+    Ivals iv[4];  // the current sub-test's sign intervals
+    bool hit = false;
+    // One polynomial per pass, so every lane reaches the root finder from the
+    // same place whatever sub-test it is in.  Phase 0-2: the vertex-face edge
+    // planes, 3: its distance polynomial (CTCD.cpp:435-534); 4 + 3e + part: the
+    // vertex-edge sub-test of face edge e (CTCD.cpp:536-634); 13: vertex-vertex.
+    int ph = 0;
+    while (ph < 13) {
+      double op[7];
+      int deg, slot;
+      bool pos;
+      if (ph < 4) {
+        slot = ph;
+        const d3 v0 = sub(q1[0], q0[0]), v1 = sub(q1[1], q0[1]), v2 = sub(q1[2], q0[2]), v3 = sub(q1[3], q0[3]);
+        if (ph == 0) {
+          plane_poly(sub(q0[0], q0[1]), cross3(sub(q0[3], q0[1]), sub(q0[2], q0[1])), sub(q0[3], q0[1]), sub(v0, v1),
+                     cross3(sub(v3, v1), sub(v2, v1)), sub(v3, v1), op);
+        } else if (ph == 1) {
+          plane_poly(sub(q0[0], q0[2]), cross3(sub(q0[1], q0[2]), sub(q0[3], q0[2])), sub(q0[1], q0[2]), sub(v0, v2),
+                     cross3(sub(v1, v2), sub(v3, v2)), sub(v1, v2), op);
+        } else if (ph == 2) {
+          plane_poly(sub(q0[0], q0[3]), cross3(sub(q0[2], q0[3]), sub(q0[1], q0[3])), sub(q0[2], q0[3]), sub(v0, v3),
+                     cross3(sub(v2, v3), sub(v1, v3)), sub(v2, v3), op);
+        } else {
+          distance_poly(sub(q0[0], q0[1]), sub(q0[2], q0[1]), sub(q0[3], q0[1]), sub(v0, v1), sub(v2, v1),
+                        sub(v3, v1), minD, op);
+        }
+        deg = ph < 3 ? 3 : 6;
+        pos = ph < 3;
+      } else {
+        const int e = (ph - 4) / 3, part = (ph - 4) % 3;
+        slot = part;
+        // face edge e runs from face vertex e to face vertex e+1 (mod 3)
+        const d3 a0 = pick3(e, q0[1], q0[2], q0[3]), b0 = pick3(e, q0[2], q0[3], q0[1]);
+        const d3 a1 = pick3(e, q1[1], q1[2], q1[3]), b1 = pick3(e, q1[2], q1[3], q1[1]);
+        deg = vertex_edge_poly(part, q0[0], a0, b0, q1[0], a1, b1, minD, op, pos);
+      }
+      Ivals cur;
+      find_intervals(op, deg, cur, pos, cosr, sinr);
+      iv[slot] = cur;
+      if (cur.n == 0) {  // the sub-test fails: on to the next one
+        ph = ph < 4 ? 4 : 4 + 3 * ((ph - 4) / 3 + 1);
+        continue;
+      }
+      if (ph == 3 || (ph >= 4 && (ph - 4) % 3 == 2)) {
+        if (combine(iv, ph == 3, T)) {
+          hit = true;
+          break;
+        }
+      }
+      ph++;
     }
     bool raw = false;  // vertex-vertex lists are appended unsorted
     for (int w = 0; w < 3 && !hit; ++w) {
