@@ -152,17 +152,18 @@ def make_workload(name):
 
 
 def cpu_baseline(name, V0, V1, F, th, gpu_leaf_pairs):
-    """Times the oracle on this host.  For the 1M-triangle workload the full CPU
-    run takes minutes, so the sample is the same generator at 45x45 quads per
-    layer (1/16 of the triangles) and the time is scaled by the ratio of
-    broadphase leaf pairs (the CPU path's work is linear in them)."""
+    """Times the oracle on this host.  For the 1M-triangle workload a CPU run
+    takes tens of seconds per frame, so the sample is the same generator at
+    90x90 quads per layer (a quarter of the triangles; about 10-20 s of host time
+    for the timed frames) and the time is scaled by the ratio of broadphase leaf
+    pairs (the CPU path's work is linear in them)."""
     from oracle.oracle import Oracle
     from mclccd_amd import scenes
 
     cores = _cores()
     if name == "layers":
-        sV0, sV1, sF, sth = scenes.stacked_layers(n=45)
-        sample = "stacked layers at 45x45 quads per layer (64,800 tris), scaled by leaf pairs"
+        sV0, sV1, sF, sth = scenes.stacked_layers(n=90)
+        sample = f"stacked layers at 90x90 quads per layer ({len(sF):,} tris), scaled by leaf pairs"
     else:
         sV0, sV1, sF, sth = V0, V1, F, th
         sample = "full workload"
