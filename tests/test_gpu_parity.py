@@ -45,7 +45,10 @@ def run_both(V0, V1, P, theta, strict, active=None, **opts):
     return t, o
 
 
-def check_parity(t, o, strict):
+def check_parity(t, o, strict, order_dependent=None):
+    """order_dependent: canonical leaf pairs whose discrete result depends on
+    the argument order (degenerate triangles); without the reference's
+    orientation (strict) they are left out of the discrete comparison."""
     # leaves: FP64 boxes, active flags and representative-triangle flags, bit-identical
     gb, ga, gf = t.leaves()
     ob, oa = o.boxes()
@@ -90,11 +93,14 @@ def check_parity(t, o, strict):
     assert int((gt >= 0).sum()) == len(hs)
     # discrete
     gd, od = t.discrete(), o.discrete()
-    assert len(gd) == len(od)
     if strict:
+        assert len(gd) == len(od)
         assert oriented_pairs(gd) == oriented_pairs(od)
     else:
-        assert canon_pairs(gd) == canon_pairs(od)
+        skip = order_dependent or set()
+        assert canon_pairs(gd) - skip == canon_pairs(od) - skip
+        if not skip:
+            assert len(gd) == len(od)
     return c
 
 
