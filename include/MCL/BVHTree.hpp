@@ -39,6 +39,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "../mclg.h"
@@ -156,6 +157,12 @@ template <typename T, int DIM> class BVHTree {
 
  public:
   typedef BVHLeaf<T, DIM> LeafType;
+  // BVHTree.hpp:36-39 (VecType needs Eigen, as in the reference)
+#ifdef MCLG_HAVE_EIGEN
+  typedef Eigen::Matrix<T, DIM, 1> VecType;
+#endif
+  typedef std::pair<Vector4i, bool> PairType;  // [sten, is_vf]
+  typedef std::pair<int, bool> NodeIndex;      // [index, isleaf]
 
   // BVHTree.hpp:42-64
   struct Options {

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <type_traits>
+#include <utility>
 #include <map>
 #include <set>
 #include <vector>
@@ -60,6 +62,10 @@ int main(int argc, char** argv) {
   if (!out) return 2;
 
   mcl::BVHTree<double, 3> tree;
+  // the reference's public typedefs (BVHTree.hpp:35-39)
+  static_assert(std::is_same<mcl::BVHTree<double, 3>::PairType, std::pair<mcl::Vector4i, bool>>::value, "PairType");
+  static_assert(std::is_same<mcl::BVHTree<double, 3>::NodeIndex, std::pair<int, bool>>::value, "NodeIndex");
+  static_assert(std::is_same<mcl::BVHTree<double, 3>::LeafType, mcl::BVHLeaf<double, 3>>::value, "LeafType");
   tree.options.box_eta = eta;
   tree.options.vf_ccd_eta = eta;
   tree.options.ee_ccd_eta = eta;
@@ -198,6 +204,20 @@ int main(int argc, char** argv) {
         box_bad += tf.leaves[i].box.t1.max()[k] != hi1 + ef;
       }
     std::fprintf(out, "float %ld %d %ld %g\n", nf_hits, (int)(hf == hd), box_bad, (double)tf.options.vf_ccd_eta);
+  }
+  // 8. refit: the same frame through refit gives the rebuilt tree's hits
+  {
+    long n_a = 0, n_b = 0;
+    tree.narrow_phase = nullptr;
+    tree.filter_pair = nullptr;
+    tree.append_discrete = nullptr;
+    tree.append_pair = [&](const mcl::Vector4i&, bool, const double&) { ++n_a; };
+    tree.update(V0.data(), V1.data(), F.data(), nf, 3);
+    tree.traverse(V0.data(), V1.data(), F.data());
+    tree.append_pair = [&](const mcl::Vector4i&, bool, const double&) { ++n_b; };
+    tree.refit(V0.data(), V1.data(), F.data(), nf, 3);
+    tree.traverse(V0.data(), V1.data(), F.data());
+    std::fprintf(out, "refit %ld %ld\n", n_a, n_b);
   }
   std::fclose(out);
   return 0;

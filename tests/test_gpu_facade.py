@@ -74,3 +74,6 @@ def test_facade_callbacks(gpu, tmp_path):
     # TOIs rounded to float; leaf boxes equal to float arithmetic
     n_float_hits, same, box_bad, eta_f = rec["float"]
     assert n_float_hits > 0 and same == 1 and box_bad == 0 and eta_f == np.float32(eta)
+    # refit on the same frame: the rebuilt tree's hits
+    n_a, n_b = rec["refit"]
+    assert n_a == n_b == len(rs)
