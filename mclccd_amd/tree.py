@@ -248,6 +248,33 @@ class BVHTree:
 
         return view(r.hit_sten4, (n, 4), "<i4"), view(r.hit_is_vf, (n,), "|u1"), view(r.hit_toi, (n,), "<f8")
 
+    def device_candidates(self):
+        """The candidate queues of the last traverse as torch views of the
+        library's device buffers (no copy; valid until the next traverse):
+        (vf_sten [n_vf,4] int32, vf_toi [n_vf] f64, ee_sten [n_ee,4] int32,
+        ee_toi [n_ee] f64); toi -1 where the narrow phase found no contact.
+        Needs a traverse run with keep_candidates."""
+        import torch
+
+        r = _lib.DeviceResults()
+        self._check(lib.mclg_get_device_results(self._h, C.byref(r)))
+        if (r.n_vf and not r.vf_toi) or (r.n_ee and not r.ee_toi):
+            raise RuntimeError("device_candidates: the last traverse did not keep candidates (keep_candidates=True)")
+        dev = torch.device("cuda", torch.cuda.current_device())
+
+        def view(ptr, shape, typestr, dtype):
+            if shape[0] == 0 or not ptr:
+                return torch.zeros(shape, dtype=dtype, device=dev)
+
+            class _A:
+                __cuda_array_interface__ = {"shape": shape, "typestr": typestr, "data": (int(ptr), False),
+                                            "version": 3}
+            return torch.as_tensor(_A(), device=dev)
+
+        nvf, nee = int(r.n_vf), int(r.n_ee)
+        return (view(r.vf_sten4, (nvf, 4), "<i4", torch.int32), view(r.vf_toi, (nvf,), "<f8", torch.float64),
+                view(r.ee_sten4, (nee, 4), "<i4", torch.int32), view(r.ee_toi, (nee,), "<f8", torch.float64))
+
     def discrete(self):
         n = self.counts()["discrete"]
         out = np.empty((n, 2), np.int32)
