@@ -17,6 +17,8 @@
 //   - vf_one_sided runs the CTCD narrow phase with its wrong-side test on the
 //     GPU for vertex-face stencils (EE stays ACCD), as BVHTree.cpp:446-447 does.
 //     GPU failures throw std::runtime_error.
+//   - One thread at a time per tree: update and traverse write the tree's
+//     device buffers (separate trees may run on separate threads).
 //   - T = double and T = float (BVHTree.cpp:504-507).  The GPU path computes in
 //     FP64: a float tree widens its vertices (exactly) to double, so its
 //     candidates, hits and discrete pairs are those of BVHTree<double,DIM> on

@@ -59,6 +59,7 @@ struct Ivals {
   double l[7], u[7];
   int n;
   __device__ __forceinline__ void push(double t1, double t2) {
+    if (n >= 7) return;  // at most degree+1 <= 7 intervals: never taken
     double a = t1, b = t2;
     if (a > b) { const double s = a; a = b; b = s; }
     put(l, n, smax(a, 0.0));
@@ -73,6 +74,7 @@ struct Times {
   int n;
   // interval sub-tests: distinct values, ascending (the values are finite)
   __device__ void insert_sorted(double x) {
+    if (n >= 20) return;  // at most 1 + 7 + 3*4 distinct starts: never taken
     int i = 0;
     while (i < n && t[i] < x) ++i;
     if (i < n && t[i] == x) return;
@@ -799,7 +801,7 @@ __device__ __noinline__ bool vertex_vertex3(const d3& q1s, const d3& q2s, const 
     roots = 1;
   } else {
     if (c <= 0) {
-      T.t[T.n++] = 0.0;
+      if (T.n < 20) T.t[T.n++] = 0.0;
       return true;
     }
     return false;
@@ -812,36 +814,36 @@ __device__ __noinline__ bool vertex_vertex3(const d3& q1s, const d3& q2s, const 
     check_interval(0, t1, op, 2, iv, false);
     if (iv.n) {
       curr = 0;
-      T.t[T.n++] = curr;
+      if (T.n < 20) T.t[T.n++] = curr;
     }
     check_interval(t1, t2, op, 2, iv, false);
     if (iv.n) {
       curr = smin(curr, t1);
-      T.t[T.n++] = curr;
+      if (T.n < 20) T.t[T.n++] = curr;
     }
     check_interval(t2, 1.0, op, 2, iv, false);
     if (iv.n) {
       curr = smin(t2, curr);
-      T.t[T.n++] = curr;
+      if (T.n < 20) T.t[T.n++] = curr;
     }
     return T.n > 0;
   }
   if (roots == 1) {
     check_interval(0, t1, op, 2, iv, false);
     if (iv.n) {
-      T.t[T.n++] = 0.0;
+      if (T.n < 20) T.t[T.n++] = 0.0;
       return true;
     }
     check_interval(t1, 1.0, op, 2, iv, false);
     if (iv.n) {
-      T.t[T.n++] = t1;
+      if (T.n < 20) T.t[T.n++] = t1;
       return true;
     }
     return false;
   }
   check_interval(0, 1.0, op, 2, iv, false);
   if (iv.n) {
-    T.t[T.n++] = 0.0;
+    if (T.n < 20) T.t[T.n++] = 0.0;
     return true;
   }
   return false;
@@ -991,8 +993,8 @@ __device__ __forceinline__ bool vertex_edge2(const d2* q0, const d2* q1, Times& 
   };
   const bool on0 = on_line(t0), on1 = on_line(t1);
   bool hit = false;
-  if (t0 >= 0 && t0 <= 1 && on0) { T.t[T.n++] = t0; hit = true; }
-  if (t1 >= 0 && t1 <= 1 && on1) { T.t[T.n++] = t1; hit = true; }
+  if (t0 >= 0 && t0 <= 1 && on0) { if (T.n < 20) T.t[T.n++] = t0; hit = true; }
+  if (t1 >= 0 && t1 <= 1 && on1) { if (T.n < 20) T.t[T.n++] = t1; hit = true; }
   return hit;
 }
 
