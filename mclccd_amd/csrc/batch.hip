@@ -518,7 +518,7 @@ int mclg_batch_autotune(mclg_batch* h, const char* cache_path, int reps) {
   reps = reps < 1 ? 3 : reps;
   double best = 1e30;
   int bt = c->tr_waves, bb = c->blocks_per_cu, bc = c->chunk;
-  const int tws[] = {2, 4, 8}, bpcs[] = {4, 8, 16}, chs[] = {1, 4};
+  const int tws[] = {2, 4, 8}, bpcs[] = {4, 8, 16, 32, 64}, chs[] = {1, 4};
   for (int tw : tws)
     for (int bpc : bpcs)
       for (int ch : chs) {

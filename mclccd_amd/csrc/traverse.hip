@@ -748,7 +748,7 @@ __global__ void k_hits_total(const unsigned long long* off, long long last, unsi
 // 64-stencil chunks per step and issues all their loads before any store, so
 // each wave keeps several independent loads in flight.  (Plain accesses:
 // non-temporal loads or stores made this kernel slower.)
-constexpr int HC_U = 4;
+constexpr int HC_U = 8;
 
 template <bool VF>
 __global__ __launch_bounds__(256) void k_hits_compact(const int4* __restrict__ q, const unsigned long long* n_dev,
