@@ -128,9 +128,9 @@ int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
  *   candidates: sten4 / is_vf as above plus the ACCD toi of each candidate
  *               (-1 when it does not hit), every candidate
  *               (MCLG_KEEP_CANDIDATES)
- * Hits are the VF hits then the EE hits, each in candidate order; candidates
- * follow the leaf-pair order, which varies from run to run (as does the
- * reference's concurrent callback order), so compare them as sets.
+ * Hits are the VF hits then the EE hits, each in the order the GPU finished
+ * them; candidates follow the leaf-pair order.  Both vary from run to run (as
+ * does the reference's concurrent callback order), so compare them as sets.
  *   leaf pairs: int[2*cap] (only when the traverse had MCLG_KEEP_LEAF_PAIRS;
  *               MCLG_ERR_STATE otherwise)
  * Stencils follow BVHTree.cpp:336-395: VF (v, f0, f1, f2) with -1 padding in

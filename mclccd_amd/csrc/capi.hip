@@ -367,6 +367,15 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       AccdLaunch L;
       L.rec = d_rec; L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; 
       L.vert_scene = nullptr; L.scene_eta = nullptr;
+      // the ACCD kernels append the hit list themselves; the scan + compaction
+      // pass below only runs when it overflowed hit_cap
+      const bool direct = accd_direct_hits(vf_cap) && accd_direct_hits(ee_cap);
+      if (direct) {
+        CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
+        CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
+        CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
+        L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
+      }
       mark(12);
       L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
       L.eta = opt.vf_ccd_eta; L.cand_toi = d_vf_toi; L.chunk_hits = d_hit_off;
@@ -383,12 +392,16 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       H.eeq = d_eeq; H.n_ee_dev = d_counters + C_CAND_EE; H.ee_cap = ee_cap; H.ee_toi = d_ee_toi;
       H.off = d_hit_off; H.scan_part = d_hit_part; H.counters = d_counters;
       H.num_cus = num_cus; H.blocks_per_cu = blocks_per_cu;
+      bool scanned = false;
       for (int attempt = 0;; ++attempt) {
-        CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
-        CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
-        CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
-        H.hit_sten = d_hit_sten; H.hit_vf = d_hit_vf; H.hit_toi = d_hit_toi; H.hit_cap = hit_cap;
-        launch_hits(dim, H, attempt == 0, stream);  // a regrowth redoes the compaction only
+        if (!(direct && attempt == 0)) {
+          CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
+          CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
+          CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
+          H.hit_sten = d_hit_sten; H.hit_vf = d_hit_vf; H.hit_toi = d_hit_toi; H.hit_cap = hit_cap;
+          launch_hits(dim, H, !scanned, stream);  // a regrowth redoes the compaction only
+          scanned = true;
+        }
         mark(13);
         HIPCHECK(this, hipGetLastError());
         CHECK(read_counters());

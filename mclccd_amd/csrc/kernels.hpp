@@ -107,7 +107,16 @@ struct AccdLaunch {
   int num_cus;
   int blocks_per_cu;
   int one_sided = 0;                 // VF: CTCD with the wrong-side test instead of ACCD
+  // direct hit list: hits appended through counters[C_HITS] (zeroed by the
+  // caller), records past hit_cap dropped.  Null, or a queue of 2^32 or
+  // more stencils: only cand_toi / chunk_hits, and launch_hits builds the list.
+  int4* hit_sten = nullptr;
+  uint8_t* hit_vf = nullptr;
+  double* hit_toi = nullptr;
+  unsigned long long hit_cap = 0;
 };
+// Whether launch_accd appends hits directly for a queue of this capacity.
+inline bool accd_direct_hits(unsigned long long cap) { return cap < (1ull << 32); }
 void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s);
 long long accd_chunks(unsigned long long cap);
 

@@ -665,23 +665,75 @@ struct AccdArgs {
   double* cand_toi;         // per-stencil toi (-1 = no hit)
   unsigned long long* chunk_hits;  // hits per 64-stencil chunk
   unsigned long long* counters;
+  // direct hit list (null: the compaction pass builds it from cand_toi)
+  int4* hit_sten;
+  uint8_t* hit_vf;
+  double* hit_toi;
+  unsigned long long hit_cap;
 };
 
 constexpr int AC_WAVES = 4;
 
+// Direct hit list: each wave stages its hits (queue index, toi) in LDS and
+// appends them with one counter atomic per flush of up to N hits.  3D:
+// 4 x 768 x 12 B = 36 KB per block keeps the four blocks per CU that the
+// kernels' registers allow; 2D (fewer registers, six blocks) stages 512.
+// Queue indices are 32-bit: the host only asks for the direct list when
+// both queues are shorter than 2^32.
+template <int D> constexpr int ac_stage() { return D == 3 ? 768 : 512; }  // multiples of 64
+
+template <int N> struct HitStage {
+  uint32_t idx[AC_WAVES][N];
+  double toi[AC_WAVES][N];
+};
+
+// Called by the whole wave (converged) after each 64-stencil chunk.
+template <bool VF, int N>
+__device__ __forceinline__ void stage_hits(HitStage<N>& S, int w, int& cnt, bool hit, long long i, double toi,
+                                           const AccdArgs& A, bool last) {
+  const int lane = lane_id();
+  const uint64_t m = __ballot(hit);
+  if (hit) {
+    const int p = cnt + popc_below(m);
+    S.idx[w][p] = (uint32_t)i;
+    S.toi[w][p] = toi;
+  }
+  cnt += __popcll(m);
+  if (cnt == 0 || (cnt <= N - kWave && !last)) return;
+  wave_lds_sync();
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)cnt);
+  base = __shfl(base, 0);
+  for (int k = lane; k < cnt; k += kWave)
+    if (base + k < A.hit_cap) {
+      const unsigned long long pos = base + k;
+      const int4 sv = A.q[S.idx[w][k]];  // read by this wave a few chunks ago
+      __builtin_nontemporal_store(nt_int4{sv.x, sv.y, sv.z, sv.w}, reinterpret_cast<nt_int4*>(A.hit_sten + pos));
+      __builtin_nontemporal_store(S.toi[w][k], A.hit_toi + pos);
+      __builtin_nontemporal_store((uint8_t)(VF ? 1 : 0), A.hit_vf + pos);
+    }
+  wave_lds_sync();
+  cnt = 0;
+}
+
 // One lane per stencil, 64 consecutive stencils per wave step; the TOI (or
-// -1) goes to cand_toi, the hit list is compacted afterwards at scanned
-// offsets, so no shared counter is touched per hit.
+// -1) goes to cand_toi and the chunk's hit count to chunk_hits.  With
+// A.hit_sten the hits are also appended to the hit list (staged per wave,
+// one counter atomic per flush), so the compaction pass only runs when the
+// list overflowed hit_cap.
 template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
   using V = typename VLoad<D>::V;
+  __shared__ HitStage<ac_stage<D>()> S;
   const int w = threadIdx.x / kWave, lane = lane_id();
   long long evals = 0;
+  int staged = 0;
   const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * AC_WAVES;
   for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
     const long long i = ch * kWave + lane;
     int hit = 0;
+    double toi = -1.0;
     if (i < n) {
       const int4 sv = A.q[i];
       const int id[4] = {sv.x, sv.y, sv.z, sv.w};
@@ -689,7 +741,6 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
 #pragma unroll
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
-      double toi = -1.0;
       int ev = 0;
       hit = accd<D, VF>(v0, v1, eta, toi, ev);
       __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
@@ -697,6 +748,7 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
     }
     const uint64_t m = __ballot(hit != 0);
     if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    if (A.hit_sten) stage_hits<VF>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
   }
   // one atomic per block (same-address atomics serialise)
   __shared__ long long s_ev[AC_WAVES];
@@ -717,13 +769,16 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
 // sinr: the root finder's shift rotation, cos/sin(94 * 0.017453293).
 template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(AccdArgs A, double cosr, double sinr) {
   using V = typename VLoad<D>::V;
+  __shared__ HitStage<ac_stage<D>()> S;
   const int w = threadIdx.x / kWave, lane = lane_id();
+  int staged = 0;
   const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * AC_WAVES;
   for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
     const long long i = ch * kWave + lane;
     int hit = 0;
+    double toi = -1.0;
     if (i < n) {
       const int4 sv = A.q[i];
       const int id[4] = {sv.x, sv.y, sv.z, sv.w};
@@ -731,12 +786,12 @@ template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(Accd
 #pragma unroll
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
-      double toi = -1.0;
       hit = ctcd::Query<D>::run(v0, v1, eta, toi, cosr, sinr) == 1;
       __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
     }
     const uint64_t m = __ballot(hit != 0);
     if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    if (A.hit_sten) stage_hits<true>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
   }
 }
 
@@ -826,6 +881,8 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   A.q = L.q; A.n_dev = L.n_dev; A.cap = L.cap; A.rec = L.rec; A.eta = L.eta;
   A.vert_scene = L.vert_scene; A.scene_eta = L.scene_eta; A.cand_toi = L.cand_toi;
   A.chunk_hits = L.chunk_hits; A.counters = L.counters;
+  const bool direct = L.hit_sten && accd_direct_hits(L.cap);
+  A.hit_sten = direct ? L.hit_sten : nullptr; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
   const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);

@@ -6,7 +6,8 @@ do not need a full CPU run:
     brute-force set over all leaves (nothing missed, nothing extra, no duplicate);
   - sampled candidate stencils get the oracle's ACCD TOI bit for bit (the same
     stencil in the same order gives the same arithmetic);
-  - the hit list is exactly the candidates with toi >= 0, VF then EE, in queue order;
+  - the hit list holds exactly the candidates with toi >= 0 (as multisets of
+    (stencil, toi) records, through a 64-bit hash sum per kind), VF hits first;
   - sampled discrete pairs intersect at V1 and sampled non-discrete leaf pairs
     do not (the oracle's tri-tri / segment test).
 The leaf boxes themselves are pinned bit for bit against the oracle at smaller
@@ -80,19 +81,44 @@ def _check_candidates_and_hits(t, V0, V1, th, rng):
         want = O.toi_list(V0, V1, S, np.full(len(S), vf, np.uint8), th, th)
         assert np.array_equal(T, want), f"{'VF' if vf else 'EE'} TOIs differ from the oracle on the same stencils"
     hs, hv, ht = t.device_hits()
-    # compared slice by slice: torch's boolean-mask indexing of a whole
-    # 297M-row queue returned zero rows past ~67M input rows on this software
-    # stack, while every slice of 8M rows is indexed correctly
-    pos = 0
+    # The ACCD kernels append hits in the order their waves finish, so the
+    # hit list is compared with the candidates as a multiset: per kind, the
+    # record count and the wrapping int64 sum of a 64-bit mix of each record
+    # (stencil, toi bits).  Worked slice by slice: torch's boolean-mask
+    # indexing of a whole 297M-row queue returned zero rows past ~67M input
+    # rows on this software stack.
+    step = 1 << 23
+    nvf = 0
     for sten, toi, flag in ((vs, vt, 1), (es, et, 0)):
-        for a in range(0, len(sten), 1 << 23):
-            m = toi[a:a + (1 << 23)] >= 0
-            k = int(m.sum())
-            assert torch.equal(hs[pos:pos + k], sten[a:a + (1 << 23)][m]), f"hit stencils differ near hit {pos}"
-            assert torch.equal(ht[pos:pos + k], toi[a:a + (1 << 23)][m])
-            assert bool((hv[pos:pos + k] == flag).all())
-            pos += k
-    assert pos == len(hs) == t.counts()["hits"]
+        want_n, want_h = 0, torch.zeros((), dtype=torch.int64, device=sten.device)
+        for a in range(0, len(sten), step):
+            m = toi[a:a + step] >= 0
+            want_n += int(m.sum())
+            want_h += torch.where(m, _record_hash(sten[a:a + step], toi[a:a + step]), 0).sum()
+        got_n, got_h = 0, torch.zeros_like(want_h)
+        for a in range(0, len(hs), step):
+            m = hv[a:a + step] == flag
+            got_n += int(m.sum())
+            got_h += torch.where(m, _record_hash(hs[a:a + step], ht[a:a + step]), 0).sum()
+        kind = "VF" if flag else "EE"
+        assert got_n == want_n, f"{kind}: {got_n} hits, {want_n} candidates with toi >= 0"
+        assert int(got_h) == int(want_h), f"{kind}: hit records differ from the candidates with toi >= 0"
+        if flag:
+            nvf = want_n
+    assert len(hs) == t.counts()["hits"]
+    assert bool((hv[:nvf] == 1).all()) and bool((hv[nvf:] == 0).all()), "VF hits do not all precede EE hits"
+
+
+def _record_hash(sten, toi):
+    """A 64-bit mix of each (stencil, toi) record, as wrapping int64 arithmetic."""
+    import torch
+
+    s = sten.to(torch.int64)
+    h = (s[:, 0] * -7046029254386353131 + s[:, 1] * -4658895280553007687 + s[:, 2] * 6620516959819538809 +
+         s[:, 3] * -3372029247567499371 + toi.view(torch.int64) * 7809847782465536322)
+    h = h ^ (h >> 31)
+    h = h * -4265267296055464877
+    return h ^ (h >> 29)
 
 
 def _check_discrete(t, V1, P, pairs, rng):
