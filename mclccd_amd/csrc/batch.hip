@@ -183,6 +183,12 @@ static void batch_enqueue(Ctx* c) {
   L.q = c->d_vfq; L.n_dev = c->d_counters + C_CAND_VF; L.cap = c->vf_cap; L.n_hint = -1; L.eta = 0;
   L.scene_eta = S.vf_eta; L.cand_toi = c->d_vf_toi; L.chunk_hits = c->d_hit_off;
   L.one_sided = o.vf_one_sided != 0;
+  // the ACCD kernels append the hits (counters[C_HITS] was cleared with the
+  // others); an overflow reruns the frame with a larger list
+  const bool direct = accd_direct_hits(c->vf_cap) && accd_direct_hits(c->ee_cap);
+  if (direct) {
+    L.hit_sten = c->d_hit_sten; L.hit_vf = c->d_hit_vf; L.hit_toi = c->d_hit_toi; L.hit_cap = c->hit_cap;
+  }
   launch_accd(dim, true, L, s);
   L.one_sided = 0;
   if (dim == 3) {
@@ -190,6 +196,7 @@ static void batch_enqueue(Ctx* c) {
     L.cand_toi = c->d_ee_toi; L.chunk_hits = c->d_hit_off + ncv;
     launch_accd(dim, false, L, s);
   }
+  if (direct) return;
   HitsLaunch H;
   H.vfq = c->d_vfq; H.n_vf_dev = c->d_counters + C_CAND_VF; H.vf_cap = c->vf_cap; H.vf_toi = c->d_vf_toi;
   H.eeq = c->d_eeq; H.n_ee_dev = c->d_counters + C_CAND_EE; H.ee_cap = c->ee_cap; H.ee_toi = c->d_ee_toi;
