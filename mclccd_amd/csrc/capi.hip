@@ -355,21 +355,30 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     // FP64 ACCD over the two queues (BVHTree.cpp:428-461)
     if (opt.continuous && n_vf + n_ee > 0) {
       if (hit_cap == 0) hit_cap = std::max<unsigned long long>(1024, (unsigned long long)(n_vf + n_ee) / 2);
-      CHECK(grow((void**)&d_vf_toi, &cap_vf_toi, sizeof(double) * (size_t)std::max(n_vf, 1LL), "candidate toi"));
-      CHECK(grow((void**)&d_ee_toi, &cap_ee_toi, sizeof(double) * (size_t)std::max(n_ee, 1LL), "candidate toi"));
+      // the ACCD kernels append the hit list themselves; the scan + compaction
+      // pass below only runs when it overflowed hit_cap.  A list with room for
+      // every candidate cannot overflow: then the per-candidate TOIs are only
+      // written for MCLG_KEEP_CANDIDATES and the per-chunk hit counts not at all.
+      const bool direct = accd_direct_hits(vf_cap) && accd_direct_hits(ee_cap);
+      const bool no_overflow = direct && hit_cap >= (unsigned long long)(n_vf + n_ee);
+      const bool cand_toi = keep_cand || !no_overflow;
+      if (cand_toi) {
+        CHECK(grow((void**)&d_vf_toi, &cap_vf_toi, sizeof(double) * (size_t)std::max(n_vf, 1LL), "candidate toi"));
+        CHECK(grow((void**)&d_ee_toi, &cap_ee_toi, sizeof(double) * (size_t)std::max(n_ee, 1LL), "candidate toi"));
+      }
       const long long ncv = accd_chunks(vf_cap), nce = dim == 3 ? accd_chunks(ee_cap) : 0;
-      CHECK(grow((void**)&d_hit_off, &cap_hit_off, sizeof(unsigned long long) * (size_t)(ncv + nce + 1), "hit offsets"));
-      CHECK(grow((void**)&d_hit_part, &cap_hit_part,
-                 sizeof(unsigned long long) * (size_t)scan_part_words(ncv + nce + 1), "hit offsets"));
-      launch_fill_u64(d_hit_off, ncv + nce + 1, 0ull, stream);
+      if (!no_overflow) {
+        CHECK(grow((void**)&d_hit_off, &cap_hit_off, sizeof(unsigned long long) * (size_t)(ncv + nce + 1),
+                   "hit offsets"));
+        CHECK(grow((void**)&d_hit_part, &cap_hit_part,
+                   sizeof(unsigned long long) * (size_t)scan_part_words(ncv + nce + 1), "hit offsets"));
+        launch_fill_u64(d_hit_off, ncv + nce + 1, 0ull, stream);
+      }
       launch_fill_u64(d_counters + C_HITS, 1, 0ull, stream);
       launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
       AccdLaunch L;
       L.rec = d_rec; L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; 
       L.vert_scene = nullptr; L.scene_eta = nullptr;
-      // the ACCD kernels append the hit list themselves; the scan + compaction
-      // pass below only runs when it overflowed hit_cap
-      const bool direct = accd_direct_hits(vf_cap) && accd_direct_hits(ee_cap);
       if (direct) {
         CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
         CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
@@ -378,13 +387,15 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       }
       mark(12);
       L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
-      L.eta = opt.vf_ccd_eta; L.cand_toi = d_vf_toi; L.chunk_hits = d_hit_off;
+      L.eta = opt.vf_ccd_eta; L.cand_toi = cand_toi ? d_vf_toi : nullptr;
+      L.chunk_hits = no_overflow ? nullptr : d_hit_off;
       L.one_sided = opt.vf_one_sided != 0;
       launch_accd(dim, true, L, stream);
       L.one_sided = 0;
       mark(16);
       L.q = d_eeq; L.n_dev = d_counters + C_CAND_EE; L.cap = ee_cap; L.n_hint = n_ee;
-      L.eta = opt.ee_ccd_eta; L.cand_toi = d_ee_toi; L.chunk_hits = d_hit_off + ncv;
+      L.eta = opt.ee_ccd_eta; L.cand_toi = cand_toi ? d_ee_toi : nullptr;
+      L.chunk_hits = no_overflow ? nullptr : d_hit_off + ncv;
       if (dim == 3) launch_accd(dim, false, L, stream);
       mark(17);
       HitsLaunch H;
@@ -406,7 +417,10 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
         HIPCHECK(this, hipGetLastError());
         CHECK(read_counters());
         if (h_counters[C_HITS] <= hit_cap) break;
-        hit_cap = h_counters[C_HITS] + h_counters[C_HITS] / 4;
+        if (no_overflow) return fail(MCLG_ERR_STATE, "narrow phase: more hits than candidates");
+        // hits never outnumber candidates: a list that size never needs this pass again
+        hit_cap = std::min<unsigned long long>(h_counters[C_HITS] + h_counters[C_HITS] / 4,
+                                               (unsigned long long)(n_vf + n_ee));
         if (attempt > 4) return fail(MCLG_ERR_STATE, "narrow phase: sizes did not converge");
       }
       counts.hits = (int64_t)h_counters[C_HITS];

@@ -101,8 +101,9 @@ struct AccdLaunch {
   double eta;
   const int* vert_scene;
   const double* scene_eta;
-  double* cand_toi;                  // per stencil: toi, -1 = no hit
-  unsigned long long* chunk_hits;    // hits per 64-stencil chunk (accd_chunks(cap) entries, zeroed by the caller)
+  double* cand_toi;                  // per stencil: toi, -1 = no hit (null: not written; needs hit_sten)
+  unsigned long long* chunk_hits;    // hits per 64-stencil chunk (accd_chunks(cap) entries, zeroed by the
+                                     // caller; null: not written; needs hit_sten)
   unsigned long long* counters;
   int num_cus;
   int blocks_per_cu;

@@ -662,8 +662,8 @@ struct AccdArgs {
   double eta;
   const int* vert_scene;    // several scenes: per-scene gaps
   const double* scene_eta;
-  double* cand_toi;         // per-stencil toi (-1 = no hit)
-  unsigned long long* chunk_hits;  // hits per 64-stencil chunk
+  double* cand_toi;         // per-stencil toi (-1 = no hit); may be null with hit_sten
+  unsigned long long* chunk_hits;  // hits per 64-stencil chunk; may be null with hit_sten
   unsigned long long* counters;
   // direct hit list (null: the compaction pass builds it from cand_toi)
   int4* hit_sten;
@@ -743,11 +743,11 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
       int ev = 0;
       hit = accd<D, VF>(v0, v1, eta, toi, ev);
-      __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
+      if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
       evals += ev;
     }
     const uint64_t m = __ballot(hit != 0);
-    if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    if (lane == 0 && A.chunk_hits) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
     if (A.hit_sten) stage_hits<VF>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
   }
   // one atomic per block (same-address atomics serialise)
@@ -787,10 +787,10 @@ template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(Accd
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
       hit = ctcd::Query<D>::run(v0, v1, eta, toi, cosr, sinr) == 1;
-      __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
+      if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
     }
     const uint64_t m = __ballot(hit != 0);
-    if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    if (lane == 0 && A.chunk_hits) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
     if (A.hit_sten) stage_hits<true>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
   }
 }
@@ -883,6 +883,7 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   A.chunk_hits = L.chunk_hits; A.counters = L.counters;
   const bool direct = L.hit_sten && accd_direct_hits(L.cap);
   A.hit_sten = direct ? L.hit_sten : nullptr; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
+  if (!direct && (!A.cand_toi || !A.chunk_hits)) return;  // caller error: no output at all
   const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);

@@ -7,7 +7,9 @@ do not need a full CPU run:
   - sampled candidate stencils get the oracle's ACCD TOI bit for bit (the same
     stencil in the same order gives the same arithmetic);
   - the hit list holds exactly the candidates with toi >= 0 (as multisets of
-    (stencil, toi) records, through a 64-bit hash sum per kind), VF hits first;
+    (stencil, toi) records, through a 64-bit hash sum per kind), VF hits first,
+    in a frame whose hit list outgrew its capacity, in the next (appended
+    directly) and in one that keeps no per-candidate TOIs;
   - sampled discrete pairs intersect at V1 and sampled non-discrete leaf pairs
     do not (the oracle's tri-tri / segment test).
 The leaf boxes themselves are pinned bit for bit against the oracle at smaller
@@ -84,29 +86,47 @@ def _check_candidates_and_hits(t, V0, V1, th, rng):
     # The ACCD kernels append hits in the order their waves finish, so the
     # hit list is compared with the candidates as a multiset: per kind, the
     # record count and the wrapping int64 sum of a 64-bit mix of each record
-    # (stencil, toi bits).  Worked slice by slice: torch's boolean-mask
-    # indexing of a whole 297M-row queue returned zero rows past ~67M input
-    # rows on this software stack.
-    step = 1 << 23
-    nvf = 0
-    for sten, toi, flag in ((vs, vt, 1), (es, et, 0)):
-        want_n, want_h = 0, torch.zeros((), dtype=torch.int64, device=sten.device)
-        for a in range(0, len(sten), step):
-            m = toi[a:a + step] >= 0
-            want_n += int(m.sum())
-            want_h += torch.where(m, _record_hash(sten[a:a + step], toi[a:a + step]), 0).sum()
-        got_n, got_h = 0, torch.zeros_like(want_h)
-        for a in range(0, len(hs), step):
-            m = hv[a:a + step] == flag
-            got_n += int(m.sum())
-            got_h += torch.where(m, _record_hash(hs[a:a + step], ht[a:a + step]), 0).sum()
-        kind = "VF" if flag else "EE"
-        assert got_n == want_n, f"{kind}: {got_n} hits, {want_n} candidates with toi >= 0"
-        assert int(got_h) == int(want_h), f"{kind}: hit records differ from the candidates with toi >= 0"
-        if flag:
-            nvf = want_n
+    # (stencil, toi bits).
+    want = _candidate_hashes(vs, vt, es, et)
+    assert _hit_hashes(hs, hv, ht) == want, "hit records differ from the candidates with toi >= 0"
+    nvf = want[1][0]
     assert len(hs) == t.counts()["hits"]
     assert bool((hv[:nvf] == 1).all()) and bool((hv[nvf:] == 0).all()), "VF hits do not all precede EE hits"
+    return want
+
+
+# Worked in slices of 8M rows: torch's boolean-mask indexing of a whole
+# 297M-row queue returned zero rows past ~67M input rows on this software stack.
+_STEP = 1 << 23
+
+
+def _candidate_hashes(vs, vt, es, et):
+    """{1: (count, hash) of VF candidates with toi >= 0, 0: the same for EE}."""
+    import torch
+
+    out = {}
+    for sten, toi, flag in ((vs, vt, 1), (es, et, 0)):
+        n, h = 0, torch.zeros((), dtype=torch.int64, device=sten.device)
+        for a in range(0, len(sten), _STEP):
+            m = toi[a:a + _STEP] >= 0
+            n += int(m.sum())
+            h += torch.where(m, _record_hash(sten[a:a + _STEP], toi[a:a + _STEP]), 0).sum()
+        out[flag] = (n, int(h))
+    return out
+
+
+def _hit_hashes(hs, hv, ht):
+    import torch
+
+    out = {}
+    for flag in (1, 0):
+        n, h = 0, torch.zeros((), dtype=torch.int64, device=hs.device)
+        for a in range(0, len(hs), _STEP):
+            m = hv[a:a + _STEP] == flag
+            n += int(m.sum())
+            h += torch.where(m, _record_hash(hs[a:a + _STEP], ht[a:a + _STEP]), 0).sum()
+        out[flag] = (n, int(h))
+    return out
 
 
 def _record_hash(sten, toi):
@@ -139,13 +159,21 @@ def _check_discrete(t, V1, P, pairs, rng):
 
 def test_full_size_3d(gpu):
     V0, V1, F, th = scenes.stacked_layers()
+    # frame 1 outgrows the first hit-list capacity (the compaction pass); the
+    # grown list holds every candidate, so frame 2 appends directly and frame
+    # 3, without kept candidates, writes no per-candidate TOIs at all
     t = _run(V0, V1, F, th)
+    first = t.counts()
+    t.run(V0, V1, F, keep_candidates=True, keep_leaf_pairs=True)
+    assert t.counts() == first
     rng = np.random.default_rng(0)
     pairs = _check_pairs(t, len(F), 3, rng)
-    _check_candidates_and_hits(t, V0, V1, th, rng)
+    want = _check_candidates_and_hits(t, V0, V1, th, rng)
     _check_discrete(t, V1, F, pairs, rng)
-    c = t.counts()
-    print(f"[full] 3D {len(F)} tris: {c}")
+    t.run(V0, V1, F)
+    assert t.counts() == first
+    assert _hit_hashes(*t.device_hits()) == want, "frame 3 hits differ"
+    print(f"[full] 3D {len(F)} tris: {first}")
 
 
 def test_full_size_2d(gpu):
