@@ -180,8 +180,9 @@ template <typename T, bool PIN = false> __device__ __forceinline__ T load16(cons
 }
 
 // Pairs found by a wave are staged in LDS and written out in batches: one
-// global atomic per flush instead of one per emitting step.
-constexpr int TR_MAX_WAVES = 4, TR_BUF = 1024;
+// global atomic per flush instead of one per emitting step.  768 pairs keep a
+// block at 26 KB of LDS, so five blocks (the kernel's register limit) share a CU.
+constexpr int TR_MAX_WAVES = 4, TR_BUF = 768;
 
 // Called by the whole wave (converged).
 __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,

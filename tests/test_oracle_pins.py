@@ -415,25 +415,34 @@ def test_oracle_toi_gradient_vs_finite_differences():
     o = orc.run_scene(V0, V1, F, th)
     s, v, toi = o.hits()
     assert v.sum() > 50 and (1 - v).sum() > 50
+    # the oracle appends hits concurrently (as the reference's callbacks run):
+    # sample them in a canonical order so every run checks the same ones
+    order = np.lexsort((s[:, 3], s[:, 2], s[:, 1], s[:, 0], v))
+    s, v, toi = s[order], v[order], toi[order]
     rng = np.random.default_rng(0)
     h = 1e-7
+    sten1 = np.arange(4, dtype=np.int32)[None, :]
     checked = 0
     for i in rng.choice(len(s), 200, replace=False):
         st = s[i][:4]
         a, b = V0[st], V1[st]
+        vf = v[i:i + 1]
         hit, t0, ga, gb = orc.accd_grad(3, a, b, th, bool(v[i]))
         assert hit and t0 == toi[i]
+        _, s0 = orc.toi_list(a, b, sten1, vf, th, th, signature=True)
         for arr, g in ((0, ga), (1, gb)):
             j, k = int(rng.integers(0, 4)), int(rng.integers(0, 3))
             ap, bp = a.copy(), b.copy()
             (ap if arr == 0 else bp)[j, k] += h
             am, bm = a.copy(), b.copy()
             (am if arr == 0 else bm)[j, k] -= h
-            hp, tp, _ = orc.accd(3, ap, bp, th, bool(v[i]))
-            hm, tm, _ = orc.accd(3, am, bm, th, bool(v[i]))
-            if not (hp and hm):
+            tp, sp = orc.toi_list(ap, bp, sten1, vf, th, th, signature=True)
+            tm, sm = orc.toi_list(am, bm, sten1, vf, th, th, signature=True)
+            # a step that changes the forward pass's branches crosses a kink
+            # of the TOI, where central differences do not apply
+            if tp[0] < 0 or tm[0] < 0 or not (sp[0] == sm[0] == s0[0]):
                 continue
-            assert abs((tp - tm) / (2 * h) - g[j, k]) <= 1e-6 * max(1.0, abs(g[j, k]))
+            assert abs((tp[0] - tm[0]) / (2 * h) - g[j, k]) <= 1e-6 * max(1.0, abs(g[j, k]))
             checked += 1
     assert checked > 350
 
