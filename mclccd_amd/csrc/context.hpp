@@ -20,8 +20,9 @@ struct Ctx {
   // tunables (the batch autotuner searches them)
   // blocks per CU in a queue kernel's grid: more than fit at once, so the
   // hardware hands out the rest as blocks finish, which evens out chunks of
-  // uneven cost (ACCD: 32 measured ~2 ms faster per 1M-triangle frame than 8)
-  int blocks_per_cu = 32;
+  // uneven cost (ACCD per 1M-triangle frame: 32 measured ~2 ms faster than 8,
+  // 64 a further ~0.35 ms; 16 slower than 32)
+  int blocks_per_cu = 64;
   int chunk = 1;           // consecutive 64-item chunks per wave step
   int tr_waves = 4;        // waves per traversal block
   mclg_options opt{};
