@@ -192,7 +192,8 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
  *   6 validate+pack  7 traversal kernel  8 candidate + discrete kernel
  *   9 ACCD kernels (traverse; 7-9 bracket the final launches only)
  *  10 TOI adjoint (last mclg_toi_grad: sort, adjoint kernels)
- *  11 VF ACCD kernel  12 EE ACCD kernel (both inside 9, which adds the hit compaction)
+ *  11 VF ACCD kernel  12 EE ACCD kernel (both inside 9, which adds the hit
+ *     compaction on a frame whose hit list outgrew its capacity)
  * Writes min(n, MCLG_NUM_TIMINGS) values. */
 #define MCLG_NUM_TIMINGS 13
 int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);

@@ -601,7 +601,7 @@ __device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a
 }
 
 // Pass 2: the discrete test and the candidates, written at the scanned offsets
-// (each chunk's stencils in lane order: the queues are deterministic).
+// (each chunk's stencils in lane order: the queues follow the pair list).
 template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
   __shared__ int4 s_st[EX_WAVES][kWave * 9];  // at most 9 EE (6 VF) stencils per pair
