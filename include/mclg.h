@@ -246,7 +246,7 @@ int mclg_batch_copy_hits(mclg_batch* b, int* scene, int* sten4, unsigned char* i
 int mclg_batch_copy_discrete(mclg_batch* b, int* scene, int* pairs2, int64_t cap);
 
 /* Execution parameters (negative / zero: keep): hipGraph replay on/off, waves
- * per traversal block (1-8), resident blocks per CU of the queue kernels,
+ * per traversal block (1-4; larger values are clamped), resident blocks per CU of the queue kernels,
  * 64-item chunks a wave takes per step.  out4 receives the current values. */
 int mclg_batch_set_params(mclg_batch* b, int use_graph, int tr_waves, int blocks_per_cu, int chunk);
 int mclg_batch_get_params(const mclg_batch* b, int* out4);

@@ -393,7 +393,7 @@ int mclg_batch_set_params(mclg_batch* h, int use_graph, int tr_waves, int blocks
   Ctx* c = B(h);
   if (!c || !c->batch) return MCLG_ERR_ARG;
   if (use_graph >= 0) c->batch->use_graph = use_graph != 0;
-  if (tr_waves > 0) c->tr_waves = std::min(tr_waves, 8);
+  if (tr_waves > 0) c->tr_waves = std::min(tr_waves, kTraverseMaxWaves);
   if (blocks_per_cu > 0) c->blocks_per_cu = blocks_per_cu;
   if (chunk > 0) c->chunk = chunk;
   drop_graph(c->batch);
@@ -525,7 +525,7 @@ int mclg_batch_autotune(mclg_batch* h, const char* cache_path, int reps) {
   reps = reps < 1 ? 3 : reps;
   double best = 1e30;
   int bt = c->tr_waves, bb = c->blocks_per_cu, bc = c->chunk;
-  const int tws[] = {2, 4, 8}, bpcs[] = {4, 8, 16, 32, 64}, chs[] = {1, 4};
+  const int tws[] = {2, kTraverseMaxWaves}, bpcs[] = {4, 8, 16, 32, 64}, chs[] = {1, 4};
   for (int tw : tws)
     for (int bpc : bpcs)
       for (int ch : chs) {

@@ -56,6 +56,8 @@ void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32
 // Leaf pairs as sorted positions (s, k), s < k.  Several scenes: scene_of_sorted
 // gives each sorted leaf's scene and scene_off the scenes' sorted-leaf offsets
 // (n+1 entries); pairs never cross scenes.
+// waves per traversal block: 1..kTraverseMaxWaves (larger values are clamped)
+constexpr int kTraverseMaxWaves = 4;
 void launch_traverse(int dim, const void* nodes, const void* cp, const int* root, const void* sboxes, int n,
                      int2* pairs, unsigned long long cap, unsigned long long* counters,
                      const uint32_t* scene_of_sorted, const int* scene_off, int waves, hipStream_t s);

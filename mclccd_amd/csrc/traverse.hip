@@ -182,7 +182,7 @@ template <typename T, bool PIN = false> __device__ __forceinline__ T load16(cons
 // Pairs found by a wave are staged in LDS and written out in batches: one
 // global atomic per flush instead of one per emitting step.  768 pairs keep a
 // block at 26 KB of LDS, so five blocks (the kernel's register limit) share a CU.
-constexpr int TR_MAX_WAVES = 4, TR_BUF = 768;
+constexpr int TR_MAX_WAVES = kTraverseMaxWaves, TR_BUF = 768;
 
 // Called by the whole wave (converged).
 __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __restrict__ pairs, unsigned long long cap,
