@@ -6,10 +6,13 @@
 // std::function callbacks and the same update/traverse overloads.  Differences
 // a user can see:
 //   - Callbacks are replayed on the calling thread, serially, after the GPU has
-//     produced every result (the reference calls them concurrently from TBB
-//     workers).  Order: for each candidate stencil filter_pair (if set), then the
-//     toi (narrow_phase if set, else the GPU's ACCD) and append_pair when
-//     toi >= 0; then append_discrete for each discrete hit until one returns true.
+//     produced every result (the reference's threaded run calls them
+//     concurrently from TBB workers).  Order: leaf pair by leaf pair, as collide
+//     does — filter_pair on the pair's candidates, the toi (narrow_phase if set,
+//     else the GPU's ACCD) and append_pair when toi >= 0, then append_discrete,
+//     whose true return ends the replay.  With options.threaded false the pairs
+//     come in the reference's serial order and are oriented by its KdBVH, so the
+//     callback sequence is the reference's own (mclg_copy_events).
 //   - The tree is the GPU's LBVH: the `tree` member (an Eigen KdBVH) does not
 //     exist.  `leaves` is filled on demand by sync_leaves().  traverse(BVHTraverse*)
 //     walks a host copy of the LBVH with the KdBVH walk's rules, so a visitor
@@ -220,36 +223,63 @@ template <typename T, int DIM> class BVHTree {
     pull_options();
   }
 
-  // BVHTree.hpp:86
+  // BVHTree.hpp:86.  Callbacks are replayed leaf pair by leaf pair in the
+  // order of mclg_copy_events (the reference's serial order when
+  // options.threaded is false), as collide does for each pair
+  // (BVHTree.cpp:252-283): filter_pair on each of its candidates, then the toi
+  // (narrow_phase, or the GPU's ACCD) and append_pair of those kept, then
+  // append_discrete, whose true return stops the replay.
   void traverse(const T* V0, const T* V1, const int* P) const {
-    const bool need_cand = (bool)filter_pair || (bool)narrow_phase;
+    const bool need_cand = options.continuous && ((bool)filter_pair || (bool)narrow_phase);
+    const bool cont = options.continuous && ((bool)append_pair || need_cand);
+    const bool disc = options.discrete && (bool)append_discrete;
     const_cast<BVHTree*>(this)->push_options();
     if (np_ <= 1) return;  // BVHTree.cpp:131-132
     BVHTree* self = const_cast<BVHTree*>(this);
-    check(mclg_traverse(ctx_.get(), widen(V0, nv_, self->w0_), widen(V1, nv_, self->w1_), P,
-                        need_cand ? MCLG_KEEP_CANDIDATES : 0u));
+    const unsigned flags = (need_cand ? MCLG_KEEP_CANDIDATES : 0u) | (cont || disc ? MCLG_KEEP_LEAF_PAIRS : 0u);
+    check(mclg_traverse(ctx_.get(), widen(V0, nv_, self->w0_), widen(V1, nv_, self->w1_), P, flags));
+    if (!cont && !disc) return;
     mclg_counts c;
     check(mclg_get_counts(ctx_.get(), &c));
-    if (options.continuous && (append_pair || need_cand)) {
-      const int64_t n = need_cand ? c.candidates : c.hits;
-      std::vector<int> sten((size_t)n * 4);
-      std::vector<unsigned char> vf((size_t)n);
-      std::vector<double> toi((size_t)n);
-      if (need_cand) check(mclg_copy_candidates(ctx_.get(), sten.data(), vf.data(), toi.data(), n));
-      else check(mclg_copy_hits(ctx_.get(), sten.data(), vf.data(), toi.data(), n));
-      for (int64_t i = 0; i < n; ++i) {
-        const Vector4i s(sten[4 * i], sten[4 * i + 1], sten[4 * i + 2], sten[4 * i + 3]);
-        const bool is_vf = vf[i] != 0;
-        if (filter_pair && filter_pair(s, is_vf)) continue;
-        const T t = narrow_phase ? narrow_phase(s, is_vf) : (T)toi[i];
-        if (t >= 0 && append_pair) append_pair(s, is_vf, t);
+    int64_t ne = 0;
+    check(mclg_copy_events(ctx_.get(), need_cand ? 1 : 0, nullptr, nullptr, 0, &ne));
+    std::vector<int64_t> ev((size_t)ne), grp((size_t)ne);
+    check(mclg_copy_events(ctx_.get(), need_cand ? 1 : 0, ev.data(), grp.data(), ne, &ne));
+    const int64_t n = cont ? (need_cand ? c.candidates : c.hits) : 0;
+    std::vector<int> sten((size_t)n * 4);
+    std::vector<unsigned char> vf((size_t)n);
+    std::vector<double> toi((size_t)n);
+    if (cont && need_cand) check(mclg_copy_candidates(ctx_.get(), sten.data(), vf.data(), toi.data(), n));
+    else if (cont) check(mclg_copy_hits(ctx_.get(), sten.data(), vf.data(), toi.data(), n));
+    std::vector<int> d(disc ? (size_t)c.discrete * 2 : 0);
+    if (disc && c.discrete > 0) check(mclg_copy_discrete(ctx_.get(), d.data(), c.discrete));
+    std::vector<int64_t> kept;
+    for (int64_t k = 0; k < ne;) {  // one leaf pair: its records, then its discrete hit
+      int64_t j = k;
+      while (j < ne && grp[(size_t)j] == grp[(size_t)k]) ++j;
+      if (cont) {
+        kept.clear();
+        for (int64_t q = k; q < j; ++q) {  // get_candidates filters every stencil of the pair first
+          const int64_t i = ev[(size_t)q];
+          if (i < 0) continue;
+          if (filter_pair && filter_pair(stencil(sten, i), vf[(size_t)i] != 0)) continue;
+          kept.push_back(i);
+        }
+        for (int64_t i : kept) {
+          const Vector4i s = stencil(sten, i);
+          const bool is_vf = vf[(size_t)i] != 0;
+          const T t = narrow_phase ? narrow_phase(s, is_vf) : (T)toi[(size_t)i];
+          if (t >= 0 && append_pair) append_pair(s, is_vf, t);
+        }
       }
-    }
-    if (options.discrete && append_discrete && c.discrete > 0) {
-      std::vector<int> d((size_t)c.discrete * 2);
-      check(mclg_copy_discrete(ctx_.get(), d.data(), c.discrete));
-      for (int64_t i = 0; i < c.discrete; ++i)
-        if (append_discrete(d[2 * i], d[2 * i + 1])) break;
+      if (disc)
+        for (int64_t q = k; q < j; ++q) {
+          const int64_t e = ev[(size_t)q];
+          if (e >= 0) continue;
+          const int64_t di = -1 - e;
+          if (append_discrete(d[2 * (size_t)di], d[2 * (size_t)di + 1])) return;  // stop (BVHTree.cpp:236-239)
+        }
+      k = j;
     }
   }
 
@@ -388,6 +418,9 @@ template <typename T, int DIM> class BVHTree {
     mirror_ = true;
   }
 
+  static Vector4i stencil(const std::vector<int>& s, int64_t i) {
+    return Vector4i(s[4 * (size_t)i], s[4 * (size_t)i + 1], s[4 * (size_t)i + 2], s[4 * (size_t)i + 3]);
+  }
   static void check(int rc, mclg_ctx* c) {
     if (rc != MCLG_OK)
       throw std::runtime_error(std::string("mclg: ") + (c ? mclg_last_error(c) : "no HIP device / bad argument"));

@@ -45,7 +45,7 @@ typedef struct mclg_options {
   double vf_ccd_eta; /* VF gap; update() clamps it to box_eta */
   double ee_ccd_eta; /* EE gap; update() clamps it to box_eta */
   int reptri;        /* representative triangles (default 1) */
-  int threaded;      /* accepted for compatibility; the GPU path is always parallel */
+  int threaded;      /* 0: the reference's serial traversal order for replay (MCLG_ORIENT_KDBVH implied) */
   int discrete;      /* discrete tri-tri (3D) / edge-edge (2D) at V1 (default 1) */
   int continuous;    /* ACCD from V0 to V1 (default 1) */
   int vf_one_sided;  /* VF stencils through CTCD with the wrong-side test (NarrowPhase.cpp:283-419)
@@ -105,6 +105,12 @@ int mclg_update_refit_device(mclg_ctx* ctx, const double* V0, const double* V1, 
  * request, the candidate stencils and leaf pairs on the device. */
 #define MCLG_KEEP_CANDIDATES 1u /* keep every candidate stencil (for filter_pair / narrow_phase replay) */
 #define MCLG_KEEP_LEAF_PAIRS 2u /* keep the broadphase leaf pairs */
+/* also run the discrete test with the two primitives of each leaf pair swapped
+ * and list the pairs whose result changes (tt_isect.hpp's predicate is not
+ * symmetric for degenerate triangles, nor the 2D segment test for collinear
+ * segments): exactly the pairs where the orientation of the reference's KdBVH
+ * decides the discrete result (mclg_copy_order_dependent) */
+#define MCLG_CHECK_ORDER 4u
 int mclg_traverse(mclg_ctx* ctx, const double* V0, const double* V1, const int* P, unsigned flags);
 int mclg_traverse_device(mclg_ctx* ctx, const double* V0, const double* V1, const int* P, unsigned flags);
 
@@ -117,6 +123,7 @@ typedef struct mclg_counts {
   int64_t dist_evals; /* ACCD pair_distance evaluations performed */
   int64_t vf_candidates;  /* of candidates: vertex-face stencils (the rest are edge-edge) */
   int64_t ee_dist_evals;  /* of dist_evals: those of edge-edge stencils */
+  int64_t order_dependent; /* leaf pairs whose discrete result depends on the argument order (MCLG_CHECK_ORDER) */
 } mclg_counts;
 int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
 
@@ -140,6 +147,11 @@ int mclg_copy_hits(mclg_ctx* ctx, int* sten4, unsigned char* is_vf, double* toi,
 int mclg_copy_discrete(mclg_ctx* ctx, int* pairs2, int64_t cap);
 int mclg_copy_candidates(mclg_ctx* ctx, int* sten4, unsigned char* is_vf, double* toi, int64_t cap);
 int mclg_copy_leaf_pairs(mclg_ctx* ctx, int* pairs2, int64_t cap);
+/* The order-dependent leaf pairs of a traverse with MCLG_CHECK_ORDER, as
+ * int[2*cap] primitive pairs oriented like the discrete hits; a pair is in the
+ * discrete list iff the discrete test holds in that orientation.  No reference
+ * counterpart (diagnostic of the orientation; MCLG_ERR_STATE without the flag). */
+int mclg_copy_order_dependent(mclg_ctx* ctx, int* pairs2, int64_t cap);
 
 /* The leaves of the last update (the reference's public `leaves` member,
  * BVHTree.hpp:39, BVHLeaf.hpp:12-52): per primitive the swept, t0 and t1 boxes
@@ -186,6 +198,53 @@ int mclg_get_device_results(const mclg_ctx* ctx, mclg_device_results* out);
  * when the primitive count changes. */
 int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
 
+/* Orientation of leaf pairs (no reference counterpart; the reference always
+ * has its KdBVH):
+ *   MCLG_ORIENT_INDEX  lower primitive index first (or the rank of
+ *                      mclg_set_rank) — the default;
+ *   MCLG_ORIENT_KDBVH  the reference's own orientation: every update and refit
+ *                      rebuilds, on the host, the order of the reference's
+ *                      median-split KdBVH over the swept-box centres
+ *                      (KdBVH.hpp:103-134,192-222: std::nth_element, the same
+ *                      comparator and input order) and orients each pair by it
+ *                      (BVHTree.cpp:288-301).  EE stencils, EE TOIs and discrete
+ *                      pairs then equal the reference's record for record; the
+ *                      host build costs O(np log np) per update.
+ * An update with options.threaded == 0 (the reference's serial traversal,
+ * BVHTree.cpp:152-157) uses the KdBVH orientation whatever the mode, so that
+ * mclg_copy_events reproduces the reference's serial callback sequence. */
+#define MCLG_ORIENT_INDEX 0
+#define MCLG_ORIENT_KDBVH 1
+int mclg_set_orientation(mclg_ctx* ctx, int mode);
+int mclg_get_orientation(const mclg_ctx* ctx, int* mode);
+
+/* The reference KdBVH's leaf order for given swept-box centres, on the host
+ * (no GPU): rank[i] = in-order position of leaf i (KdBVH.hpp:192-222, objects
+ * reordered as in KdBVH.hpp:130-133); gapnode[g] (n-1 entries, may be NULL) =
+ * the index of the internal node that splits positions g and g+1, internal
+ * nodes numbered in the order the reference pushes their volumes, which is
+ * the order of the traversal's frontlist (BVHTree.cpp:168-185).  centres:
+ * n x dim, (min + max) / 2 of each swept box. */
+int mclg_kdbvh_order(int dim, const double* centres, int n, int* rank, int* gapnode);
+
+/* The callbacks of the last traverse in the order the reference's serial
+ * traversal calls them (BVHTree.cpp:152-157,230-302), for host replay: leaf
+ * pairs in visiting order — the reference's frontlist order under
+ * MCLG_ORIENT_KDBVH or threaded == 0 (by the node that first separates the two
+ * leaves, then by their positions), else by (rank of the first leaf, rank of
+ * the second) — and per leaf pair its CCD records in get_candidates order
+ * (VF first leaf -> second, VF second -> first, EE; BVHTree.cpp:327-395)
+ * followed by its discrete hit.  events[k] >= 0 indexes the hit list
+ * (mclg_copy_hits) or, with over_candidates, the candidate list
+ * (mclg_copy_candidates); events[k] < 0 is discrete hit -1-events[k]
+ * (mclg_copy_discrete).  group[k] numbers the leaf pair of event k (0, 1, ...
+ * over the pairs that have events).  Either output may be NULL; *n_events
+ * receives the count (hits or candidates, plus discrete hits).  Needs a
+ * traverse with MCLG_KEEP_LEAF_PAIRS (and MCLG_KEEP_CANDIDATES for
+ * over_candidates).  Host work: O(records log records). */
+int mclg_copy_events(mclg_ctx* ctx, int over_candidates, int64_t* events, int64_t* group, int64_t cap,
+                     int64_t* n_events);
+
 /* Device milliseconds of the phases of the last update / traverse, measured
  * with HIP events on the context's stream:
  *   0 validate+pack  1 reptri  2 leaf boxes  3 morton  4 sort  5 lbvh (update)
@@ -209,8 +268,24 @@ int mclg_get_timings(const mclg_ctx* ctx, double* ms, int n);
  * through.  Host pointers, or device pointers when device_ptrs.  Allocates
  * scratch on the first call (about 16 bytes per hit for the visiting order).
  * MCLG_ERR_UNSUPPORTED when the last traverse ran with vf_one_sided (CTCD
- * times have no adjoint here). */
+ * times have no adjoint here); MCLG_ERR_STATE when no traverse completed
+ * since the last update or refit (the hits would be paired with vertex
+ * positions they were not computed from). */
 int mclg_toi_grad(mclg_ctx* ctx, const double* w, double* dV0, double* dV1, int device_ptrs);
+
+/* The TOI adjoint without any allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * gradient as mclg_toi_grad(ctx, w, dV0, dV1, 1) (device pointers only) up to
+ * the order of the floating-point sums; w, dV0 and dV1 are the caller's.  Its
+ * scratch (the hits' visiting order and the queue of multi-step hits) is one
+ * static array of about 25 MB that each GPU the library runs on holds until
+ * the process ends, sized for at most MCLG_NOALLOC_MAX_HITS hits: a larger
+ * hit list fails with MCLG_ERR_CAPACITY.  Every call on a GPU shares that
+ * GPU's array, so calls that overlap on one GPU (from several contexts or
+ * threads) race.  The context's device must be current. */
+#define MCLG_NOALLOC_MAX_HITS (1 << 20)
+int mclg_toi_grad_noalloc(mclg_ctx* ctx, const double* w, double* dV0, double* dV1);
 
 /* ------------------------------------------------------------------------
  * Batched scenes.  No reference counterpart: the reference runs one

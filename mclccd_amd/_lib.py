@@ -19,9 +19,13 @@ ERR_NOMEM = -3
 ERR_STATE = -4
 ERR_UNSUPPORTED = -5
 ERR_CAPACITY = -6
+NOALLOC_MAX_HITS = 1 << 20
 
 KEEP_CANDIDATES = 1
 KEEP_LEAF_PAIRS = 2
+CHECK_ORDER = 4
+ORIENT_INDEX = 0
+ORIENT_KDBVH = 1
 NUM_TIMINGS = 13
 TIMING_NAMES = ("update_pack", "update_reptri", "update_leaf_boxes", "update_morton", "update_sort",
                 "update_lbvh", "traverse_pack", "traverse", "expand", "accd", "adjoint", "accd_vf", "accd_ee")
@@ -36,7 +40,7 @@ class Options(C.Structure):
 class Counts(C.Structure):
     _fields_ = [("leaf_pairs", C.c_int64), ("candidates", C.c_int64), ("hits", C.c_int64),
                 ("discrete", C.c_int64), ("dist_evals", C.c_int64),
-                ("vf_candidates", C.c_int64), ("ee_dist_evals", C.c_int64)]
+                ("vf_candidates", C.c_int64), ("ee_dist_evals", C.c_int64), ("order_dependent", C.c_int64)]
 
 
 class DeviceResults(C.Structure):
@@ -88,10 +92,12 @@ def _load():
         "mclg_copy_discrete": (I, [P, P, I64]),
         "mclg_copy_candidates": (I, [P, P, P, P, I64]),
         "mclg_copy_leaf_pairs": (I, [P, P, I64]),
+        "mclg_copy_order_dependent": (I, [P, P, I64]),
         "mclg_get_device_results": (I, [P, P]),
         "mclg_copy_leaves": (I, [P, P, P, P, I64]),
         "mclg_copy_tree": (I, [P, P, P, P, P, I64]),
         "mclg_toi_grad": (I, [P, P, P, P, I]),
+        "mclg_toi_grad_noalloc": (I, [P, P, P, P]),
         "mclg_batch_create": (I, [I, P]),
         "mclg_batch_destroy": (None, [P]),
         "mclg_batch_last_error": (C.c_char_p, [P]),
@@ -106,6 +112,10 @@ def _load():
         "mclg_batch_get_params": (I, [P, P]),
         "mclg_batch_autotune": (I, [P, C.c_char_p, I]),
         "mclg_set_rank": (I, [P, P, I]),
+        "mclg_set_orientation": (I, [P, I]),
+        "mclg_get_orientation": (I, [P, P]),
+        "mclg_kdbvh_order": (I, [I, P, I, P, P]),
+        "mclg_copy_events": (I, [P, I, P, P, I64, P]),
         "mclg_get_timings": (I, [P, P, I]),
     }
     for name, (res, args) in sig.items():

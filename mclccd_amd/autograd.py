@@ -34,7 +34,7 @@ class TOIFunction(torch.autograd.Function):
     def backward(ctx, gtoi, _gs, _gv):
         tree = ctx.tree
         if tree.serial != ctx.serial:
-            raise RuntimeError("the tree ran another traverse since this forward pass")
+            raise RuntimeError("the tree ran another update or traverse since this forward pass")
         g = gtoi.contiguous().to(torch.float64)
         dV0 = torch.empty(ctx.shape, dtype=torch.float64, device=g.device)
         dV1 = torch.empty(ctx.shape, dtype=torch.float64, device=g.device)

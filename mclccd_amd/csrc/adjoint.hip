@@ -491,13 +491,140 @@ __global__ __launch_bounds__(ADJ_T) void k_toi_adjoint(const int4* __restrict__ 
   toi_adjoint<D, VF>(sv, rec, eta, wi, s_S + threadIdx.x, queue == nullptr, gV0, gV1);
 }
 
+// Device scratch of one adjoint call: the visiting order (radix sort of the
+// hits by (kind, first vertex); keys null: hits in list order) and the queue
+// of multi-step hits with its counter.
+struct AdjScratch {
+  uint32_t *keys = nullptr, *vals = nullptr, *ktmp = nullptr, *vtmp = nullptr, *hist = nullptr;
+  long long* queue = nullptr;
+  long long qcap = 0;
+  unsigned long long* qn = nullptr;
+};
+
 template <int D, bool VF>
 static void launch_adjoint(Ctx* c, const double* gw, const uint32_t* order, const uint32_t* skey, int kind_shift,
-                           long long nh, double eta, double* g0, double* g1) {
+                           long long nh, double eta, double* g0, double* g1, const AdjScratch& S) {
   const unsigned g = (unsigned)((nh + ADJ_T * ADJ_R - 1) / (ADJ_T * ADJ_R));
   k_toi_adjoint_k1<D, VF><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, order, skey, kind_shift, nh,
-                                                      c->d_rec, eta, g0, g1, c->d_adj_q, (long long)c->cap_adj_q,
-                                                      c->d_adj_n);
+                                                      c->d_rec, eta, g0, g1, S.queue, S.qcap, S.qn);
+}
+
+// Bits of the vertex index in the visiting-order sort key (kind above them).
+static int adj_vertex_bits(int nv) {
+  int vbits = 1;
+  while (vbits < 31 && (1ll << vbits) < (long long)nv) ++vbits;
+  return vbits;
+}
+
+// The adjoint of the last traverse's hits into device gradients g0, g1
+// (dim*nv each, zeroed here) from device weights gw; S.keys null: no sort.
+// Synchronises the context's stream (reads the multi-step count).
+static int toi_grad_run(Ctx* c, const double* gw, double* g0, double* g1, const AdjScratch& S) {
+  const long long nh = c->counts.hits;
+  launch_fill_u64(reinterpret_cast<unsigned long long*>(g0), (long long)c->dim * c->nv, 0ull, c->stream);
+  launch_fill_u64(reinterpret_cast<unsigned long long*>(g1), (long long)c->dim * c->nv, 0ull, c->stream);
+  c->adj_multi = 0;
+  if (nh == 0) return MCLG_OK;
+  launch_fill_u64(S.qn, 1, 0ull, c->stream);
+  // Visit hits sorted by (kind, first vertex) so that each block's LDS table
+  // merges most of its contributions; the caller's hit order is untouched.
+  const int kind_shift = adj_vertex_bits(c->nv);
+  const uint32_t *order = nullptr, *skey = nullptr;
+  if (S.keys) {
+    k_adj_keys<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->dim,
+                                                                     kind_shift, S.keys, S.vals);
+    launch_radix_sort(S.keys, S.vals, S.ktmp, S.vtmp, S.hist, (int)nh, kind_shift + 1, c->stream);
+    order = S.vals;
+    skey = S.keys;
+  }
+  if (c->dim == 3) {
+    launch_adjoint<3, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1, S);
+    launch_adjoint<3, false>(c, gw, order, skey, kind_shift, nh, c->opt.ee_ccd_eta, g0, g1, S);
+  } else {
+    launch_adjoint<2, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1, S);
+  }
+  if (hipGetLastError() != hipSuccess) return c->fail(MCLG_ERR_HIP, "toi_grad: kernel launch failed");
+  // the context's pinned counter mirror receives the multi-step count
+  if (hipMemcpyAsync(c->h_counters, S.qn, sizeof(unsigned long long), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return c->fail(MCLG_ERR_HIP, "toi_grad: reading the multi-step count failed");
+  const unsigned long long nq = c->h_counters[0];
+  if (nq > 0) {
+    const bool all = nq > (unsigned long long)S.qcap;  // queue overflow: rescan every hit
+    const long long n = all ? nh : (long long)nq;
+    const long long* q = all ? nullptr : S.queue;
+    const unsigned g = (unsigned)((n + ADJ_T - 1) / ADJ_T);
+    if (c->dim == 3) {
+      k_toi_adjoint<3, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                         c->opt.vf_ccd_eta, g0, g1);
+      k_toi_adjoint<3, false><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                          c->opt.ee_ccd_eta, g0, g1);
+    } else {
+      k_toi_adjoint<2, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
+                                                         c->opt.vf_ccd_eta, g0, g1);
+    }
+    if (hipGetLastError() != hipSuccess) return c->fail(MCLG_ERR_HIP, "toi_grad: kernel launch failed");
+  }
+  c->adj_multi = (long long)nq;
+  return MCLG_OK;
+}
+
+// Preconditions shared by both entry points: hits of an ACCD traverse on the
+// current tree.
+static int toi_grad_check(Ctx* c) {
+  if (c->hits_one_sided)
+    return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad: the TOI adjoint covers ACCD only, not the vf_one_sided CTCD path");
+  if (c->hits_gen != c->tree_gen)
+    return c->fail(MCLG_ERR_STATE,
+                   "toi_grad: no traverse since the last update or refit (the hits would be paired with the new "
+                   "vertex positions); traverse again first");
+  return MCLG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free scratch: one static array per GPU (each device holds its own
+// copy for the life of the process).  Host code must use the address from
+// hipGetSymbolAddress, never the array's name (a host-side stand-in).
+// ---------------------------------------------------------------------------
+constexpr long long kNoAllocHits = 1ll << 20;  // MCLG_NOALLOC_MAX_HITS
+static_assert(kNoAllocHits == MCLG_NOALLOC_MAX_HITS, "header and scratch agree");
+constexpr size_t kNaKeys = sizeof(uint32_t) * kNoAllocHits;  // keys, vals, ktmp, vtmp: one each
+constexpr size_t kNaHist = sizeof(uint32_t) * ((256 * ((kNoAllocHits + 4095) / 4096) * 2 + 255) / 256 * 256);
+constexpr size_t kNaQueue = sizeof(long long) * kNoAllocHits;  // multi-step queue: room for every hit
+constexpr size_t kNaBytes = 4 * kNaKeys + kNaHist + kNaQueue + 256;
+
+__device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kMaxDevices = 64;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch's device address on the current device (looked up once per device).
+static int noalloc_scratch(Ctx* c, AdjScratch& S) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev != c->device)
+    return c->fail(MCLG_ERR_STATE, "toi_grad_noalloc: the context's device %d is not current", c->device);
+  if (dev < 0 || dev >= kMaxDevices) return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad_noalloc: device %d", dev);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return c->fail(MCLG_ERR_HIP, "toi_grad_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  unsigned char* base = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  if (radix_hist_words((int)kNoAllocHits) * sizeof(uint32_t) > kNaHist)
+    return c->fail(MCLG_ERR_STATE, "toi_grad_noalloc: histogram scratch too small");
+  S.keys = reinterpret_cast<uint32_t*>(base);
+  S.vals = reinterpret_cast<uint32_t*>(base + kNaKeys);
+  S.ktmp = reinterpret_cast<uint32_t*>(base + 2 * kNaKeys);
+  S.vtmp = reinterpret_cast<uint32_t*>(base + 3 * kNaKeys);
+  S.hist = reinterpret_cast<uint32_t*>(base + 4 * kNaKeys);
+  S.queue = reinterpret_cast<long long*>(base + 4 * kNaKeys + kNaHist);
+  S.qcap = kNoAllocHits;
+  S.qn = reinterpret_cast<unsigned long long*>(base + 4 * kNaKeys + kNaHist + kNaQueue);
+  return MCLG_OK;
 }
 
 }  // namespace mclg
@@ -514,8 +641,7 @@ using namespace mclg;
 extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* dV1, int device_ptrs) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c || !w || !dV0 || !dV1) return MCLG_ERR_ARG;
-  if (c->hits_one_sided)
-    return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad: the TOI adjoint covers ACCD only, not the vf_one_sided CTCD path");
+  CHECK(toi_grad_check(c));
   const long long nh = c->counts.hits;
   const size_t gbytes = sizeof(double) * c->dim * (size_t)c->nv;
   double *gw = nullptr, *g0 = nullptr, *g1 = nullptr;
@@ -531,10 +657,7 @@ extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* 
     g1 = c->d_adj_g + (size_t)c->dim * c->nv;
     if (nh > 0) HIPCHECK(c, hipMemcpyAsync(gw, w, sizeof(double) * nh, hipMemcpyHostToDevice, c->stream));
   }
-  c->timing = true;
-  c->mark(14);
-  launch_fill_u64(reinterpret_cast<unsigned long long*>(g0), (long long)c->dim * c->nv, 0ull, c->stream);
-  launch_fill_u64(reinterpret_cast<unsigned long long*>(g1), (long long)c->dim * c->nv, 0ull, c->stream);
+  AdjScratch S;
   if (nh > 0) {
     CHECK(c->grow((void**)&c->d_adj_n, &c->cap_adj_n, sizeof(unsigned long long), "adjoint"));
     if (c->cap_adj_q == 0) {
@@ -542,14 +665,10 @@ extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* 
       CHECK(c->grow((void**)&c->d_adj_q, &c->cap_adj_q_bytes, sizeof(long long) * kAdjQueue, "adjoint"));
       c->cap_adj_q = kAdjQueue;
     }
-    launch_fill_u64(c->d_adj_n, 1, 0ull, c->stream);
-    // Visit hits sorted by (kind, first vertex) so that each block's LDS table
-    // merges most of its contributions; the caller's hit order is untouched.
-    int vbits = 1;
-    while (vbits < 31 && (1ll << vbits) < (long long)c->nv) ++vbits;
-    const int kind_shift = vbits;
-    const uint32_t *order = nullptr, *skey = nullptr;
-    if (nh <= (long long)INT32_MAX && vbits <= 30) {
+    S.queue = c->d_adj_q;
+    S.qcap = (long long)c->cap_adj_q;
+    S.qn = c->d_adj_n;
+    if (nh <= (long long)INT32_MAX && adj_vertex_bits(c->nv) <= 30) {
       const size_t b4 = sizeof(uint32_t) * (size_t)nh;
       CHECK(c->grow((void**)&c->d_adj_keys, &c->cap_adj_keys, b4, "adjoint"));
       CHECK(c->grow((void**)&c->d_adj_vals, &c->cap_adj_vals, b4, "adjoint"));
@@ -557,46 +676,43 @@ extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* 
       CHECK(c->grow((void**)&c->d_adj_vtmp, &c->cap_adj_vtmp, b4, "adjoint"));
       CHECK(c->grow((void**)&c->d_adj_hist, &c->cap_adj_hist, sizeof(uint32_t) * radix_hist_words((int)nh),
                     "adjoint"));
-      k_adj_keys<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->dim,
-                                                                       kind_shift, c->d_adj_keys, c->d_adj_vals);
-      launch_radix_sort(c->d_adj_keys, c->d_adj_vals, c->d_adj_ktmp, c->d_adj_vtmp, c->d_adj_hist, (int)nh,
-                        vbits + 1, c->stream);
-      order = c->d_adj_vals;
-      skey = c->d_adj_keys;
+      S.keys = c->d_adj_keys; S.vals = c->d_adj_vals; S.ktmp = c->d_adj_ktmp; S.vtmp = c->d_adj_vtmp;
+      S.hist = c->d_adj_hist;
     }
-    if (c->dim == 3) {
-      launch_adjoint<3, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1);
-      launch_adjoint<3, false>(c, gw, order, skey, kind_shift, nh, c->opt.ee_ccd_eta, g0, g1);
-    } else {
-      launch_adjoint<2, true>(c, gw, order, skey, kind_shift, nh, c->opt.vf_ccd_eta, g0, g1);
-    }
-    HIPCHECK(c, hipGetLastError());
-    unsigned long long nq = 0;
-    HIPCHECK(c, hipMemcpyAsync(&nq, c->d_adj_n, sizeof(nq), hipMemcpyDeviceToHost, c->stream));
-    HIPCHECK(c, hipStreamSynchronize(c->stream));
-    if (nq > 0) {
-      const bool all = nq > (unsigned long long)c->cap_adj_q;
-      const long long n = all ? nh : (long long)nq;
-      const long long* q = all ? nullptr : c->d_adj_q;
-      const unsigned g = (unsigned)((n + ADJ_T - 1) / ADJ_T);
-      if (c->dim == 3) {
-        k_toi_adjoint<3, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
-                                                           c->opt.vf_ccd_eta, g0, g1);
-        k_toi_adjoint<3, false><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
-                                                            c->opt.ee_ccd_eta, g0, g1);
-      } else {
-        k_toi_adjoint<2, true><<<g, ADJ_T, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, gw, q, n, c->d_rec,
-                                                           c->opt.vf_ccd_eta, g0, g1);
-      }
-      HIPCHECK(c, hipGetLastError());
-    }
-    c->adj_multi = (long long)nq;
   }
+  c->timing = true;
+  c->mark(14);
+  CHECK(toi_grad_run(c, gw, g0, g1, S));
   c->mark(15);
   if (!device_ptrs) {
     HIPCHECK(c, hipMemcpyAsync(dV0, g0, gbytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipMemcpyAsync(dV1, g1, gbytes, hipMemcpyDeviceToHost, c->stream));
   }
+  HIPCHECK(c, hipStreamSynchronize(c->stream));
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, c->ev[14], c->ev[15]) == hipSuccess) c->times[10] = ms;
+  return MCLG_OK;
+}
+
+// The same adjoint with every scratch buffer in the static array above: no
+// device or host allocation.  Device pointers only.
+extern "C" int mclg_toi_grad_noalloc(mclg_ctx* h, const double* w, double* dV0, double* dV1) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c || !w || !dV0 || !dV1) return MCLG_ERR_ARG;
+  CHECK(toi_grad_check(c));
+  const long long nh = c->counts.hits;
+  if (nh > kNoAllocHits)
+    return c->fail(MCLG_ERR_CAPACITY,
+                   "toi_grad_noalloc: %lld hits exceed the static scratch's maximum of %lld; use mclg_toi_grad",
+                   nh, kNoAllocHits);
+  if (adj_vertex_bits(c->nv) > 30)
+    return c->fail(MCLG_ERR_CAPACITY, "toi_grad_noalloc: %d vertices exceed the sort key's 2^30", c->nv);
+  AdjScratch S;
+  CHECK(noalloc_scratch(c, S));
+  c->timing = true;
+  c->mark(14);
+  CHECK(toi_grad_run(c, w, dV0, dV1, S));
+  c->mark(15);
   HIPCHECK(c, hipStreamSynchronize(c->stream));
   float ms = 0.f;
   if (hipEventElapsedTime(&ms, c->ev[14], c->ev[15]) == hipSuccess) c->times[10] = ms;

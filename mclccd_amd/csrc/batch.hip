@@ -505,7 +505,10 @@ int mclg_batch_autotune(mclg_batch* h, const char* cache_path, int reps) {
   std::string arch = prop.gcnArchName;
   arch = arch.substr(0, arch.find(':'));
   std::ostringstream key;
-  key << arch << " dim=" << c->dim << " scenes=" << b->nscenes << " prims=" << b->np << " graph=" << b->use_graph;
+  // the format tag changes whenever the parameters' meaning or ranges do, so
+  // entries written by another version are never reused
+  key << "v2 " << arch << " dim=" << c->dim << " scenes=" << b->nscenes << " prims=" << b->np
+      << " graph=" << b->use_graph;
   if (cache_path) {
     std::ifstream in(cache_path);
     std::string line;
@@ -513,7 +516,9 @@ int mclg_batch_autotune(mclg_batch* h, const char* cache_path, int reps) {
       const size_t bar = line.find('|');
       if (bar == std::string::npos || line.substr(0, bar) != key.str()) continue;
       int tw, bpc, ch;
-      if (std::sscanf(line.c_str() + bar + 1, "%d %d %d", &tw, &bpc, &ch) == 3) {
+      // a hand-edited or damaged entry is ignored (and re-tuned), never clamped silently
+      if (std::sscanf(line.c_str() + bar + 1, "%d %d %d", &tw, &bpc, &ch) == 3 && tw >= 1 &&
+          tw <= kTraverseMaxWaves && bpc >= 1 && ch >= 1) {
         c->tr_waves = tw;
         c->blocks_per_cu = bpc;
         c->chunk = ch;

@@ -174,10 +174,12 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   if (refit && !(have_tree && np_ == np && nv_ == nv && pdim_ == pdim))
     return fail(MCLG_ERR_STATE, "refit: needs a previous update with the same nv, np and pdim (have %d/%d/%d, got %d/%d/%d)",
                 have_tree ? nv : 0, have_tree ? np : 0, have_tree ? pdim : 0, nv_, np_, pdim_);
+  ++tree_gen;  // from here on the last traverse's hits no longer match the vertex records
   if (np_ == 0) {  // BVHTree.cpp:29-34
     np = 0;
     have_tree = false;
     rank_set = false;
+    kd_valid = false;
     return MCLG_OK;
   }
   if (nv_ == 0 || !V0 || !V1 || !P) return fail(MCLG_ERR_ARG, "update: null input");
@@ -233,6 +235,14 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_cbounds, &cap_cbounds, sizeof(unsigned long long) * 6, "bounds"));
   CHECK(grow((void**)&d_partial, &cap_partial, sizeof(double) * 6 * (size_t)leaf_box_blocks(np), "bounds"));
   launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, SceneArgs(), stream);
+  // the reference's KdBVH order (KdBVH.hpp:192-222) when asked for, or for a
+  // serial traversal whose callback order must be the reference's
+  if (orient == MCLG_ORIENT_KDBVH || !opt.threaded) {
+    CHECK(kdbvh_from_leaves());
+  } else if (kd_valid) {  // ranks of an earlier KdBVH update no longer apply
+    rank_set = false;
+    kd_valid = false;
+  }
   mark(3);
   // Morton + sort
   CHECK(grow((void**)&d_keys, &cap_keys, sizeof(uint32_t) * (size_t)np, "keys"));
@@ -280,10 +290,15 @@ int Ctx::read_counters() {
 // BVHTree.cpp:129-159 (+ collide / get_candidates / narrow phase / discrete)
 int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned flags) {
   counts = mclg_counts{0, 0, 0, 0, 0, 0, 0};
+  hits_gen = kNoGen;
   hits_one_sided = opt.vf_one_sided != 0;
   keep_cand = (flags & MCLG_KEEP_CANDIDATES) != 0;
   keep_pairs = (flags & MCLG_KEEP_LEAF_PAIRS) != 0;
-  if (!have_tree || np <= 1) return MCLG_OK;  // BVHTree.cpp:131-132
+  check_order = (flags & MCLG_CHECK_ORDER) != 0;
+  if (!have_tree || np <= 1) {  // BVHTree.cpp:131-132
+    hits_gen = tree_gen;
+    return MCLG_OK;
+  }
   if (pdim != dim) return fail(MCLG_ERR_ARG, "traverse: primitives must have pdim == dim (%d != %d)", pdim, dim);
   if (!V0 || !V1 || !P) return fail(MCLG_ERR_ARG, "traverse: null input");
   timing = true;
@@ -319,10 +334,12 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     if (vf_cap == 0) vf_cap = std::max<unsigned long long>(1024, (unsigned long long)(1.5 * npairs));
     if (ee_cap == 0 && dim == 3) ee_cap = std::max<unsigned long long>(1024, (unsigned long long)(2.5 * npairs));
     if (disc_cap == 0) disc_cap = 1024;
+    if (odep_cap == 0) odep_cap = 256;
     for (int attempt = 0;; ++attempt) {
       CHECK(grow((void**)&d_vfq, &cap_vfq, sizeof(int4) * vf_cap, "VF stencils"));
       if (dim == 3) CHECK(grow((void**)&d_eeq, &cap_eeq, sizeof(int4) * ee_cap, "EE stencils"));
       CHECK(grow((void**)&d_disc, &cap_disc, sizeof(int2) * disc_cap, "discrete"));
+      if (check_order) CHECK(grow((void**)&d_odep, &cap_odep, sizeof(int2) * odep_cap, "order-dependent pairs"));
       launch_fill_u64(d_counters + C_CAND_VF, C_COUNT - C_CAND_VF, 0ull, stream);
       ExpandLaunch E;
       E.pairs = d_pairs; E.npairs_dev = d_counters + C_LEAF_PAIRS; E.pair_cap = pair_cap; E.npairs_hint = npairs;
@@ -335,6 +352,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       E.discrete = opt.discrete; E.continuous = opt.continuous;
       E.vfq = d_vfq; E.vf_cap = vf_cap; E.eeq = d_eeq; E.ee_cap = dim == 3 ? ee_cap : 0;
       E.disc = d_disc; E.disc_cap = disc_cap; E.counters = d_counters; E.num_cus = num_cus;
+      E.odep = check_order ? d_odep : nullptr; E.odep_cap = odep_cap;
       mark(10);
       launch_expand(dim, E, stream);
       mark(11);
@@ -344,6 +362,10 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       if (h_counters[C_CAND_VF] > vf_cap) { vf_cap = h_counters[C_CAND_VF] + h_counters[C_CAND_VF] / 4; again = true; }
       if (h_counters[C_CAND_EE] > ee_cap) { ee_cap = h_counters[C_CAND_EE] + h_counters[C_CAND_EE] / 4; again = true; }
       if (h_counters[C_DISCRETE] > disc_cap) { disc_cap = h_counters[C_DISCRETE] + h_counters[C_DISCRETE] / 4; again = true; }
+      if (check_order && h_counters[C_ORDER_DEP] > odep_cap) {
+        odep_cap = h_counters[C_ORDER_DEP] + h_counters[C_ORDER_DEP] / 4;
+        again = true;
+      }
       if (!again) break;
       if (attempt > 4) return fail(MCLG_ERR_STATE, "candidate generation: sizes did not converge");
     }
@@ -352,6 +374,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
     counts.candidates = n_vf + n_ee;
     counts.vf_candidates = n_vf;
     counts.discrete = (int64_t)h_counters[C_DISCRETE];
+    counts.order_dependent = check_order ? (int64_t)h_counters[C_ORDER_DEP] : 0;
     // FP64 ACCD over the two queues (BVHTree.cpp:428-461)
     if (opt.continuous && n_vf + n_ee > 0) {
       if (hit_cap == 0) hit_cap = std::max<unsigned long long>(1024, (unsigned long long)(n_vf + n_ee) / 2);
@@ -452,6 +475,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       times[12] = ms;
     }
   }
+  hits_gen = tree_gen;
   return MCLG_OK;
 }
 
@@ -459,7 +483,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
                   d_vtmp, d_hist, d_sboxes, d_face, d_v1, d_cp, d_root, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
-                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
+                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_odep, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
@@ -573,6 +597,16 @@ int mclg_copy_discrete(mclg_ctx* h, int* pairs2, int64_t cap) {
   return MCLG_OK;
 }
 
+int mclg_copy_order_dependent(mclg_ctx* h, int* pairs2, int64_t cap) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  if (!c->check_order) return c->fail(MCLG_ERR_STATE, "copy_order_dependent: the traverse had no MCLG_CHECK_ORDER");
+  const int64_t n = c->counts.order_dependent;
+  if (cap < n) return c->fail(MCLG_ERR_CAPACITY, "copy_order_dependent: cap < count");
+  if (pairs2) CHECK(copy_out(c, pairs2, c->d_odep, sizeof(int2) * (size_t)n));
+  return MCLG_OK;
+}
+
 int mclg_copy_candidates(mclg_ctx* h, int* sten4, unsigned char* is_vf, double* toi, int64_t cap) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return MCLG_ERR_ARG;
@@ -652,6 +686,7 @@ int mclg_get_device_results(const mclg_ctx* h, mclg_device_results* out) {
 int mclg_set_rank(mclg_ctx* h, const int* rank, int np) {
   Ctx* c = reinterpret_cast<Ctx*>(h);
   if (!c) return MCLG_ERR_ARG;
+  c->kd_valid = false;  // an explicit rank replaces the library's KdBVH order until the next update
   if (!rank) { c->rank_set = false; return MCLG_OK; }
   if (np != c->np || np <= 0) return c->fail(MCLG_ERR_ARG, "set_rank: np %d != tree size %d", np, c->np);
   CHECK(c->grow((void**)&c->d_rank, &c->cap_rank, sizeof(int) * (size_t)np, "rank"));

@@ -90,7 +90,8 @@ enum Counter : int {
   C_DIST_EVALS,
   C_ERRORS,     // kernels that detected an inconsistent state
   C_EE_EVALS,   // distance evaluations of edge-edge stencils (also counted in C_DIST_EVALS)
-  C_COUNT = 8
+  C_ORDER_DEP,  // leaf pairs whose discrete result depends on the argument order (when checked)
+  C_COUNT = 9
 };
 
 // ---- wave helpers ---------------------------------------------------------

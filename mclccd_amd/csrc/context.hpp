@@ -2,6 +2,7 @@
 #pragma once
 #include <cstdarg>
 #include <string>
+#include <vector>
 
 #include "../../include/mclg.h"
 #include "common.hpp"
@@ -36,8 +37,20 @@ struct Ctx {
   int nv = 0, np = 0, pdim = 0;
   bool have_tree = false;
   bool rank_set = false;
+  // orientation: MCLG_ORIENT_INDEX (primitive index, or a rank from
+  // mclg_set_rank) or MCLG_ORIENT_KDBVH (the reference's KdBVH order, rebuilt
+  // on the host at every update; also used while opt.threaded == 0)
+  int orient = MCLG_ORIENT_INDEX;
+  bool kd_valid = false;           // kd_rank / kd_gapnode describe the current leaves
+  std::vector<int> kd_rank;        // in-order position of each primitive
+  std::vector<int> kd_gapnode;     // frontlist index of the node splitting positions g, g+1
   bool keep_cand = false, keep_pairs = false;
   bool hits_one_sided = false;  // the last traverse used CTCD for VF (no TOI adjoint for it)
+  // generations: tree_gen counts updates and refits; hits_gen is the tree_gen
+  // the last complete traverse ran on (kNoGen: none since the last update),
+  // so the adjoint can refuse hits whose vertex records were replaced
+  static constexpr unsigned long long kNoGen = ~0ull;
+  unsigned long long tree_gen = 0, hits_gen = kNoGen;
   mclg_counts counts{0, 0, 0, 0, 0};
 
   // device buffers (grown on demand; cap in bytes)
@@ -82,6 +95,9 @@ struct Ctx {
   unsigned long long hit_cap = 0;
   int2* d_disc = nullptr; size_t cap_disc = 0;
   unsigned long long disc_cap = 0;
+  int2* d_odep = nullptr; size_t cap_odep = 0;  // order-dependent discrete pairs (MCLG_CHECK_ORDER)
+  unsigned long long odep_cap = 0;
+  bool check_order = false;
   int4* d_vfq = nullptr; size_t cap_vfq = 0;
   int4* d_eeq = nullptr; size_t cap_eeq = 0;
   unsigned long long vf_cap = 0, ee_cap = 0;
@@ -122,6 +138,8 @@ struct Ctx {
   int update(const double* V0, const double* V1, int nv, const int* P, int np, int pdim, const int* active,
              bool refit = false);
   int traverse(const double* V0, const double* V1, const int* P, unsigned flags);
+  int kdbvh_from_leaves();  // the reference's KdBVH order of the current leaves -> rank
+  int events(bool cand, int64_t* out, int64_t* group, int64_t cap, int64_t* nout);
   FaceArrays faces() { return face_arrays(d_face, (size_t)np, d_v1); }
 };
 

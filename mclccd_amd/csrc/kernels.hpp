@@ -87,6 +87,8 @@ struct ExpandLaunch {
   unsigned long long ee_cap;
   int2* disc;
   unsigned long long disc_cap;
+  int2* odep = nullptr;  // pairs whose discrete result depends on the argument order (null: not checked)
+  unsigned long long odep_cap = 0;
   unsigned long long* counters;
   int num_cus;
   int blocks_per_cu;

@@ -430,6 +430,8 @@ struct ExpandArgs {
   unsigned long long ee_cap;
   int2* disc;
   unsigned long long disc_cap;
+  int2* odep;  // non-null: pairs whose discrete result depends on the argument order
+  unsigned long long odep_cap;
   unsigned long long* counters;
   // per 64-pair chunk: VF / EE candidate counts, then (scanned) output offsets;
   // entry nchunks_cap holds the totals after the scan
@@ -602,11 +604,15 @@ __device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a
 
 // Pass 2: the discrete test and the candidates, written at the scanned offsets
 // (each chunk's stencils in lane order: the queues follow the pair list).
-template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
+// ODEP: also evaluate the discrete test with the two primitives swapped and
+// list the pairs whose result changes (its staging buffer only exists then,
+// so the default kernel keeps the LDS size that allows four blocks per CU).
+template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
+  __shared__ int2 s_o[EX_WAVES][ODEP ? EX_DSTAGE : 1];
   __shared__ int4 s_st[EX_WAVES][kWave * 9];  // at most 9 EE (6 VF) stencils per pair
   const int w = threadIdx.x / kWave, lane = lane_id();
-  int nd = 0;  // staged discrete records (wave-uniform)
+  int nd = 0, no = 0;  // staged discrete / order-dependent records (wave-uniform)
   const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
   const long long nchunks = (npairs + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * EX_WAVES * A.chunk;
@@ -623,21 +629,27 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
       for (int x = 0; x < D; ++x)
 #pragma unroll
         for (int y = 0; y < D; ++y) shared |= ha.p[x] == hb.p[y];
-      int dh = 0;
+      int dh = 0, dh_swap = 0;
       if (valid && !shared) {
         double a[3 * D], b[3 * D];
         face_v1<D>(A.faces.v1, ha, a);
         face_v1<D>(A.faces.v1, hb, b);
         if constexpr (D == 3) {
-          dh = tri_tri3(d3{a[0], a[1], a[2]}, d3{a[3], a[4], a[5]}, d3{a[6], a[7], a[8]}, d3{b[0], b[1], b[2]},
-                        d3{b[3], b[4], b[5]}, d3{b[6], b[7], b[8]});
+          const d3 p1{a[0], a[1], a[2]}, q1{a[3], a[4], a[5]}, r1{a[6], a[7], a[8]};
+          const d3 p2{b[0], b[1], b[2]}, q2{b[3], b[4], b[5]}, r2{b[6], b[7], b[8]};
+          dh = tri_tri3(p1, q1, r1, p2, q2, r2);
+          if (ODEP) dh_swap = tri_tri3(p2, q2, r2, p1, q1, r1);
         } else {
-          dh = edge_edge2(d2{a[0], a[1]}, d2{a[2], a[3]}, d2{b[0], b[1]}, d2{b[2], b[3]});
+          const d2 p0{a[0], a[1]}, p1{a[2], a[3]}, q0{b[0], b[1]}, q1{b[2], b[3]};
+          dh = edge_edge2(p0, p1, q0, q1);
+          if (ODEP) dh_swap = edge_edge2(q0, q1, p0, p1);
         }
       }
       int2 rec = make_int2(0, 0);
-      if (dh) rec = make_int2(A.faces.prim[pa], A.faces.prim[pb]);
+      if (dh || dh_swap) rec = make_int2(A.faces.prim[pa], A.faces.prim[pb]);
       stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, rec, A.disc, A.disc_cap, A.counters + C_DISCRETE);
+      if constexpr (ODEP)
+        stage_push<int2, EX_DSTAGE>(s_o[w], no, dh != dh_swap, rec, A.odep, A.odep_cap, A.counters + C_ORDER_DEP);
     }
     if (!A.continuous) continue;
     uint32_t vfm = 0, eem = 0;
@@ -648,6 +660,7 @@ template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(Ex
       write_stencils<D, false>(s_st[w], eem, x0, x1, x2, y0, y1, y2, A.eeq, A.off_ee[ch], A.ee_cap);
   }
   if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
+  if (ODEP && no) stage_flush(s_o[w], no, A.odep, A.odep_cap, A.counters + C_ORDER_DEP);
 }
 
 // ---------------------------------------------------------------------------
@@ -858,6 +871,7 @@ void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
   A.discrete = L.discrete; A.continuous = L.continuous;
   A.vfq = L.vfq; A.vf_cap = L.vf_cap; A.eeq = L.eeq; A.ee_cap = L.ee_cap;
   A.disc = L.disc; A.disc_cap = L.disc_cap; A.counters = L.counters;
+  A.odep = L.odep; A.odep_cap = L.odep_cap;
   A.off_vf = L.off_vf; A.off_ee = L.off_ee; A.nchunks_cap = expand_chunks(L.pair_cap);
   A.chunk = L.chunk < 1 ? 1 : L.chunk;
   const long long items = L.npairs_hint >= 0 ? L.npairs_hint : (long long)L.pair_cap;
@@ -871,8 +885,14 @@ void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
     launch_scan_u64(A.off_ee, A.nchunks_cap + 1, L.scan_part, s);
     k_expand_totals<<<1, 1, 0, s>>>(A.off_vf, A.off_ee, A.nchunks_cap, A.counters);
   }
-  if (dim == 3) k_expand<3><<<g, EX_WAVES * kWave, 0, s>>>(A);
-  else k_expand<2><<<g, EX_WAVES * kWave, 0, s>>>(A);
+  const bool od = A.odep != nullptr && L.discrete;
+  if (dim == 3) {
+    if (od) k_expand<3, true><<<g, EX_WAVES * kWave, 0, s>>>(A);
+    else k_expand<3, false><<<g, EX_WAVES * kWave, 0, s>>>(A);
+  } else {
+    if (od) k_expand<2, true><<<g, EX_WAVES * kWave, 0, s>>>(A);
+    else k_expand<2, false><<<g, EX_WAVES * kWave, 0, s>>>(A);
+  }
 }
 
 long long accd_chunks(unsigned long long cap) { return (long long)((cap + kWave - 1) / kWave); }

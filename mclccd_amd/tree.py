@@ -8,14 +8,19 @@ continuous, vf_one_sided, ee_robust), ``update(V0, V1, P, active)``,
 ``narrow_phase(sten, is_vf) -> toi``.
 
 The GPU computes every result of a traverse; the callbacks are then replayed
-on the calling thread, serially (the reference calls them concurrently from TBB
-workers, so any serial order is one it can produce):
-  1. continuous: for each candidate stencil, filter_pair (if set) may skip it;
-     the toi is narrow_phase(sten, is_vf) when that callback is set, else the
-     GPU's ACCD result; toi >= 0 calls append_pair.  Without filter_pair and
+on the calling thread, serially, leaf pair by leaf pair in the order of
+mclg_copy_events (with options.threaded False, or the KdBVH orientation, the
+reference's own serial order, BVHTree.cpp:152-157; its threaded run calls them
+concurrently from TBB workers, so any serial order is one it can produce).
+Per leaf pair, as collide does (BVHTree.cpp:252-283):
+  1. continuous: filter_pair (if set) on each of the pair's candidates in
+     get_candidates order, then for each one kept the toi —
+     narrow_phase(sten, is_vf) when that callback is set, else the GPU's ACCD
+     result — and append_pair when toi >= 0.  Without filter_pair and
      narrow_phase only the hits are read back.
-  2. discrete: append_discrete(p0, p1) per discrete hit; a True return stops
-     the replay (the reference's stop flag, BVHTree.cpp:236-239,280-281).
+  2. discrete: append_discrete(p0, p1) for the pair's discrete hit; a True
+     return stops the replay, so no later pair delivers anything (the
+     reference's stop flag, BVHTree.cpp:236-239,280-281).
 
 ``traverse(visitor)`` (BVHTree.hpp:96) walks a host copy of the tree with a
 ``BVHTraverse``-like object: ``intersect_volume(box) -> bool`` (True: go down)
@@ -108,7 +113,7 @@ class BVHTree:
         self._h = h
         self.nv = 0
         self.np = 0
-        self.serial = 0  # counts traversals (the adjoint refers to the last one)
+        self.serial = 0  # counts updates and traversals (the adjoint refers to the last traverse)
         self._mirror = None  # host copy of the tree for visitor walks (per update)
 
     def __del__(self):
@@ -166,6 +171,7 @@ class BVHTree:
             nv, npr, pdim = V0.shape[0], P.shape[0], P.shape[1]
             fn = lib.mclg_update_refit if refit else lib.mclg_update
             rc = fn(self._h, _ptr(V0), _ptr(V1), nv, _ptr(P), npr, pdim, _ptr(active))
+        self.serial += 1  # the last traverse's hits no longer match the vertex records
         self._check(rc)
         self._pull_options()
         self.nv, self.np = nv, npr
@@ -178,33 +184,59 @@ class BVHTree:
             return self.visit(V0)
         if len(V0) != len(V1):
             return
-        need_cand = self.narrow_phase is not None or self.filter_pair is not None
-        self.run(V0, V1, P, keep_candidates=need_cand)
-        if self.options.continuous and (self.append_pair is not None or need_cand):
-            if need_cand:
-                sten, is_vf, toi = self.candidates()
-                for i in range(len(sten)):
-                    s, v = tuple(int(x) for x in sten[i]), bool(is_vf[i])
-                    if self.filter_pair is not None and self.filter_pair(s, v):
-                        continue
+        need_cand = self.options.continuous and (self.narrow_phase is not None or self.filter_pair is not None)
+        cont = self.options.continuous and (self.append_pair is not None or need_cand)
+        disc = self.options.discrete and self.append_discrete is not None
+        self.run(V0, V1, P, keep_candidates=need_cand, keep_leaf_pairs=cont or disc)
+        if cont or disc:
+            self._replay(need_cand, cont, disc)
+
+    def _replay(self, need_cand, cont, disc):
+        ev, grp = self.events(over_candidates=need_cand)
+        if cont:
+            sten, is_vf, toi = self.candidates() if need_cand else self.hits()
+        dp = self.discrete() if disc else None
+        k, n = 0, len(ev)
+        while k < n:  # one leaf pair per iteration: its records, then its discrete hit
+            j = k
+            while j < n and grp[j] == grp[k]:
+                j += 1
+            recs = [int(e) for e in ev[k:j] if e >= 0]
+            dis = [int(-1 - e) for e in ev[k:j] if e < 0]
+            k = j
+            if cont:
+                items = [(tuple(int(x) for x in sten[i]), bool(is_vf[i]), i) for i in recs]
+                if self.filter_pair is not None:  # get_candidates filters all of the pair's stencils first
+                    items = [it for it in items if not self.filter_pair(it[0], it[1])]
+                for s, v, i in items:
                     t = self.narrow_phase(s, v) if self.narrow_phase is not None else float(toi[i])
                     if t >= 0 and self.append_pair is not None:
                         self.append_pair(s, v, t)
-            else:
-                sten, is_vf, toi = self.hits()
-                for i in range(len(sten)):
-                    self.append_pair(tuple(int(x) for x in sten[i]), bool(is_vf[i]), float(toi[i]))
-        if self.options.discrete and self.append_discrete is not None:
-            for p0, p1 in self.discrete():
-                if self.append_discrete(int(p0), int(p1)):
-                    break
+            if disc:
+                for d in dis:
+                    if self.append_discrete(int(dp[d, 0]), int(dp[d, 1])):
+                        return  # stop: every later collide returns at entry
+
+    def events(self, over_candidates=False):
+        """(events, group) of the last traverse in replay order (mclg_copy_events):
+        events >= 0 index hits() (or candidates()), events < 0 are discrete
+        hit -1-e; group numbers the leaf pair of each event."""
+        n = C.c_int64(0)
+        self._check(lib.mclg_copy_events(self._h, int(over_candidates), None, None, 0, C.byref(n)))
+        ev = np.empty(n.value, np.int64)
+        grp = np.empty(n.value, np.int64)
+        self._check(lib.mclg_copy_events(self._h, int(over_candidates), _ptr(ev), _ptr(grp), n.value, C.byref(n)))
+        return ev, grp
 
     # -- bulk access (no callbacks) --------------------------------------------
-    def run(self, V0, V1, P, keep_candidates=False, keep_leaf_pairs=False):
-        """GPU traverse only; results stay on the device (see hits(), discrete())."""
+    def run(self, V0, V1, P, keep_candidates=False, keep_leaf_pairs=False, check_order=False):
+        """GPU traverse only; results stay on the device (see hits(), discrete()).
+        check_order: also list the pairs whose discrete result depends on the
+        argument order (order_dependent())."""
         self._push_options()
         self.serial += 1
         flags = (_lib.KEEP_CANDIDATES if keep_candidates else 0) | (_lib.KEEP_LEAF_PAIRS if keep_leaf_pairs else 0)
+        flags |= _lib.CHECK_ORDER if check_order else 0
         if _is_device(V0):
             V0, V1 = _dev_check(V0, "torch.float64"), _dev_check(V1, "torch.float64")
             P = _dev_check(P, "torch.int32")
@@ -289,6 +321,14 @@ class BVHTree:
         self._check(lib.mclg_copy_candidates(self._h, _ptr(s), _ptr(v), _ptr(t), n))
         return s, v, t
 
+    def order_dependent(self):
+        """Leaf pairs of the last traverse (run with check_order) whose discrete
+        result changes when the two primitives are swapped."""
+        n = self.counts()["order_dependent"]
+        out = np.empty((n, 2), np.int32)
+        self._check(lib.mclg_copy_order_dependent(self._h, _ptr(out), n))
+        return out
+
     def leaf_pairs(self):
         n = self.counts()["leaf_pairs"]
         out = np.empty((n, 2), np.int32)
@@ -332,6 +372,13 @@ class BVHTree:
         if self._mirror is None:
             self._mirror = _HostTree(self)
         self._mirror.walk(visitor)
+
+    def set_orientation(self, mode: str):
+        """'index' (default: lower primitive index first) or 'kdbvh' (the
+        reference's KdBVH orientation, rebuilt on the host at every update;
+        mclg_set_orientation).  Takes effect at the next update."""
+        m = {"index": _lib.ORIENT_INDEX, "kdbvh": _lib.ORIENT_KDBVH}[mode]
+        self._check(lib.mclg_set_orientation(self._h, m))
 
     def set_rank(self, rank):
         """Orient leaf pairs by rank (the reference's KdBVH in-order position)."""
@@ -427,3 +474,18 @@ class _HostTree:
             for c in ch:
                 if c >= n - 1 and visitor.intersect_object(Leaf(int(self.order[c - (n - 1)]), self._box(c))):
                     return
+
+
+def kdbvh_order(centres):
+    """The reference KdBVH's leaf order for swept-box centres (n x dim), computed
+    on the host by the library (mclg_kdbvh_order; no GPU needed): (rank, gapnode)
+    — rank[i] the in-order position of leaf i, gapnode[g] the frontlist index of
+    the node that splits positions g and g+1."""
+    c = np.ascontiguousarray(centres, np.float64)
+    n, dim = c.shape
+    rank = np.empty(n, np.int32)
+    gap = np.empty(max(n - 1, 0), np.int32)
+    rc = lib.mclg_kdbvh_order(dim, _ptr(c), n, _ptr(rank), _ptr(gap) if n > 1 else None)
+    if rc != _lib.OK:
+        raise _lib.MclgError(rc, "mclg_kdbvh_order: bad arguments")
+    return rank, gap

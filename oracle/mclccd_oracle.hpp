@@ -712,17 +712,30 @@ template <int D> class Tree {
     q.emplace_back(l, r);
   }
 
+  // The order in which collide produces its records: per leaf pair reached
+  // (ordinal pair), its candidates (kind 0, when kept), hits (kind 1) and
+  // discrete hit (kind 2), each an index into that list.  With one thread
+  // (threaded = false) this is the reference's serial callback sequence.
+  struct Event {
+    long long pair;
+    int kind;
+    long long idx;
+  };
+  std::vector<Event> events;
+
   struct Local {
     std::vector<std::pair<int, int>> lp;
     std::vector<Stencil> cand;
     std::vector<Hit> hits;
     std::vector<std::pair<int, int>> disc;
+    std::vector<Event> ev;
+    long long npairs = 0;
     AccdStats st;
   };
 
   // BVHTree.cpp:129-159
   void traverse(const double* V0, const double* V1, const int* P) {
-    leaf_pairs.clear(); candidates.clear(); hits.clear(); discrete.clear(); stats = AccdStats();
+    leaf_pairs.clear(); candidates.clear(); hits.clear(); discrete.clear(); events.clear(); stats = AccdStats();
     if (leaves.size() <= 1) return;
     std::vector<std::pair<Node, Node>> q;
     q.reserve(leaves.size());
@@ -743,6 +756,7 @@ template <int D> class Tree {
       candidates.insert(candidates.end(), L.cand.begin(), L.cand.end());
       hits.insert(hits.end(), L.hits.begin(), L.hits.end());
       discrete.insert(discrete.end(), L.disc.begin(), L.disc.end());
+      if (nt == 1) events = std::move(L.ev);
       stats.dist_evals += L.st.dist_evals;
       stats.iters += L.st.iters;
     }
@@ -754,17 +768,27 @@ template <int D> class Tree {
     if (!boxes_intersect<D>(get_box(l), get_box(r))) return;
     if (l.second && r.second) {
       if (want_leaf_pairs) out.lp.emplace_back(l.first, r.first);
+      const long long pi = out.npairs++;
       if (options.continuous) {
         Stencil cs[15];
         const int nc = get_candidates(l.first, r.first, P, cs);
         for (int c = 0; c < nc; ++c) {
-          if (want_candidates) out.cand.push_back(cs[c]);
+          if (want_candidates) {
+            out.ev.push_back({pi, 0, (long long)out.cand.size()});
+            out.cand.push_back(cs[c]);
+          }
           double toi = narrow_phase(V0, V1, cs[c], &out.st);
-          if (toi >= 0) out.hits.push_back({cs[c], toi});
+          if (toi >= 0) {
+            out.ev.push_back({pi, 1, (long long)out.hits.size()});
+            out.hits.push_back({cs[c], toi});
+          }
         }
       }
       if (options.discrete) {
-        if (discrete_test(V1, P + l.first * D, P + r.first * D)) out.disc.emplace_back(l.first, r.first);
+        if (discrete_test(V1, P + l.first * D, P + r.first * D)) {
+          out.ev.push_back({pi, 2, (long long)out.disc.size()});
+          out.disc.emplace_back(l.first, r.first);
+        }
       }
       return;
     }

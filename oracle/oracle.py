@@ -49,6 +49,8 @@ def lib():
         for f in ("oracle_copy_leaf_pairs", "oracle_copy_discrete", "oracle_copy_rank", "oracle_copy_flags"):
             getattr(L, f).argtypes = [P, P]
         L.oracle_copy_candidates.argtypes = [P, P, P]
+        L.oracle_events.restype = LL
+        L.oracle_events.argtypes = [P, P, P, P]
         L.oracle_copy_hits.argtypes = [P, P, P, P]
         L.oracle_copy_boxes.argtypes = [P, P, P]
         L.oracle_brute_force_pairs.restype = LL
@@ -156,6 +158,20 @@ class Oracle:
         out = np.zeros((self.count(3), 2), np.int32)
         lib().oracle_copy_discrete(self._h, _p(out))
         return out
+
+    def events(self):
+        """(pair, kind, idx) of a single-threaded traverse (threaded=False): the
+        records in the order collide produced them — kind 0 candidate, 1 hit, 2
+        discrete, idx into candidates() / hits() / discrete(), pair the ordinal of
+        the leaf pair reached."""
+        n = int(lib().oracle_events(self._h, None, None, None))
+        if n < 0:
+            raise RuntimeError("oracle events need a single-threaded traverse (threaded=False)")
+        pair = np.zeros(n, np.int64)
+        kind = np.zeros(n, np.int32)
+        idx = np.zeros(n, np.int64)
+        lib().oracle_events(self._h, _p(pair), _p(kind), _p(idx))
+        return pair, kind, idx
 
     def rank(self):
         out = np.zeros(self._P.shape[0], np.int32)

@@ -158,6 +158,24 @@ int oracle_copy_hits(void* h, int* sten, unsigned char* is_vf, double* toi) {
   return 0;
 }
 
+// The serial record sequence of a single-threaded traverse (threaded = false):
+// per event the leaf-pair ordinal, kind (0 candidate, 1 hit, 2 discrete) and
+// index into that list.  Returns the count (outputs may be null), or -1 when
+// the traverse ran on several threads.
+long long oracle_events(void* h, long long* pair, int* kind, long long* idx) {
+  auto* c = static_cast<OracleCtx*>(h);
+  auto cp = [&](auto& t) -> long long {
+    if (t.options.threaded && t.nthreads_eff() > 1) return -1;
+    for (size_t i = 0; i < t.events.size(); ++i) {
+      if (pair) pair[i] = t.events[i].pair;
+      if (kind) kind[i] = t.events[i].kind;
+      if (idx) idx[i] = t.events[i].idx;
+    }
+    return (long long)t.events.size();
+  };
+  return c->dim == 2 ? cp(*c->t2) : cp(*c->t3);
+}
+
 // rank[leaf] = in-order position in the KdBVH (left subtree first)
 int oracle_copy_rank(void* h, int* rank) {
   auto* c = static_cast<OracleCtx*>(h);

@@ -219,6 +219,31 @@ int main(int argc, char** argv) {
     tree.traverse(V0.data(), V1.data(), F.data());
     std::fprintf(out, "refit %ld %ld\n", n_a, n_b);
   }
+  // 9. the serial traversal (threaded = false): the reference's own callback
+  // sequence, once in full and once with append_discrete stopping at the
+  // middle discrete hit (BVHTree.cpp:236-239: nothing after the stop)
+  {
+    mcl::BVHTree<double, 3> ts;
+    ts.options.box_eta = ts.options.vf_ccd_eta = ts.options.ee_ccd_eta = eta;
+    ts.options.threaded = false;
+    const char* tag = "seqfull";
+    long n_d = 0, stop_at = -1;
+    ts.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const double& toi) {
+      std::fprintf(out, "%s h %d %d %d %d %d %.17g\n", tag, s[0], s[1], s[2], s[3], (int)is_vf, toi);
+    };
+    ts.append_discrete = [&](int p0, int p1) {
+      std::fprintf(out, "%s d %d %d\n", tag, p0, p1);
+      return ++n_d == stop_at;
+    };
+    ts.update(V0.data(), V1.data(), F.data(), nf, 3);
+    ts.traverse(V0.data(), V1.data(), F.data());
+    const long total_d = n_d;
+    tag = "seqstop";
+    n_d = 0;
+    stop_at = (total_d + 1) / 2;
+    ts.traverse(V0.data(), V1.data(), F.data());
+    std::fprintf(out, "serial %ld %ld\n", total_d, stop_at);
+  }
   std::fclose(out);
   return 0;
 }

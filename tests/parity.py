@@ -70,6 +70,16 @@ def compare_hits(gpu, ref, canonical: bool, rtol: float = 0.0):
     return bad, maxrel, over
 
 
+def grad_close(got, ref, rtol: float = 1e-6, floor: float = 1e-12) -> float:
+    """Worst per-entry error of a gradient against a reference gradient, in
+    units of the allowed error rtol*|ref| + floor*max|ref|: <= 1 passes.  The
+    floor only matters for entries near zero, where the summation order of
+    many contributions decides the last bits."""
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
+    tol = rtol * np.abs(ref) + floor * max(float(np.abs(ref).max()), 1e-300)
+    return float((np.abs(got - ref) / tol).max())
+
+
 def quirk_region(q, a, b, c):
     """Projection.hpp:123-133: for s >= 0, t < 0, s + t >= det and a+d <= b+e the
     reference sets s = clamp(-e/c), t = 0, i.e. a point of edge (p1,p2) chosen

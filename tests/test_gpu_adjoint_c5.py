@@ -3,7 +3,7 @@ triangles, about 8.3e8 CCD hits), w_i ~ U(0,1).
 
 (a) The GPU gradient at 1,000 seeded random coordinates of V0 and V1 equals
     the oracle's dual-number gradient summed over every hit touching that
-    vertex (relative 1e-6).
+    vertex (per entry: relative 1e-6, plus 1e-12 of the largest sampled entry).
 (b) Central differences (h = 1e-7) of the oracle's forward ACCD agree with
     the GPU gradient (relative 1e-6) on 1,000 seeded hits, one coordinate
     each; w is non-zero on a vertex-disjoint subset of them so each checked
@@ -17,6 +17,7 @@ import pytest
 
 from mclccd_amd import scenes
 from oracle import oracle as orc
+from tests.parity import grad_close
 
 pytestmark = pytest.mark.gpu
 
@@ -85,8 +86,9 @@ def test_full_size_adjoint_vs_dual_numbers(full_scene):
     got = np.where(which == 0, g0[vi, k], g1[vi, k])
     ref = np.where(which == 0, r0[vi, k], r1[vi, k])
     assert np.count_nonzero(ref) > 900
-    err = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
-    assert err.max() <= 1e-6, err.max()
+    # per entry: 1e-6 relative, plus 1e-12 of the largest sampled entry (summation order near zero)
+    worst = grad_close(got, ref)
+    assert worst <= 1.0, worst
 
 
 def test_full_size_adjoint_vs_central_differences(full_scene):

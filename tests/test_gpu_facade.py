@@ -35,7 +35,8 @@ def test_facade_callbacks(gpu, tmp_path):
                        timeout=120)
     assert r.returncode == 0, r.stderr
     lines = open(tmp_path / "out.txt").read().splitlines()
-    rec = {l.split()[0]: list(map(float, l.split()[1:])) for l in lines if not l.startswith(("hit ", "disc "))}
+    rec = {l.split()[0]: list(map(float, l.split()[1:])) for l in lines
+           if not l.startswith(("hit ", "disc ", "seqfull ", "seqstop "))}
     hits = [l.split()[1:] for l in lines if l.startswith("hit ")]
     disc = [tuple(map(int, l.split()[1:])) for l in lines if l.startswith("disc ")]
 
@@ -77,3 +78,24 @@ def test_facade_callbacks(gpu, tmp_path):
     # refit on the same frame: the rebuilt tree's hits
     n_a, n_b = rec["refit"]
     assert n_a == n_b == len(rs)
+    # serial traversal (threaded = false): the oracle's single-threaded collide
+    # sequence record for record (TOIs bit-identical: the KdBVH orientation), and
+    # a stop at the middle discrete hit delivers exactly the prefix up to it
+    total_d, stop_at = (int(x) for x in rec["serial"])
+    assert total_d == o.count(3) and total_d >= 2
+    full = [l.split()[1:] for l in lines if l.startswith("seqfull ")]
+    stop = [l.split()[1:] for l in lines if l.startswith("seqstop ")]
+    os_ = run_scene(V0, V1, F, eta, threaded=False, candidates=False)
+    ss, sv, st = os_.hits()
+    sd = os_.discrete()
+    expect = []
+    for _, kind, idx in zip(*os_.events()):
+        if kind == 1:
+            expect.append(["h"] + [str(int(x)) for x in ss[idx]] + [str(int(sv[idx])), float(st[idx])])
+        elif kind == 2:
+            expect.append(["d", str(int(sd[idx, 0])), str(int(sd[idx, 1]))])
+    got = [e[:6] + [float(e[6])] if e[0] == "h" else e for e in full]
+    assert got == expect
+    cut = [i for i, e in enumerate(expect) if e[0] == "d"][stop_at - 1]
+    got_stop = [e[:6] + [float(e[6])] if e[0] == "h" else e for e in stop]
+    assert got_stop == expect[:cut + 1]
