@@ -17,21 +17,11 @@
 #include <cmath>
 
 #include "ctcd.hpp"
-#include "geom.hpp"
-#include "kernels.hpp"
+#include "pairs.hpp"
 
 #pragma clang fp contract(off)
 
 namespace mclg {
-
-// int4 as a clang vector: the operand type of the non-temporal builtins
-typedef int nt_int4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------------------
 // Leaf-pair tests
@@ -384,37 +374,6 @@ void launch_pairs_to_prims(int2* pairs, long long n, const FaceArrays& faces, hi
 // Narrow phase
 // ---------------------------------------------------------------------------
 // ---------------------------------------------------------------------------
-// Per-wave LDS staging of output records: lanes deposit records at
-// ballot-computed slots; a full buffer is written out with one global atomic.
-// Called with the whole wave converged (pred carries per-lane validity).
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void stage_flush(T* buf, int cnt, T* __restrict__ dst, unsigned long long cap,
-                                            unsigned long long* counter) {
-  wave_lds_sync();
-  unsigned long long base = 0;
-  const int lane = lane_id();
-  if (lane == 0) base = atomicAdd(counter, (unsigned long long)cnt);
-  base = __shfl(base, 0);
-  for (int i = lane; i < cnt; i += kWave)
-    if (base + i < cap) dst[base + i] = buf[i];
-  wave_lds_sync();
-}
-
-template <typename T, int CAP>
-__device__ __forceinline__ void stage_push(T* buf, int& cnt, bool pred, const T& v, T* __restrict__ dst,
-                                           unsigned long long cap, unsigned long long* counter) {
-  const uint64_t m = __ballot(pred);
-  if (m == 0) return;
-  if (pred) buf[cnt + popc_below(m)] = v;
-  cnt += __popcll(m);
-  if (cnt > CAP - kWave) {
-    stage_flush(buf, cnt, dst, cap, counter);
-    cnt = 0;
-  }
-}
-
-// ---------------------------------------------------------------------------
 // k_expand: per leaf pair (BVHTree.cpp:252-283) the discrete test at V1 and the
 // representative-triangle candidates, appended to a VF and an EE stencil queue.
 // ---------------------------------------------------------------------------
@@ -442,68 +401,6 @@ struct ExpandArgs {
 };
 
 constexpr int EX_WAVES = 4, EX_DSTAGE = 128;
-
-// A face's vertex ids and reptri flags (bits v0..v2, e0..e2) from its head.
-template <int D> struct FaceIds {
-  int p[D];
-  uint32_t flags;
-};
-template <int D> __device__ __forceinline__ FaceIds<D> face_ids(int4 h) {
-  FaceIds<D> f;
-  f.p[0] = h.x;
-  f.p[1] = h.y;
-  if constexpr (D == 3) {
-    f.p[2] = h.z;
-    f.flags = (uint32_t)h.w;
-  } else {
-    f.flags = (uint32_t)h.z;
-  }
-  return f;
-}
-
-// The pair's two faces, lower rank first (BVHTree.cpp:252-283 visits each leaf
-// pair once in the KdBVH's order; the rank reproduces that orientation); pa,
-// pb receive their sorted positions.
-template <int D>
-__device__ __forceinline__ void load_pair(const ExpandArgs& A, long long i, FaceIds<D>& fa, FaceIds<D>& fb, int& pa,
-                                          int& pb) {
-  const int2 sp = A.pairs[i];
-  const int4 x = A.faces.head[sp.x], y = A.faces.head[sp.y];
-  const int rx = A.faces.rank[sp.x], ry = A.faces.rank[sp.y];
-  const bool keep = rx < ry;
-  fa = face_ids<D>(keep ? x : y);
-  fb = face_ids<D>(keep ? y : x);
-  pa = keep ? sp.x : sp.y;
-  pb = keep ? sp.y : sp.x;
-}
-
-// BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (3i + j)
-template <int D>
-__device__ __forceinline__ void candidate_masks(const FaceIds<D>& ha, const FaceIds<D>& hb, uint32_t& vfm,
-                                                uint32_t& eem) {
-  vfm = 0;
-  eem = 0;
-#pragma unroll
-  for (int x = 0; x < D; ++x) {
-    bool sh0 = false, sh1 = false;
-#pragma unroll
-    for (int y = 0; y < D; ++y) { sh0 |= ha.p[x] == hb.p[y]; sh1 |= hb.p[x] == ha.p[y]; }
-    if (((ha.flags >> x) & 1u) && !sh0) vfm |= 1u << x;
-    if (((hb.flags >> x) & 1u) && !sh1) vfm |= 1u << (3 + x);
-  }
-  if constexpr (D == 3) {
-#pragma unroll
-    for (int x = 0; x < 3; ++x) {
-#pragma unroll
-      for (int y = 0; y < 3; ++y) {
-        const int a0 = ha.p[x], a1 = ha.p[(x + 1) % 3], b0 = hb.p[y], b1 = hb.p[(y + 1) % 3];
-        const bool ok = ((ha.flags >> (3 + x)) & 1u) && ((hb.flags >> (3 + y)) & 1u) && a0 != b0 && a0 != b1 &&
-                        a1 != b0 && a1 != b1;
-        if (ok) eem |= 1u << (3 * x + y);
-      }
-    }
-  }
-}
 
 // Pass 1: candidate counts per 64-pair chunk (zero past the last pair).
 template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand_count(ExpandArgs A) {
@@ -547,21 +444,6 @@ __global__ void k_expand_totals(const unsigned long long* off_vf, const unsigned
   counters[C_CAND_EE] = off_ee[nch];
 }
 
-// V1 of a face's vertices (from the per-vertex V1 array).
-template <int D> __device__ __forceinline__ void face_v1(const double* v1, const FaceIds<D>& f, double* x) {
-#pragma unroll
-  for (int j = 0; j < D; ++j) {
-    if constexpr (D == 3) {
-      const double2* r = reinterpret_cast<const double2*>(v1 + 4 * (size_t)f.p[j]);
-      const double2 r0 = r[0], r1 = r[1];
-      x[3 * j] = r0.x; x[3 * j + 1] = r0.y; x[3 * j + 2] = r1.x;
-    } else {
-      const double2 r0 = reinterpret_cast<const double2*>(v1 + 2 * (size_t)f.p[j])[0];
-      x[2 * j] = r0.x; x[2 * j + 1] = r0.y;
-    }
-  }
-}
-
 // A wave's stencils of one chunk, staged in LDS in lane order and written out
 // by consecutive lanes (coalesced) at the chunk's scanned offset.  Vertex ids
 // are plain values (a0..a2 of the first face, b0..b2 of the second): indexing
@@ -576,22 +458,7 @@ __device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a
   while (mask) {
     const int c = __ffs(mask) - 1;
     mask &= mask - 1;
-    int4 st;
-    if constexpr (VF) {  // vertex k of one face against the other face
-      const bool fwd = c < 3;
-      const int k = fwd ? c : c - 3;
-      st.x = fwd ? (k == 0 ? a0 : (k == 1 ? a1 : a2)) : (k == 0 ? b0 : (k == 1 ? b1 : b2));
-      st.y = fwd ? b0 : a0;
-      st.z = fwd ? b1 : a1;
-      st.w = D == 3 ? (fwd ? b2 : a2) : -1;
-    } else {  // edge x of the first face against edge y of the second
-      const int x = c / 3, y = c - 3 * (c / 3);
-      st.x = x == 0 ? a0 : (x == 1 ? a1 : a2);
-      st.y = x == 0 ? a1 : (x == 1 ? a2 : a0);
-      st.z = y == 0 ? b0 : (y == 1 ? b1 : b2);
-      st.w = y == 0 ? b1 : (y == 1 ? b2 : b0);
-    }
-    stage[p++] = st;
+    stage[p++] = candidate_stencil<D, VF>(c, a0, a1, a2, b0, b1, b2);
   }
   wave_lds_sync();
   for (uint32_t j = lane_id(); j < total; j += kWave)  // streamed: read back by the ACCD kernels from HBM
@@ -622,29 +489,10 @@ template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void 
     const bool valid = i < npairs;
     FaceIds<D> ha = {}, hb = {};
     int pa = 0, pb = 0;
-    if (valid) load_pair<D>(A, i, ha, hb, pa, pb);
+    if (valid) load_pair_faces<D>(A.pairs, A.faces, i, ha, hb, pa, pb);
     if (A.discrete) {  // BVHTree.cpp:464-500
-      bool shared = false;
-#pragma unroll
-      for (int x = 0; x < D; ++x)
-#pragma unroll
-        for (int y = 0; y < D; ++y) shared |= ha.p[x] == hb.p[y];
-      int dh = 0, dh_swap = 0;
-      if (valid && !shared) {
-        double a[3 * D], b[3 * D];
-        face_v1<D>(A.faces.v1, ha, a);
-        face_v1<D>(A.faces.v1, hb, b);
-        if constexpr (D == 3) {
-          const d3 p1{a[0], a[1], a[2]}, q1{a[3], a[4], a[5]}, r1{a[6], a[7], a[8]};
-          const d3 p2{b[0], b[1], b[2]}, q2{b[3], b[4], b[5]}, r2{b[6], b[7], b[8]};
-          dh = tri_tri3(p1, q1, r1, p2, q2, r2);
-          if (ODEP) dh_swap = tri_tri3(p2, q2, r2, p1, q1, r1);
-        } else {
-          const d2 p0{a[0], a[1]}, p1{a[2], a[3]}, q0{b[0], b[1]}, q1{b[2], b[3]};
-          dh = edge_edge2(p0, p1, q0, q1);
-          if (ODEP) dh_swap = edge_edge2(q0, q1, p0, p1);
-        }
-      }
+      int dh, dh_swap;
+      discrete_test<D, ODEP>(A.faces.v1, ha, hb, valid, dh, dh_swap);
       int2 rec = make_int2(0, 0);
       if (dh || dh_swap) rec = make_int2(A.faces.prim[pa], A.faces.prim[pb]);
       stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, rec, A.disc, A.disc_cap, A.counters + C_DISCRETE);

@@ -42,6 +42,17 @@ def run_both(V0, V1, P, theta, strict, active=None, **opts):
     if strict:
         t.set_rank(o.rank())
     t.run(V0, V1, P, keep_candidates=True, keep_leaf_pairs=True)
+    # the same problem without a kept candidate list: no per-candidate TOIs,
+    # the ACCD kernels write the hit list directly (the benchmark's path)
+    f = gpu_tree(dim)
+    f.options.box_eta = f.options.vf_ccd_eta = f.options.ee_ccd_eta = theta
+    for k, v in opts.items():
+        setattr(f.options, k, v)
+    f.update(V0, V1, P, active)
+    if strict:
+        f.set_rank(o.rank())
+    f.run(V0, V1, P)
+    t.fused = f
     return t, o
 
 
@@ -101,6 +112,15 @@ def check_parity(t, o, strict, order_dependent=None):
         assert canon_pairs(gd) - skip == canon_pairs(od) - skip
         if not skip:
             assert len(gd) == len(od)
+    f = getattr(t, "fused", None)
+    if f is not None:  # direct hit list: the records of the run above, bit for bit (same orientation)
+        fc = f.counts()
+        for k in ("leaf_pairs", "candidates", "vf_candidates", "hits", "discrete", "dist_evals", "ee_dist_evals"):
+            assert fc[k] == c[k], (k, fc[k], c[k])
+        fs, fv, ft = f.hits()
+        rec = lambda s, v, x: sorted(zip(map(tuple, s.tolist()), v.tolist(), x.tolist()))
+        assert rec(fs, fv, ft) == rec(hs, hv, ht)
+        assert oriented_pairs(f.discrete()) == oriented_pairs(gd)
     return c
 
 
@@ -178,7 +198,7 @@ def test_tiny(gpu, nprim):
     c = t.counts()
     if nprim <= 1:
         assert c == {"leaf_pairs": 0, "candidates": 0, "hits": 0, "discrete": 0, "dist_evals": 0,
-                     "vf_candidates": 0, "ee_dist_evals": 0}
+                     "vf_candidates": 0, "ee_dist_evals": 0, "order_dependent": 0}
     else:
         o = run_scene(V0, V1, F, 1e-2)
         check_parity(t, o, strict=False)
