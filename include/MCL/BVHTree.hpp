@@ -31,8 +31,9 @@
 //     discrete tests widen to double as here.  Leaf boxes handed back (leaves,
 //     visitor volumes) equal the reference's float boxes: a float minus a float
 //     computed in double and rounded once to float is the float difference.
-// When Eigen is available the Eigen overloads of update/traverse are provided
-// and the callbacks take Eigen::Vector4i, as in the reference.
+// The matrix overloads of update/traverse take Eigen matrices as in the
+// reference, or any matrix type with rows(), cols() and operator()(i, j); when
+// Eigen is available the callbacks take Eigen::Vector4i, as in the reference.
 #ifndef MCL_BVHTREE_HPP
 #define MCL_BVHTREE_HPP 1
 
@@ -283,36 +284,33 @@ template <typename T, int DIM> class BVHTree {
     }
   }
 
-#ifdef MCLG_HAVE_EIGEN
-  // BVHTree.hpp:151-186: row-major copies, silent return on mismatched rows
-  template <typename DerivedV, typename DerivedP>
-  void update(const Eigen::MatrixBase<DerivedV>& V0, const Eigen::MatrixBase<DerivedV>& V1,
-              const Eigen::MatrixBase<DerivedP>& P, const Eigen::VectorXi& active = Eigen::VectorXi()) {
-    if (V0.rows() != V1.rows()) return;
-    const int* act = active.size() == V0.rows() ? active.data() : nullptr;
-    typedef Eigen::Matrix<T, Eigen::Dynamic, DIM, Eigen::RowMajor> RV;
-    typedef Eigen::Matrix<int, Eigen::Dynamic, Eigen::Dynamic, Eigen::RowMajor> RI;
-    RV a = V0.template cast<T>(), b = V1.template cast<T>();
-    RI p = P.template cast<int>();
-    push_options();
-    check(mclg_update(ctx_.get(), widen(a.data(), (int)a.rows(), w0_), widen(b.data(), (int)b.rows(), w1_),
-                      (int)a.rows(), p.data(), (int)p.rows(), (int)p.cols(), act));
-    np_ = (int)p.rows();
-    nv_ = (int)a.rows();
-    mirror_ = false;
-    pull_options();
+  // BVHTree.hpp:151-186: the matrix overloads.  Any matrix type with rows(),
+  // cols() and operator()(i, j) — Eigen::MatrixBase among them — is copied to
+  // row-major arrays (V as T, P as int); V0 and V1 with different row counts
+  // return silently, as in the reference.  active: any vector of int with
+  // size() and data() (Eigen::VectorXi, std::vector<int>), used only when its
+  // size is the vertex count.  V0 and V1 need dim columns (else a silent return
+  // too: the reference's fixed-width copy has no other outcome).
+  template <typename MV, typename MP, typename = decltype(std::declval<const MV&>()(0, 0) + std::declval<const MV&>().cols())>
+  void update(const MV& V0, const MV& V1, const MP& P) {
+    update_rows(V0, V1, P, nullptr, 0);
   }
-  template <typename DerivedV, typename DerivedP>
-  void traverse(const Eigen::MatrixBase<DerivedV>& V0, const Eigen::MatrixBase<DerivedV>& V1,
-                const Eigen::MatrixBase<DerivedP>& P) {
-    if (V0.rows() != V1.rows()) return;
-    typedef Eigen::Matrix<T, Eigen::Dynamic, DIM, Eigen::RowMajor> RV;
-    typedef Eigen::Matrix<int, Eigen::Dynamic, Eigen::Dynamic, Eigen::RowMajor> RI;
-    RV a = V0.template cast<T>(), b = V1.template cast<T>();
-    RI p = P.template cast<int>();
+  template <typename MV, typename MP, typename VA,
+            typename = decltype(std::declval<const MV&>()(0, 0) + std::declval<const MV&>().cols() +
+                                *std::declval<const VA&>().data() + std::declval<const VA&>().size())>
+  void update(const MV& V0, const MV& V1, const MP& P, const VA& active) {
+    update_rows(V0, V1, P, active.data(), (long long)active.size());
+  }
+  template <typename MV, typename MP, typename = decltype(std::declval<const MV&>()(0, 0) + std::declval<const MV&>().cols())>
+  void traverse(const MV& V0, const MV& V1, const MP& P) const {
+    if (V0.rows() != V1.rows() || V0.cols() != DIM || V1.cols() != DIM) return;
+    std::vector<T> a, b;
+    std::vector<int> p;
+    row_major(V0, a);
+    row_major(V1, b);
+    row_major(P, p);
     traverse(a.data(), b.data(), p.data());
   }
-#endif
 
   // Fills `leaves` from the device (boxes, active, reptri flags, idx).
   void sync_leaves() {
@@ -418,6 +416,30 @@ template <typename T, int DIM> class BVHTree {
     mirror_ = true;
   }
 
+  // Row-major copy of a matrix (rows(), cols(), operator()(i, j)).
+  template <typename M, typename E> static void row_major(const M& m, std::vector<E>& out) {
+    const long long r = (long long)m.rows(), c = (long long)m.cols();
+    out.resize((size_t)(r * c));
+    for (long long i = 0; i < r; ++i)
+      for (long long j = 0; j < c; ++j) out[(size_t)(i * c + j)] = static_cast<E>(m(i, j));
+  }
+  template <typename MV, typename MP>
+  void update_rows(const MV& V0, const MV& V1, const MP& P, const int* active, long long nactive) {
+    if (V0.rows() != V1.rows() || V0.cols() != DIM || V1.cols() != DIM) return;  // BVHTree.hpp:160
+    std::vector<T> a, b;
+    std::vector<int> p;
+    row_major(V0, a);
+    row_major(V1, b);
+    row_major(P, p);
+    const int nv = (int)V0.rows(), np = (int)P.rows(), pdim = (int)P.cols();
+    push_options();
+    check(mclg_update(ctx_.get(), widen(a.data(), nv, w0_), widen(b.data(), nv, w1_), nv, p.data(), np, pdim,
+                      nactive == nv ? active : nullptr));
+    np_ = np;
+    nv_ = nv;
+    mirror_ = false;
+    pull_options();
+  }
   static Vector4i stencil(const std::vector<int>& s, int64_t i) {
     return Vector4i(s[4 * (size_t)i], s[4 * (size_t)i + 1], s[4 * (size_t)i + 2], s[4 * (size_t)i + 3]);
   }

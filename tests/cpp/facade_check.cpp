@@ -45,6 +45,18 @@ struct StopFirst : mcl::BVHTraverse<double, 3> {
   }
 };
 
+// A minimal row-major matrix with the interface the matrix overloads read.
+template <typename E> struct MatT {
+  long r, c;
+  std::vector<E> v;
+  MatT(long r_, long c_, std::vector<E> v_) : r(r_), c(c_), v(std::move(v_)) {}
+  long rows() const { return r; }
+  long cols() const { return c; }
+  E operator()(long i, long j) const { return v[(size_t)(i * c + j)]; }
+};
+typedef MatT<double> Mat;
+typedef MatT<int> IMat;
+
 int main(int argc, char** argv) {
   if (argc < 3) return 2;
   FILE* f = std::fopen(argv[1], "rb");
@@ -243,6 +255,28 @@ int main(int argc, char** argv) {
     stop_at = (total_d + 1) / 2;
     ts.traverse(V0.data(), V1.data(), F.data());
     std::fprintf(out, "serial %ld %ld\n", total_d, stop_at);
+  }
+  // 10. the matrix overloads (BVHTree.hpp:151-186) with a plain row-major
+  // matrix type: the raw path's hits; V0 and V1 with different row counts
+  // return silently (update leaves the tree as it was, traverse calls nothing)
+  {
+    Mat M0(nv, 3, V0), M1(nv, 3, V1);
+    Mat Mshort(nv - 1, 3, std::vector<double>(V1.begin(), V1.end() - 3));
+    IMat MF(nf, 3, F);
+    mcl::BVHTree<double, 3> tm;
+    tm.options.box_eta = tm.options.vf_ccd_eta = tm.options.ee_ccd_eta = eta;
+    long n_mat = 0, n_after_bad_update = 0, n_bad_traverse = 0;
+    tm.append_pair = [&](const mcl::Vector4i&, bool, const double&) { ++n_mat; };
+    std::vector<int> active((size_t)nv, 1);
+    tm.update(M0, M1, MF, active);
+    tm.traverse(M0, M1, MF);
+    const mclg_counts c1 = tm.counts();
+    tm.update(M0, Mshort, MF);  // silent return: the tree stays
+    tm.append_pair = [&](const mcl::Vector4i&, bool, const double&) { ++n_after_bad_update; };
+    tm.traverse(V0.data(), V1.data(), F.data());
+    tm.append_pair = [&](const mcl::Vector4i&, bool, const double&) { ++n_bad_traverse; };
+    tm.traverse(M0, Mshort, MF);
+    std::fprintf(out, "matrix %ld %ld %ld %lld\n", n_mat, n_after_bad_update, n_bad_traverse, (long long)c1.hits);
   }
   std::fclose(out);
   return 0;

@@ -99,3 +99,6 @@ def test_facade_callbacks(gpu, tmp_path):
     cut = [i for i, e in enumerate(expect) if e[0] == "d"][stop_at - 1]
     got_stop = [e[:6] + [float(e[6])] if e[0] == "h" else e for e in stop]
     assert got_stop == expect[:cut + 1]
+    # matrix overloads: the raw path's hits; mismatched V0 / V1 rows return silently
+    n_mat, n_after_bad_update, n_bad_traverse, c_hits = rec["matrix"]
+    assert n_mat == c_hits == len(rs) and n_after_bad_update == len(rs) and n_bad_traverse == 0

@@ -172,7 +172,7 @@ def test_kdbvh_orientation_without_the_oracle(gpu, case):
     assert rec(hs, hv, ht) == rec(rs, rv, rt)  # EE included: bit-identical TOIs
     gd, od = t.discrete(), o.discrete()
     assert len(gd) == len(od) and oriented_pairs(gd) == oriented_pairs(od)
-    # the fused (default) traverse gives the same records
+    # the default traverse (no candidate list kept, hits written directly) gives the same records
     t.run(V0, V1, F)
     hs, hv, ht = t.hits()
     assert rec(hs, hv, ht) == rec(rs, rv, rt)
@@ -209,3 +209,23 @@ def test_check_order_lists_the_order_dependent_pairs(gpu, case):
     disc = {(min(a, b), max(a, b)) for a, b in t.discrete().tolist()}
     ref = {(min(a, b), max(a, b)) for a, b in o.discrete().tolist()}
     assert disc - expect == ref - expect
+
+
+def test_mismatched_rows_return_silently(gpu):
+    # BVHTree.hpp:160,179: V0 and V1 with different row counts -> no update, no traverse
+    V0, V1, F, th = CASES["soup"]()
+    t = _tree(3, th)
+    t.update(V0, V1, F)
+    log = []
+    _record(t, log)
+    t.traverse(V0, V1, F)
+    assert len(log) > 0
+    t.update(V0, V1[:-1], F)  # returns silently; the last update's tree stays
+    again = []
+    _record(t, again)
+    t.traverse(V0, V1, F)
+    assert again == log
+    none = []
+    _record(t, none)
+    t.traverse(V0, V1[:-1], F)
+    assert none == []
