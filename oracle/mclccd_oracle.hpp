@@ -751,6 +751,11 @@ template <int D> class Tree {
 #endif
       collide(V0, V1, P, q[i].first, q[i].second, loc[tid]);
     }
+    // merge the per-thread lists; each is released once copied, so the peak is
+    // about one copy of the results beside the merged lists
+    size_t nlp = 0, nc = 0, nh = 0, nd = 0;
+    for (auto& L : loc) { nlp += L.lp.size(); nc += L.cand.size(); nh += L.hits.size(); nd += L.disc.size(); }
+    leaf_pairs.reserve(nlp); candidates.reserve(nc); hits.reserve(nh); discrete.reserve(nd);
     for (auto& L : loc) {
       leaf_pairs.insert(leaf_pairs.end(), L.lp.begin(), L.lp.end());
       candidates.insert(candidates.end(), L.cand.begin(), L.cand.end());
@@ -759,6 +764,7 @@ template <int D> class Tree {
       if (nt == 1) events = std::move(L.ev);
       stats.dist_evals += L.st.dist_evals;
       stats.iters += L.st.iters;
+      L = Local();
     }
   }
 
