@@ -9,9 +9,13 @@ inputs already resident in HBM; the step ends when every result is on the
 device and its counts are known on the host.
 
 Prints ONE JSON line.  The roofline figure is for the dominant kernel, timed
-with HIP events recorded on the library's stream around its launches.  The CPU
-baseline is the oracle (a C++ restatement of the reference path, OpenMP over
-the same decomposition) timed on this host on a bounded sample.
+with HIP events recorded on the library's stream around its launches; the
+update kernels' HBM rates are under kernels.build.  The CPU baseline is the
+oracle (a C++ restatement of the reference path, OpenMP over the same
+decomposition) timed on this host's cores on the same full workload (one warm
+frame at 1M triangles), with the CPU model, the cold first update and the
+8,192-triangle cloth at all cores and at 1 thread beside it.  Other workloads:
+--workload cloth | disk | disk-boundary (the 2D edge paths).
 """
 from __future__ import annotations
 
@@ -29,9 +33,12 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector, FMA counted as 2 (39.3 T non-FMA ops/s)
 
 # Algorithmic flops per unit (add/sub/mul/div/sqrt = 1; compare, min/max,
-# negation = 0), from NarrowPhase.cpp:634-750 and Projection.hpp:60-145,241-302:
-FLOPS_ACCD_SETUP = 73  # displacements, centring, l_p (4 norms)
-FLOPS_PER_EVAL = 105   # one ACCD iteration: step x += t_l p (24), distance (~68), eps and t_l (~13)
+# negation = 0), from NarrowPhase.cpp:634-750 and Projection.hpp:60-145,193-204,241-302.
+# Setup: displacements, centring, l_p (one norm per vertex), per stencil.
+FLOPS_ACCD_SETUP = {3: 73, 2: 37}
+# One ACCD iteration: step x += t_l p (3D 24, 2D 12), the distance (VF 68, EE 67,
+# 2D 21) and eps / t_l (13).
+FLOPS_PER_EVAL = {"vf3": 105, "ee3": 104, "vf2": 46}
 
 
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) on
@@ -89,45 +96,95 @@ def profiled_fp64(kernel, ms):
     return None
 
 
-def roofline(c, avg, n, nv):
+def accd_flops(c, dim):
+    """Algorithmic ACCD flops of one traverse: (VF, EE)."""
+    vf_cand = c["vf_candidates"]
+    ee_cand = c["candidates"] - vf_cand
+    vf_evals = c["dist_evals"] - c["ee_dist_evals"]
+    su = FLOPS_ACCD_SETUP[dim]
+    if dim == 2:
+        return su * c["candidates"] + FLOPS_PER_EVAL["vf2"] * c["dist_evals"], 0
+    return (su * vf_cand + FLOPS_PER_EVAL["vf3"] * vf_evals,
+            su * ee_cand + FLOPS_PER_EVAL["ee3"] * c["ee_dist_evals"])
+
+
+def roofline(c, avg, n, nv, dim):
     """Roofline of the longest single kernel of the step (HIP-event times)."""
     kern = {"traverse": avg["traverse"], "accd_vf": avg["accd_vf"], "accd_ee": avg["accd_ee"],
             "expand": avg["expand"]}
     dom = max(kern, key=kern.get)
+    node_b, leaf_b = (64, 144) if dim == 3 else (48, 96)
     if dom == "traverse":
         # compulsory bytes: sorted nodes and FP64 leaf boxes once, 8 B per emitted pair
-        byts = 64 * (2 * n - 1) + 144 * n + 8 * c["leaf_pairs"]
+        byts = node_b * (2 * n - 1) + leaf_b * n + 8 * c["leaf_pairs"]
         achieved = byts / (avg["traverse"] * 1e-3) / 1e9
-        return {"kernel": "k_traverse<3>", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        kernel = f"k_traverse<{dim}>"
+        return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": profiled_traffic("k_traverse<3>"),
+                "traffic": profiled_traffic(kernel) if dim == 3 else None,
                 "algorithmic_bytes": byts, "ms": round(avg["traverse"], 3),
-                "note": "algorithmic bytes = 64*(2n-1) nodes + 144*n FP64 leaf boxes + 8*leaf_pairs; traffic = "
-                        "rocprof FETCH_SIZE+WRITE_SIZE per launch (calibrated on a 1 GiB copy)"}
+                "note": f"algorithmic bytes = {node_b}*(2n-1) nodes + {leaf_b}*n FP64 leaf boxes + 8*leaf_pairs; "
+                        "traffic = rocprof FETCH_SIZE+WRITE_SIZE per launch (calibrated on a 1 GiB copy)"}
     if dom == "expand":
-        byts = 16 * c["leaf_pairs"] + 24 * n + 32 * nv + 16 * c["candidates"] + 8 * c["discrete"]
+        head_b, vert_b = (24, 32) if dim == 3 else (16, 16)
+        byts = 16 * c["leaf_pairs"] + head_b * n + vert_b * nv + 16 * c["candidates"] + 8 * c["discrete"]
         achieved = byts / (avg["expand"] * 1e-3) / 1e9
-        return {"kernel": "k_expand<3> (+ count pass and scans)", "bound": "hbm", "achieved": round(achieved, 1),
+        kernel = f"k_expand<{dim}>"
+        return {"kernel": kernel + " (+ count pass and scans)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": profiled_traffic("k_expand<3>"), "algorithmic_bytes": byts,
-                "note": "algorithmic bytes = pairs read twice (8+8) + 24 B face arrays per leaf + 32 B V1 per vertex + 16 B per "
-                        "stencil + 8 B per discrete hit"}
+                "traffic": profiled_traffic(kernel) if dim == 3 else None, "algorithmic_bytes": byts,
+                "note": f"algorithmic bytes = pairs read twice (8+8) + {head_b} B face arrays per leaf + {vert_b} B V1 "
+                        "per vertex + 16 B per stencil + 8 B per discrete hit"}
     # one ACCD kernel: the vertex-face or the edge-edge stencils
     vf = dom == "accd_vf"
+    fv, fe = accd_flops(c, dim)
+    flops = fv if vf else fe
     cand = c["vf_candidates"] if vf else c["candidates"] - c["vf_candidates"]
     evals = c["dist_evals"] - c["ee_dist_evals"] if vf else c["ee_dist_evals"]
-    flops = FLOPS_ACCD_SETUP * cand + FLOPS_PER_EVAL * evals
+    per_eval = FLOPS_PER_EVAL["vf2" if dim == 2 else ("vf3" if vf else "ee3")]
     ms = avg[dom]
     achieved = flops / (ms * 1e-3) / 1e12
-    kernel = "k_accd<3, true>" if vf else "k_accd<3, false>"
+    kernel = f"k_accd<{dim}, true>" if vf else f"k_accd<{dim}, false>"
     return {"kernel": f"{kernel} (FP64 ACCD, {'VF' if vf else 'EE'} stencils)", "bound": "valu",
             "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kernel),
+            "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kernel) if dim == 3 else None,
             "algorithmic_flops": flops, "stencils": cand, "dist_evals": evals, "ms": round(ms, 3),
-            "pmc": profiled_fp64(kernel, ms),
-            "note": f"algorithmic flops = {FLOPS_ACCD_SETUP}*stencils + {FLOPS_PER_EVAL}*distance evaluations, "
+            "pmc": profiled_fp64(kernel, ms) if dim == 3 else None,
+            "note": f"algorithmic flops = {FLOPS_ACCD_SETUP[dim]}*stencils + {per_eval}*distance evaluations, "
                     "div/sqrt = 1 flop; FP64 vector peak 78.6 TFLOP/s counts FMA as 2 and parity forbids "
                     "contraction, so the usable rate is 39.3 T ops/s"}
+
+
+def build_rates(avg, n, nv, dim, pdim):
+    """HBM rate of each update kernel group against its compulsory bytes."""
+    rec_b = 64 if dim == 3 else 32        # packed vertex record (t0 and t1 positions)
+    leaf_b = 144 if dim == 3 else 96      # FP64 leaf box (swept, t0, t1)
+    node_b = 64 if dim == 3 else 48       # FP32 node
+    cp_b = 2 * node_b                     # child-pair record per internal node
+    phases = {
+        "pack": (4 * pdim * n + 16 * dim * nv + rec_b * nv,
+                 "index range check reads P; read V0 and V1, write the packed records"),
+        "leaf_boxes": (4 * pdim * n + rec_b * nv + (leaf_b + 1) * n,
+                       "read P and each vertex record once, write FP64 leaf boxes and active flags"),
+        "morton": ((leaf_b // 3) * n + 8 * n, "read swept boxes, write keys and indices"),
+        "sort": (4 * 20 * n, "4 passes of 8-bit digits: histogram reads keys (4 B), scatter reads and writes "
+                             "key + value (16 B)"),
+        "lbvh": (8 * n + 2 * leaf_b * n + node_b * (2 * n - 1) + (cp_b + 16) * (n - 1),
+                 "read keys, indices and leaf boxes; write sorted leaf boxes, nodes, child pairs, child and range "
+                 "links"),
+    }
+    out = {}
+    total_b, total_ms = 0, 0.0
+    for k, (b, what) in phases.items():
+        ms = avg["update_" + k]
+        gbs = b / (ms * 1e-3) / 1e9 if ms > 0 else None
+        out[k] = {"ms": round(ms, 4), "bytes": int(b), "gb_s": round(gbs, 1) if gbs else None,
+                  "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "bytes_are": what}
+        total_b += b
+        total_ms += ms
+    out["total"] = {"ms": round(total_ms, 4), "bytes": int(total_b),
+                    "gb_s": round(total_b / (total_ms * 1e-3) / 1e9, 1) if total_ms > 0 else None}
+    return out
 
 
 def _cores():
@@ -146,49 +203,77 @@ def make_workload(name):
     elif name == "cloth":
         V0, V1, F, th = scenes.cloth_fold()
         desc = "64x64-quad cloth sheet folded by 0.3 sin(3 pi x) sin(2 pi y) + noise, seed 0"
+    elif name in ("disk", "disk-boundary"):
+        edges = "boundary" if name == "disk-boundary" else "all"
+        V0, V1, F, th = scenes.disk2d(edges=edges)
+        desc = (f"2D triangulated disk, 150k uniform points (seed 1), {edges} edges, swirl 0.4 r exp(-r^2) + "
+                "noise 1e-3, theta 1e-3")
     else:
         raise SystemExit(f"unknown workload {name}")
     return V0, V1, F, th, desc
 
 
-def cpu_baseline(name, V0, V1, F, th, gpu_leaf_pairs):
-    """Times the oracle on this host.  For the 1M-triangle workload a CPU run
-    takes tens of seconds per frame, so the sample is the same generator at
-    90x90 quads per layer (a quarter of the triangles; about 10-20 s of host time
-    for the timed frames) and the time is scaled by the ratio of broadphase leaf
-    pairs (the CPU path's work is linear in them)."""
-    from oracle.oracle import Oracle
-    from mclccd_amd import scenes
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
 
-    cores = _cores()
-    if name == "layers":
-        sV0, sV1, sF, sth = scenes.stacked_layers(n=90)
-        sample = f"stacked layers at 90x90 quads per layer ({len(sF):,} tris), scaled by leaf pairs"
-    else:
-        sV0, sV1, sF, sth = V0, V1, F, th
-        sample = "full workload"
-    o = Oracle(3)
-    o.set_options(box_eta=sth, vf_ccd_eta=sth, ee_ccd_eta=sth, nthreads=cores)
-    o.update(sV0, sV1, sF)  # cold update (reptri) outside the timing
+    return platform.processor() or "unknown"
+
+
+def _oracle_frames(V0, V1, F, th, threads, runs, deadline_s):
+    """Cold update (reptri) once, then warm update+traverse frames; callbacks
+    append to per-thread lists (the oracle keeps every hit).  Returns (cold
+    update ms, sorted warm ms, oracle)."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(V0.shape[1])
+    o.set_options(box_eta=th, vf_ccd_eta=th, ee_ccd_eta=th, nthreads=threads)
+    t0 = time.perf_counter()
+    o.update(V0, V1, F)
+    cold = (time.perf_counter() - t0) * 1e3
     times = []
-    t_end = time.time() + 20.0
-    for _ in range(5):
+    t_end = time.time() + deadline_s
+    for _ in range(runs):
         t0 = time.perf_counter()
-        o.update(sV0, sV1, sF)
+        o.update(V0, V1, F)
         o.traverse(leaf_pairs=False, candidates=False)
         times.append((time.perf_counter() - t0) * 1e3)
         if time.time() > t_end:
             break
-    times.sort()
+    return cold, sorted(times), o
+
+
+def cpu_baseline(name, V0, V1, F, th, gpu_counts):
+    """Times the oracle (the reference path restated in C++, OpenMP over the
+    same decomposition: leaves, then the frontlist pairs) on this host's cores,
+    on the full workload: one warm frame for the 1M-triangle scene (tens of
+    seconds), the median of up to 5 otherwise; plus the 8,192-triangle cloth
+    at all cores and at 1 thread (median of 5)."""
+    from mclccd_amd import scenes
+
+    cores = _cores()
+    runs = 1 if name == "layers" else 5
+    cold, times, o = _oracle_frames(V0, V1, F, th, cores, runs, 60.0)
     ms = times[len(times) // 2]
-    lp = o.count(0)
-    # leaf pairs are counted by the brute-force-checked traversal: rerun once with them kept
-    o.traverse(leaf_pairs=True, candidates=False)
-    lp = o.count(0)
-    scale = gpu_leaf_pairs / lp if name == "layers" and lp > 0 else 1.0
-    return {"value": round(ms * scale, 3), "unit": "ms", "cores": cores, "kind": "port",
-            "sample": f"{sample}: median of {len(times)} warm update+traverse runs = {ms:.1f} ms for {lp} leaf pairs"
-                      + (f", x{scale:.1f}" if scale != 1.0 else "")}
+    same = {k: o.count(i) == gpu_counts[k] for i, k in ((2, "hits"), (3, "discrete"))}
+    del o
+    cV0, cV1, cF, cth = scenes.cloth_fold()
+    _, c_all, _ = _oracle_frames(cV0, cV1, cF, cth, cores, 5, 10.0)
+    _, c_one, _ = _oracle_frames(cV0, cV1, cF, cth, 1, 5, 20.0)
+    what = "one warm update+traverse" if runs == 1 else f"median of {len(times)} warm update+traverse runs"
+    return {"value": round(ms, 3), "unit": "ms", "cores": cores, "kind": "port",
+            "sample": f"full workload, {what} (callbacks append every hit and discrete pair to per-thread lists); "
+                      f"cold first update (reptri) {cold:.1f} ms",
+            "cpu_model": cpu_model(), "cold_update_ms": round(cold, 1),
+            "same_counts_as_gpu": same,
+            "cloth_8192_tris_ms": {"all_cores": round(c_all[len(c_all) // 2], 3),
+                                   "one_thread": round(c_one[len(c_one) // 2], 3)}}
 
 
 def batch_bench(steps, autotune=True):
@@ -252,13 +337,39 @@ def batch_bench(steps, autotune=True):
                     "plain = one update+traverse per scene on one context"}
 
 
+def cpu_batch():
+    """The 512-scene mix on the oracle, scene after scene at all host cores:
+    one warm pass (cold updates first, outside the timing)."""
+    from mclccd_amd import scenes
+    from oracle.oracle import Oracle
+
+    cores = _cores()
+    sc = scenes.batch_sheets()
+    trees = []
+    for a0, a1, F, th in sc:
+        o = Oracle(3)
+        o.set_options(box_eta=th, vf_ccd_eta=th, ee_ccd_eta=th, nthreads=cores)
+        o.update(a0, a1, F)
+        trees.append(o)
+    hits = 0
+    t0 = time.perf_counter()
+    for (a0, a1, F, th), o in zip(sc, trees):
+        o.update(a0, a1, F)
+        o.traverse(leaf_pairs=False, candidates=False)
+        hits += o.count(2)
+    ms = (time.perf_counter() - t0) * 1e3
+    return {"ms_per_frame": round(ms, 1), "scenes_per_s": round(len(sc) / ms * 1e3, 2), "cores": cores,
+            "hits": hits, "sample": "one warm pass over all 512 scenes"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="layers", choices=["layers", "cloth"])
+    ap.add_argument("--workload", default="layers", choices=["layers", "cloth", "disk", "disk-boundary"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the TOI adjoint")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-scenes comparison")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one 1 GiB device-to-device copy (reference kernel for PMC byte counters)")
@@ -286,9 +397,10 @@ def main():
     dF = torch.from_numpy(F).to(dev)
     torch.cuda.synchronize()
 
-    t = BVHTree(3)
+    dim = V0.shape[1]
+    t = BVHTree(dim)
     t.options.box_eta = t.options.vf_ccd_eta = t.options.ee_ccd_eta = th
-    print(f"[bench] workload: {desc}; {len(F)} tris", file=sys.stderr, flush=True)
+    print(f"[bench] workload: {desc}; {len(F)} primitives", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         t.update(dV0, dV1, dF)
         t.run(dV0, dV1, dF)
@@ -312,34 +424,37 @@ def main():
     print(f"[bench] counts {c}", file=sys.stderr, flush=True)
     print(f"[bench] phase ms {avg}", file=sys.stderr, flush=True)
 
-    roof = roofline(c, avg, len(F), len(V0))
+    roof = roofline(c, avg, len(F), len(V0), dim)
     # the other large kernels, for context
+    fv, fe = accd_flops(c, dim)
     others = {"traverse_ms": round(avg["traverse"], 3), "expand_ms": round(avg["expand"], 3),
               "accd_vf_ms": round(avg["accd_vf"], 3), "accd_ee_ms": round(avg["accd_ee"], 3),
-              "accd_tflops": round((FLOPS_ACCD_SETUP * c["candidates"] + FLOPS_PER_EVAL * c["dist_evals"]) /
-                                   ((avg["accd_vf"] + avg["accd_ee"]) * 1e-3) / 1e12, 3)
-              if avg["accd_vf"] + avg["accd_ee"] > 0 else None}
+              "accd_tflops": round((fv + fe) / ((avg["accd_vf"] + avg["accd_ee"]) * 1e-3) / 1e12, 3)
+              if avg["accd_vf"] + avg["accd_ee"] > 0 else None,
+              "build": build_rates(avg, len(F), len(V0), dim, F.shape[1])}
 
     # TOI adjoint over every hit of the last traverse, w_i ~ U(0,1) (seed 4)
     import ctypes as C
 
     from mclccd_amd._lib import lib
 
-    gen = torch.Generator(device=dev).manual_seed(4)
-    w = torch.rand(c["hits"], dtype=torch.float64, device=dev, generator=gen)
-    gV0 = torch.empty_like(dV0)
-    gV1 = torch.empty_like(dV0)
-    adj = []
-    for i in range(args.warmup + args.steps):
-        torch.cuda.synchronize()
-        t._check(lib.mclg_toi_grad(t._h, C.c_void_p(w.data_ptr()), C.c_void_p(gV0.data_ptr()),
-                                   C.c_void_p(gV1.data_ptr()), 1))
-        if i >= args.warmup:
-            adj.append(t.timings()["adjoint"])
-    del w
-    adjoint = {"ms": round(float(np.median(adj)), 3), "hits": c["hits"], "weights": "U(0,1), seed 4",
-               "note": "dL/dV0, dL/dV1 of L = sum w_i toi_i over every CCD hit (FP64, HIP events)"}
-    print(f"[bench] adjoint {adjoint}", file=sys.stderr, flush=True)
+    adjoint = None
+    if not args.no_adjoint:
+        gen = torch.Generator(device=dev).manual_seed(4)
+        w = torch.rand(c["hits"], dtype=torch.float64, device=dev, generator=gen)
+        gV0 = torch.empty_like(dV0)
+        gV1 = torch.empty_like(dV0)
+        adj = []
+        for i in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t._check(lib.mclg_toi_grad(t._h, C.c_void_p(w.data_ptr()), C.c_void_p(gV0.data_ptr()),
+                                       C.c_void_p(gV1.data_ptr()), 1))
+            if i >= args.warmup:
+                adj.append(t.timings()["adjoint"])
+        del w, gV0, gV1
+        adjoint = {"ms": round(float(np.median(adj)), 3), "hits": c["hits"], "weights": "U(0,1), seed 4",
+                   "note": "dL/dV0, dL/dV1 of L = sum w_i toi_i over every CCD hit (FP64, HIP events)"}
+        print(f"[bench] adjoint {adjoint}", file=sys.stderr, flush=True)
 
     batched = None
     if not args.no_batch:
@@ -348,7 +463,15 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.workload, V0, V1, F, th, c["leaf_pairs"])
+        del t
+        torch.cuda.empty_cache()
+        print("[bench] CPU baseline (oracle, all host cores) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.workload, V0, V1, F, th, c)
+        print(f"[bench] cpu {cpu}", file=sys.stderr, flush=True)
+        if batched is not None:
+            batched["cpu"] = cpu_batch()
+            batched["cpu"]["same_hits_as_gpu"] = batched["cpu"]["hits"] == batched["hits"]
+            print(f"[bench] cpu batched {batched['cpu']}", file=sys.stderr, flush=True)
 
     line = {
         "metric": METRIC,
@@ -363,7 +486,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "tris": int(len(F)), "verts": int(len(V0)), "theta": th,
+        "config": {"workload": desc, "prims": int(len(F)), "verts": int(len(V0)), "theta": th,
+                   "results": "left on the device (hit, discrete and pair lists in HBM; the host reads the counts); "
+                              "no copy to the host and no callback replay inside the timed region",
                    "leaf_pairs": c["leaf_pairs"], "candidates": c["candidates"], "hits": c["hits"],
                    "discrete": c["discrete"], "dist_evals": c["dist_evals"],
                    "phase_ms": {k: round(v, 4) for k, v in avg.items()}},
