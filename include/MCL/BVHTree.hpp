@@ -22,15 +22,12 @@
 //     GPU failures throw std::runtime_error.
 //   - One thread at a time per tree: update and traverse write the tree's
 //     device buffers (separate trees may run on separate threads).
-//   - T = double and T = float (BVHTree.cpp:504-507).  The GPU path computes in
-//     FP64: a float tree widens its vertices (exactly) to double, so its
-//     candidates, hits and discrete pairs are those of BVHTree<double,DIM> on
-//     the same coordinates, and TOIs are rounded to float on output.  The
-//     reference's float tree pads boxes and runs ACCD in float arithmetic, so a
-//     pair within float rounding of a threshold can differ; its CTCD and
-//     discrete tests widen to double as here.  Leaf boxes handed back (leaves,
-//     visitor volumes) equal the reference's float boxes: a float minus a float
-//     computed in double and rounded once to float is the float difference.
+//   - T = double and T = float (BVHTree.cpp:504-507).  A float tree runs the
+//     library in MCLG_PRECISION_F32: its float vertices are widened exactly for
+//     the C ABI, and the GPU pads boxes in float, orders the KdBVH by float
+//     centres and runs ACCD in float arithmetic as NarrowPhaseACCD<float,DIM>
+//     does; CTCD and the discrete tests widen to double as the reference's
+//     float specialisations do.  TOIs are float values.
 // The matrix overloads of update/traverse take Eigen matrices as in the
 // reference, or any matrix type with rows(), cols() and operator()(i, j); when
 // Eigen is available the callbacks take Eigen::Vector4i, as in the reference.
@@ -199,6 +196,7 @@ template <typename T, int DIM> class BVHTree {
     mclg_ctx* c = nullptr;
     check(mclg_create(DIM, &c), c);
     ctx_.reset(c, [](mclg_ctx* p) { mclg_destroy(p); });
+    if (std::is_same<T, float>::value) check(mclg_set_precision(c, MCLG_PRECISION_F32), c);
   }
   virtual ~BVHTree() {}
 

@@ -218,6 +218,28 @@ int mclg_set_rank(mclg_ctx* ctx, const int* rank, int np);
 int mclg_set_orientation(mclg_ctx* ctx, int mode);
 int mclg_get_orientation(const mclg_ctx* ctx, int* mode);
 
+/* The scalar type T of mcl::BVHTree<T,DIM> (BVHTree.cpp:504-507):
+ *   MCLG_PRECISION_F64  T = double (the default);
+ *   MCLG_PRECISION_F32  T = float: vertex coordinates and the three etas are
+ *                       rounded to float at each update / traverse, leaf boxes
+ *                       are padded in float (BVHLeaf bounds), the KdBVH order
+ *                       uses float centres, and ACCD runs in float arithmetic as
+ *                       NarrowPhaseACCD<float,DIM> does (NarrowPhase.cpp:634-711,
+ *                       757-770); CTCD and the discrete tests widen to double as
+ *                       the reference's float specialisations do
+ *                       (NarrowPhase.cpp:208-219,259-269,425-444) and CTCD TOIs
+ *                       are rounded to float.  Inputs stay double arrays (a
+ *                       float caller widens exactly); outputs are float values
+ *                       held in doubles.  The TOI adjoint is FP64 only
+ *                       (MCLG_ERR_UNSUPPORTED after a float traverse); the
+ *                       batched path is FP64 only.
+ * Set it before the first update (MCLG_ERR_STATE afterwards, as T is fixed for
+ * the life of a reference tree). */
+#define MCLG_PRECISION_F64 0
+#define MCLG_PRECISION_F32 1
+int mclg_set_precision(mclg_ctx* ctx, int precision);
+int mclg_get_precision(const mclg_ctx* ctx, int* precision);
+
 /* The reference KdBVH's leaf order for given swept-box centres, on the host
  * (no GPU): rank[i] = in-order position of leaf i (KdBVH.hpp:192-222, objects
  * reordered as in KdBVH.hpp:130-133); gapnode[g] (n-1 entries, may be NULL) =

@@ -26,6 +26,8 @@ KEEP_LEAF_PAIRS = 2
 CHECK_ORDER = 4
 ORIENT_INDEX = 0
 ORIENT_KDBVH = 1
+PRECISION_F64 = 0
+PRECISION_F32 = 1
 NUM_TIMINGS = 13
 TIMING_NAMES = ("update_pack", "update_reptri", "update_leaf_boxes", "update_morton", "update_sort",
                 "update_lbvh", "traverse_pack", "traverse", "expand", "accd", "adjoint", "accd_vf", "accd_ee")
@@ -114,6 +116,8 @@ def _load():
         "mclg_set_rank": (I, [P, P, I]),
         "mclg_set_orientation": (I, [P, I]),
         "mclg_get_orientation": (I, [P, P]),
+        "mclg_set_precision": (I, [P, I]),
+        "mclg_get_precision": (I, [P, P]),
         "mclg_kdbvh_order": (I, [I, P, I, P, P]),
         "mclg_copy_events": (I, [P, I, P, P, I64, P]),
         "mclg_get_timings": (I, [P, P, I]),

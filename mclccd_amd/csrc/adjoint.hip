@@ -170,7 +170,7 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
   using V = typename AdjV<D>::V;
   constexpr int ns = D + 1;
   V v0[ns];
-  Stencil<D, VF> st;
+  Stencil<D, VF, double> st;
   double l_p;
   {
     V v1[ns];
@@ -322,7 +322,7 @@ __device__ bool toi_adjoint_k1(const int* sv, const double* rec, double xsi, dou
                                double* gV0, double* gV1) {
   using V = typename AdjV<D>::V;
   constexpr int ns = D + 1;
-  Stencil<D, VF> st;
+  Stencil<D, VF, double> st;
   double l_p;
   {
     V v0[ns], v1[ns];
@@ -574,6 +574,8 @@ static int toi_grad_run(Ctx* c, const double* gw, double* g0, double* g1, const 
 static int toi_grad_check(Ctx* c) {
   if (c->hits_one_sided)
     return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad: the TOI adjoint covers ACCD only, not the vf_one_sided CTCD path");
+  if (c->hits_f32)
+    return c->fail(MCLG_ERR_UNSUPPORTED, "toi_grad: the TOI adjoint is FP64; the last traverse ran in float precision");
   if (c->hits_gen != c->tree_gen)
     return c->fail(MCLG_ERR_STATE,
                    "toi_grad: no traverse since the last update or refit (the hits would be paired with the new "

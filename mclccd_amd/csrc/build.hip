@@ -10,23 +10,26 @@ namespace mclg {
 // ---------------------------------------------------------------------------
 // Vertex records (both time steps of a vertex in one 64 B / 32 B record)
 // ---------------------------------------------------------------------------
+// f32: a float-precision tree computes on float(V) (BVHTree<float,DIM> takes
+// float vertices); the records hold those values exactly.
 template <int D>
 __global__ void k_pack(const double* __restrict__ V0, const double* __restrict__ V1, int nv, double* __restrict__ rec,
-                       double* __restrict__ v1) {
+                       double* __restrict__ v1, bool f32) {
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
+  auto rd = [f32](double x) { return f32 ? (double)(float)x : x; };
   double* r = rec + (size_t)v * VRec<D>::stride;
   if constexpr (D == 3) {
-    const double x1 = V1[3 * (size_t)v], y1 = V1[3 * (size_t)v + 1], z1 = V1[3 * (size_t)v + 2];
-    r[0] = V0[3 * (size_t)v]; r[1] = V0[3 * (size_t)v + 1]; r[2] = V0[3 * (size_t)v + 2]; r[3] = 0.0;
+    const double x1 = rd(V1[3 * (size_t)v]), y1 = rd(V1[3 * (size_t)v + 1]), z1 = rd(V1[3 * (size_t)v + 2]);
+    r[0] = rd(V0[3 * (size_t)v]); r[1] = rd(V0[3 * (size_t)v + 1]); r[2] = rd(V0[3 * (size_t)v + 2]); r[3] = 0.0;
     r[4] = x1; r[5] = y1; r[6] = z1; r[7] = 0.0;
     if (v1) {
       double* o = v1 + (size_t)v * 4;
       o[0] = x1; o[1] = y1; o[2] = z1; o[3] = 0.0;
     }
   } else {
-    const double x1 = V1[2 * (size_t)v], y1 = V1[2 * (size_t)v + 1];
-    r[0] = V0[2 * (size_t)v]; r[1] = V0[2 * (size_t)v + 1];
+    const double x1 = rd(V1[2 * (size_t)v]), y1 = rd(V1[2 * (size_t)v + 1]);
+    r[0] = rd(V0[2 * (size_t)v]); r[1] = rd(V0[2 * (size_t)v + 1]);
     r[2] = x1; r[3] = y1;
     if (v1) {
       v1[2 * (size_t)v] = x1;
@@ -35,11 +38,11 @@ __global__ void k_pack(const double* __restrict__ V0, const double* __restrict__
   }
 }
 
-void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1) {
+void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1, bool f32) {
   if (nv <= 0) return;
   const int g = (nv + 255) / 256;
-  if (dim == 3) k_pack<3><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1);
-  else k_pack<2><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1);
+  if (dim == 3) k_pack<3><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1, f32);
+  else k_pack<2><<<g, 256, 0, s>>>(V0, V1, nv, rec, v1, f32);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,7 +195,7 @@ template <int D>
 __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ rec, const int* __restrict__ P, int np,
                                                     int pdim, const int* __restrict__ active, double eta,
                                                     LeafBox<D>* __restrict__ boxes, uint8_t* __restrict__ act,
-                                                    double* __restrict__ partial, SceneArgs S) {
+                                                    double* __restrict__ partial, SceneArgs S, bool f32) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (S.prim_scene && i < np) eta = S.box_eta[S.prim_scene[i]];
   double clo[D], chi[D];
@@ -224,10 +227,26 @@ __global__ __launch_bounds__(256) void k_leaf_boxes(const double* __restrict__ r
       }
       if (active != nullptr && active[v]) a = true;
     }
+    if (f32) {
+      // float tree (float coordinates, float eta): the float sum or
+      // difference, i.e. the exact one rounded once to float (one binary64
+      // operation then a rounding to binary32 gives exactly that)
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        b.t0_lo[k] = (double)(float)(b.t0_lo[k] - eta); b.t1_lo[k] = (double)(float)(b.t1_lo[k] - eta);
+        b.sw_lo[k] = (double)(float)(b.sw_lo[k] - eta);
+        b.t0_hi[k] = (double)(float)(b.t0_hi[k] + eta); b.t1_hi[k] = (double)(float)(b.t1_hi[k] + eta);
+        b.sw_hi[k] = (double)(float)(b.sw_hi[k] + eta);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        b.t0_lo[k] -= eta; b.t1_lo[k] -= eta; b.sw_lo[k] -= eta;
+        b.t0_hi[k] += eta; b.t1_hi[k] += eta; b.sw_hi[k] += eta;
+      }
+    }
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-      b.t0_lo[k] -= eta; b.t1_lo[k] -= eta; b.sw_lo[k] -= eta;
-      b.t0_hi[k] += eta; b.t1_hi[k] += eta; b.sw_hi[k] += eta;
       const double c = (b.sw_lo[k] + b.sw_hi[k]) / 2.0;
       clo[k] = c; chi[k] = c;
     }
@@ -338,17 +357,17 @@ int leaf_box_blocks(int np) { return (np + 255) / 256; }
 
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
                        void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, const SceneArgs& S,
-                       hipStream_t s) {
+                       hipStream_t s, bool f32) {
   const int g = leaf_box_blocks(np);
   if (S.prim_scene) {
     const unsigned long long lo0 = dbl_to_ord(__builtin_huge_val()), hi0 = dbl_to_ord(-__builtin_huge_val());
     launch_bounds_init(S.bounds, S.n, dim, lo0, hi0, s);
   }
   if (dim == 3) {
-    k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, partial, S);
+    k_leaf_boxes<3><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<3>*)boxes, act, partial, S, f32);
     if (!S.prim_scene) k_reduce_bounds<3><<<1, 256, 0, s>>>(partial, g, cbounds);
   } else {
-    k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, partial, S);
+    k_leaf_boxes<2><<<g, 256, 0, s>>>(rec, P, np, pdim, active, eta, (LeafBox<2>*)boxes, act, partial, S, f32);
     if (!S.prim_scene) k_reduce_bounds<2><<<1, 256, 0, s>>>(partial, g, cbounds);
   }
 }

@@ -184,6 +184,12 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   }
   if (nv_ == 0 || !V0 || !V1 || !P) return fail(MCLG_ERR_ARG, "update: null input");
   if (np_ > (1 << 26)) return fail(MCLG_ERR_ARG, "update: %d primitives exceeds the 2^26 limit", np_);
+  if (precision == MCLG_PRECISION_F32) {  // BVHTree<float,DIM>: the options are floats
+    opt.box_eta = (double)(float)opt.box_eta;
+    opt.vf_ccd_eta = (double)(float)opt.vf_ccd_eta;
+    opt.ee_ccd_eta = (double)(float)opt.ee_ccd_eta;
+  }
+  const bool f32 = precision == MCLG_PRECISION_F32;
   // BVHTree.cpp:39-40
   opt.vf_ccd_eta = std::min(opt.vf_ccd_eta, opt.box_eta);
   opt.ee_ccd_eta = std::min(opt.ee_ccd_eta, opt.box_eta);
@@ -192,7 +198,7 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(check_indices(P, (long long)np_ * pdim_, nv_, "update: P"));
   const int stride = dim == 3 ? 8 : 4;
   CHECK(grow((void**)&d_rec, &cap_rec, sizeof(double) * stride * (size_t)nv_, "vertex records"));
-  launch_pack(dim, V0, V1, nv_, d_rec, stream);
+  launch_pack(dim, V0, V1, nv_, d_rec, stream, nullptr, f32);
   mark(1);
   // leaf resize + reptri (BVHTree.cpp:44-80): flags survive a resize for the
   // surviving leaves; new leaves start at (1,1,1)/(1,1,1) as BVHLeaf() sets them
@@ -234,7 +240,8 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_act, &cap_act, (size_t)np, "active"));
   CHECK(grow((void**)&d_cbounds, &cap_cbounds, sizeof(unsigned long long) * 6, "bounds"));
   CHECK(grow((void**)&d_partial, &cap_partial, sizeof(double) * 6 * (size_t)leaf_box_blocks(np), "bounds"));
-  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, SceneArgs(), stream);
+  launch_leaf_boxes(dim, d_rec, P, np, pdim, active, opt.box_eta, d_boxes, d_act, d_partial, d_cbounds, SceneArgs(), stream,
+                    f32);
   // the reference's KdBVH order (KdBVH.hpp:192-222) when asked for, or for a
   // serial traversal whose callback order must be the reference's
   if (orient == MCLG_ORIENT_KDBVH || !opt.threaded) {
@@ -292,6 +299,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   counts = mclg_counts{0, 0, 0, 0, 0, 0, 0};
   hits_gen = kNoGen;
   hits_one_sided = opt.vf_one_sided != 0;
+  hits_f32 = precision == MCLG_PRECISION_F32;
   keep_cand = (flags & MCLG_KEEP_CANDIDATES) != 0;
   keep_pairs = (flags & MCLG_KEEP_LEAF_PAIRS) != 0;
   check_order = (flags & MCLG_CHECK_ORDER) != 0;
@@ -305,7 +313,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   mark(7);
   CHECK(check_indices(P, (long long)np * dim, nv, "traverse: P"));
   CHECK(grow((void**)&d_v1, &cap_v1, sizeof(double) * v1_stride(dim) * (size_t)nv, "V1"));
-  launch_pack(dim, V0, V1, nv, d_rec, stream, d_v1);
+  launch_pack(dim, V0, V1, nv, d_rec, stream, d_v1, hits_f32);
   CHECK(grow(&d_face, &cap_face, face_bytes((size_t)np), "face arrays"));
   launch_face_records(dim, d_order, np, P, d_flags, rank_set ? d_rank : nullptr, faces(), stream);
   pairs_are_prims = false;
@@ -401,7 +409,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
       AccdLaunch L;
       L.rec = d_rec; L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; 
-      L.vert_scene = nullptr; L.scene_eta = nullptr;
+      L.vert_scene = nullptr; L.scene_eta = nullptr; L.f32 = hits_f32;
       if (direct) {
         CHECK(grow((void**)&d_hit_sten, &cap_hit_sten, sizeof(int4) * hit_cap, "hits"));
         CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));

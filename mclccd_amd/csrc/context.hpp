@@ -41,6 +41,10 @@ struct Ctx {
   // mclg_set_rank) or MCLG_ORIENT_KDBVH (the reference's KdBVH order, rebuilt
   // on the host at every update; also used while opt.threaded == 0)
   int orient = MCLG_ORIENT_INDEX;
+  // MCLG_PRECISION_F32: BVHTree<float,DIM> arithmetic (float coordinates,
+  // boxes and ACCD); hits_f32: the last traverse ran in it
+  int precision = MCLG_PRECISION_F64;
+  bool hits_f32 = false;
   bool kd_valid = false;           // kd_rank / kd_gapnode describe the current leaves
   std::vector<int> kd_rank;        // in-order position of each primitive
   std::vector<int> kd_gapnode;     // frontlist index of the node splitting positions g, g+1

@@ -16,16 +16,33 @@
 
 namespace mclg {
 
-struct d3 { double x, y, z; };
-struct d2 { double x, y; };
+// Scalar type T: double for BVHTree<double,DIM>, float for BVHTree<float,DIM>
+// (NarrowPhase.cpp:757-770).  Float '+', '-', '*' are the hardware's IEEE
+// single-precision operations; float '/' and sqrt go through double and are
+// rounded once to float, which is the correctly rounded float result (binary64
+// carries more than 2*24+2 bits, so the second rounding cannot differ).
+template <typename T> struct vec3 { T x, y, z; };
+template <typename T> struct vec2 { T x, y; };
+using d3 = vec3<double>;
+using d2 = vec2<double>;
 
-__device__ __forceinline__ d3 sub(const d3& a, const d3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-__device__ __forceinline__ d2 sub(const d2& a, const d2& b) { return {a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ double dot(const d3& a, const d3& b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
-__device__ __forceinline__ double dot(const d2& a, const d2& b) { return a.x * b.x + a.y * b.y; }
+template <typename T> __device__ __forceinline__ vec3<T> sub(const vec3<T>& a, const vec3<T>& b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <typename T> __device__ __forceinline__ vec2<T> sub(const vec2<T>& a, const vec2<T>& b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename T> __device__ __forceinline__ T dot(const vec3<T>& a, const vec3<T>& b) {
+  return (a.x * b.x + a.y * b.y) + a.z * b.z;
+}
+template <typename T> __device__ __forceinline__ T dot(const vec2<T>& a, const vec2<T>& b) { return a.x * b.x + a.y * b.y; }
 // libstdc++ std::max / std::min (NaN and signed zero as the reference sees them)
-__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
-__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+template <typename T> __device__ __forceinline__ T smax(T a, T b) { return (a < b) ? b : a; }
+template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double cr_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float cr_div(float a, float b) { return (float)((double)a / (double)b); }
+__device__ __forceinline__ double cr_sqrt(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float cr_sqrt(float a) { return (float)__builtin_sqrt((double)a); }
 
 // ---------------------------------------------------------------------------
 // VF distance: |q - closest point of triangle (a,b,c)|
@@ -33,25 +50,26 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 // ---------------------------------------------------------------------------
 enum : int { R_S = 0, R_T = 1, R_IN = 2, R_EDGE = 3 };
 
-__device__ __forceinline__ double dist_vf(const d3& q, const d3& a, const d3& b, const d3& c) {
-  const d3 e0 = sub(b, a), e1 = sub(c, a), w = sub(a, q);
-  const double A = dot(e0, e0), B = dot(e0, e1), C = dot(e1, e1);
-  const double Dd = dot(e0, w), E = dot(e1, w);
-  const double det = A * C - B * B;
-  double s = B * E - C * Dd;
-  double t = B * Dd - A * E;
-  double num, den;
+template <typename T>
+__device__ __forceinline__ T dist_vf(const vec3<T>& q, const vec3<T>& a, const vec3<T>& b, const vec3<T>& c) {
+  const vec3<T> e0 = sub(b, a), e1 = sub(c, a), w = sub(a, q);
+  const T A = dot(e0, e0), B = dot(e0, e1), C = dot(e1, e1);
+  const T Dd = dot(e0, w), E = dot(e1, w);
+  const T det = A * C - B * B;
+  T s = B * E - C * Dd;
+  T t = B * Dd - A * E;
+  T num, den;
   int kind;
   if (s + t < det) {
     if (s < 0) {
       if (t < 0 && Dd < 0) { kind = R_S; num = -Dd; den = A; }
       else { kind = R_T; num = -E; den = C; }
     } else if (t < 0) { kind = R_S; num = -Dd; den = A; }
-    else { kind = R_IN; num = 1.0; den = det; }
+    else { kind = R_IN; num = T(1); den = det; }
   } else {
-    const double ee = A - 2.0 * B + C;
+    const T ee = A - T(2) * B + C;
     if (s < 0) {
-      const double t0 = B + Dd, t1 = C + E;
+      const T t0 = B + Dd, t1 = C + E;
       if (t1 > t0) { kind = R_EDGE; num = t1 - t0; den = ee; }
       else { kind = R_T; num = -E; den = C; }
     } else if (t < 0) {
@@ -59,64 +77,65 @@ __device__ __forceinline__ double dist_vf(const d3& q, const d3& a, const d3& b,
       else { kind = R_S; num = -E; den = C; }
     } else { kind = R_EDGE; num = C + E - B - Dd; den = ee; }
   }
-  const double r = num / den;
-  const double rc = r < 0 ? 0.0 : (r > 1 ? 1.0 : r);  // clamp_zero_one
-  if (kind == R_S) { s = rc; t = 0.0; }
-  else if (kind == R_T) { s = 0.0; t = rc; }
+  const T r = cr_div(num, den);
+  const T rc = r < 0 ? T(0) : (r > 1 ? T(1) : r);  // clamp_zero_one
+  if (kind == R_S) { s = rc; t = T(0); }
+  else if (kind == R_T) { s = T(0); t = rc; }
   else if (kind == R_IN) { s *= r; t *= r; }
-  else { s = rc; t = 1.0 - s; }
-  const d3 pt = {(a.x + e0.x * s) + e1.x * t, (a.y + e0.y * s) + e1.y * t, (a.z + e0.z * s) + e1.z * t};
-  const d3 dv = sub(q, pt);
-  return __builtin_sqrt(dot(dv, dv));
+  else { s = rc; t = T(1) - s; }
+  const vec3<T> pt = {(a.x + e0.x * s) + e1.x * t, (a.y + e0.y * s) + e1.y * t, (a.z + e0.z * s) + e1.z * t};
+  const vec3<T> dv = sub(q, pt);
+  return cr_sqrt(dot(dv, dv));
 }
 
 // ---------------------------------------------------------------------------
 // EE distance: |shortest vector between segments p and q|
 // (Projection.hpp:241-302 then NarrowPhase.cpp:745-746)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double clamp01(double x) { return smin(1.0, smax(x, 0.0)); }
+template <typename T> __device__ __forceinline__ T clamp01(T x) { return smin(T(1), smax(x, T(0))); }
 
-__device__ __forceinline__ double dist_ee(const d3& p0, const d3& p1, const d3& q0, const d3& q1) {
-  const d3 d1 = sub(p1, p0), d2 = sub(q1, q0), r = sub(p0, q0);
-  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
-  const double c = dot(d1, r), b = dot(d1, d2);
-  const double denom = a * e - b * b;
-  double s = 0.0, t;
-  if (denom != 0.0) s = clamp01((b * f - c * e) / denom);
-  const double tnom = b * s + f;
-  double num, den;
+template <typename T>
+__device__ __forceinline__ T dist_ee(const vec3<T>& p0, const vec3<T>& p1, const vec3<T>& q0, const vec3<T>& q1) {
+  const vec3<T> d1 = sub(p1, p0), d2 = sub(q1, q0), r = sub(p0, q0);
+  const T a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const T c = dot(d1, r), b = dot(d1, d2);
+  const T denom = a * e - b * b;
+  T s = T(0), t;
+  if (denom != T(0)) s = clamp01(cr_div(b * f - c * e, denom));
+  const T tnom = b * s + f;
+  T num, den;
   int kind;  // 0: t=0, s=clamp(-c/a); 1: t=1, s=clamp((b-c)/a); 2: t=tnom/e
-  if (tnom < 0 || e == 0) { kind = 0; num = -c; den = a; t = 0.0; }
-  else if (tnom > e) { kind = 1; num = b - c; den = a; t = 1.0; }
+  if (tnom < 0 || e == 0) { kind = 0; num = -c; den = a; t = T(0); }
+  else if (tnom > e) { kind = 1; num = b - c; den = a; t = T(1); }
   else { kind = 2; num = tnom; den = e; }
-  if (kind == 2 || a != 0.0) {
-    const double q = num / den;
+  if (kind == 2 || a != T(0)) {
+    const T q = cr_div(num, den);
     if (kind == 2) t = q;
     else s = clamp01(q);
   } else {
-    s = 0.0;
+    s = T(0);
   }
-  const d3 v = {(q0.x + t * d2.x) - (p0.x + s * d1.x), (q0.y + t * d2.y) - (p0.y + s * d1.y),
-                (q0.z + t * d2.z) - (p0.z + s * d1.z)};
-  return __builtin_sqrt(dot(v, v));
+  const vec3<T> v = {(q0.x + t * d2.x) - (p0.x + s * d1.x), (q0.y + t * d2.y) - (p0.y + s * d1.y),
+                     (q0.z + t * d2.z) - (p0.z + s * d1.z)};
+  return cr_sqrt(dot(v, v));
 }
 
 // ---------------------------------------------------------------------------
 // 2D VF distance: |p - closest point of segment (e0,e1)| (Projection.hpp:193-204)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double dist_pe2(const d2& p, const d2& e0, const d2& e1) {
-  const d2 e = sub(e1, e0);
-  const double len2 = dot(e, e);
-  d2 pt = e0;
-  if (len2 > 0.0 || !(len2 <= 0.0)) {  // i.e. not (len2 <= 0)
-    const d2 pe = sub(p, e0);
-    const double t = dot(pe, e) / len2;
-    if (t < 0.0) pt = e0;
-    else if (t > 1.0) pt = e1;
+template <typename T> __device__ __forceinline__ T dist_pe2(const vec2<T>& p, const vec2<T>& e0, const vec2<T>& e1) {
+  const vec2<T> e = sub(e1, e0);
+  const T len2 = dot(e, e);
+  vec2<T> pt = e0;
+  if (len2 > T(0) || !(len2 <= T(0))) {  // i.e. not (len2 <= 0)
+    const vec2<T> pe = sub(p, e0);
+    const T t = cr_div(dot(pe, e), len2);
+    if (t < T(0)) pt = e0;
+    else if (t > T(1)) pt = e1;
     else pt = {e0.x + t * e.x, e0.y + t * e.y};
   }
-  const d2 dv = sub(p, pt);
-  return __builtin_sqrt(dot(dv, dv));
+  const vec2<T> dv = sub(p, pt);
+  return cr_sqrt(dot(dv, dv));
 }
 
 // ---------------------------------------------------------------------------
@@ -131,16 +150,19 @@ __device__ __forceinline__ double dist_pe2(const d2& p, const d2& e0, const d2& 
 // is unused and the miss is returned without evaluating it.  g and the first
 // t_l are only needed once the pair is not already within the gap.  Returns 1
 // on hit (toi set), 0 otherwise; evals counts the distance evaluations made.
+// With T = float every quantity is a float as in NarrowPhaseACCD<float,DIM>,
+// except the step t_l = 0.9 * (...) / (...), whose double literal makes the
+// reference compute the product and quotient in double before storing a float.
 // ---------------------------------------------------------------------------
-template <int D, bool VF> struct Stencil;
+template <int D, bool VF, typename T> struct Stencil;
 
-template <bool VF> struct Stencil<3, VF> {
-  d3 x[4], p[4];
-  __device__ __forceinline__ double dist() const {
+template <bool VF, typename T> struct Stencil<3, VF, T> {
+  vec3<T> x[4], p[4];
+  __device__ __forceinline__ T dist() const {
     if (VF) return dist_vf(x[0], x[1], x[2], x[3]);
     return dist_ee(x[0], x[1], x[2], x[3]);
   }
-  __device__ __forceinline__ void step(double tl) {
+  __device__ __forceinline__ void step(T tl) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       x[i].x = x[i].x + tl * p[i].x;
@@ -149,9 +171,9 @@ template <bool VF> struct Stencil<3, VF> {
     }
   }
   // v0[i], v1[i] are the stencil's vertices at t0 and t1; returns l_p, sets fin
-  __device__ __forceinline__ double setup(const d3* v0, const d3* v1, bool& fin) {
-    const double w = 1.0 / 4.0;
-    d3 pb = {0.0, 0.0, 0.0};
+  __device__ __forceinline__ T setup(const vec3<T>* v0, const vec3<T>* v1, bool& fin) {
+    const T w = T(1) / T(4);
+    vec3<T> pb = {T(0), T(0), T(0)};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       p[i] = sub(v1[i], v0[i]);
@@ -164,34 +186,34 @@ template <bool VF> struct Stencil<3, VF> {
     // maxima are taken over the squared norms and rooted once per group: a
     // correctly rounded square root is monotone, so sqrt(max) = max(sqrt)
     // bit for bit (NaNs are skipped by smax either way).
-    double lf = 0.0, ls = 0.0, sum = 0.0;
+    T lf = T(0), ls = T(0), sum = T(0);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       p[i] = sub(p[i], pb);
       sum += (p[i].x + p[i].y) + p[i].z;
-      const double n2 = dot(p[i], p[i]);
+      const T n2 = dot(p[i], p[i]);
       const bool first = VF ? (i == 0) : (i <= 1);
       if (first) lf = smax(lf, n2);
       else ls = smax(ls, n2);
     }
     fin = __builtin_isfinite(sum);
-    return __builtin_sqrt(lf) + __builtin_sqrt(ls);
+    return cr_sqrt(lf) + cr_sqrt(ls);
   }
 };
 
-template <> struct Stencil<2, true> {
-  d2 x[3], p[3];
-  __device__ __forceinline__ double dist() const { return dist_pe2(x[0], x[1], x[2]); }
-  __device__ __forceinline__ void step(double tl) {
+template <typename T> struct Stencil<2, true, T> {
+  vec2<T> x[3], p[3];
+  __device__ __forceinline__ T dist() const { return dist_pe2(x[0], x[1], x[2]); }
+  __device__ __forceinline__ void step(T tl) {
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       x[i].x = x[i].x + tl * p[i].x;
       x[i].y = x[i].y + tl * p[i].y;
     }
   }
-  __device__ __forceinline__ double setup(const d2* v0, const d2* v1, bool& fin) {
-    const double w = 1.0 / 3.0;
-    d2 pb = {0.0, 0.0};
+  __device__ __forceinline__ T setup(const vec2<T>* v0, const vec2<T>* v1, bool& fin) {
+    const T w = T(1) / T(3);
+    vec2<T> pb = {T(0), T(0)};
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       p[i] = sub(v1[i], v0[i]);
@@ -199,38 +221,38 @@ template <> struct Stencil<2, true> {
       pb.x = pb.x + w * p[i].x;
       pb.y = pb.y + w * p[i].y;
     }
-    double lf = 0.0, ls = 0.0, sum = 0.0;  // squared norms, rooted once per group (as in 3D)
+    T lf = T(0), ls = T(0), sum = T(0);  // squared norms, rooted once per group (as in 3D)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       p[i] = sub(p[i], pb);
       sum += p[i].x + p[i].y;
-      const double n2 = dot(p[i], p[i]);
+      const T n2 = dot(p[i], p[i]);
       if (i == 0) lf = smax(lf, n2);
       else ls = smax(ls, n2);
     }
     fin = __builtin_isfinite(sum);
-    return __builtin_sqrt(lf) + __builtin_sqrt(ls);
+    return cr_sqrt(lf) + cr_sqrt(ls);
   }
 };
 
-template <int D, bool VF, typename V>
-__device__ __forceinline__ int accd(const V* v0, const V* v1, double xsi, double& toi, int& evals) {
-  Stencil<D, VF> st;
+template <int D, bool VF, typename T, typename V>
+__device__ __forceinline__ int accd(const V* v0, const V* v1, T xsi, T& toi, int& evals) {
+  Stencil<D, VF, T> st;
   bool fin;
-  const double l_p = st.setup(v0, v1, fin);
+  const T l_p = st.setup(v0, v1, fin);
   evals = 0;
   if (l_p <= 0) return 0;
-  double d = st.dist();
+  T d = st.dist();
   evals = 1;
-  toi = 0.0;
+  toi = T(0);
   if (d <= xsi) return 1;
-  const double g = 1.0 * (d * d - xsi * xsi) / (d - xsi);
-  double tl = (1.0 - 1.0) * (d * d - xsi * xsi) / ((d + xsi) * l_p);
-  bool skip = tl == 0.0 && fin;  // iteration 0 leaves x unchanged: its distance is d0
+  const T g = cr_div(T(1) * (d * d - xsi * xsi), d - xsi);
+  T tl = cr_div((T(1) - T(1)) * (d * d - xsi * xsi), (d + xsi) * l_p);
+  bool skip = tl == T(0) && fin;  // iteration 0 leaves x unchanged: its distance is d0
   // One distance evaluation site per iteration (reference loop NarrowPhase.cpp:684-707).
   for (int iter = 0; iter < 1000; ++iter) {
-    const bool can_break = toi > 0.0 && iter > 0;
-    if (!can_break && toi + tl > 1.0) return 0;  // the reference's distance here is unused
+    const bool can_break = toi > T(0) && iter > 0;
+    if (!can_break && toi + tl > T(1)) return 0;  // the reference's distance here is unused
     if (!skip) {
       st.step(tl);
       d = st.dist();
@@ -238,12 +260,12 @@ __device__ __forceinline__ int accd(const V* v0, const V* v1, double xsi, double
     }
     skip = false;
     if (can_break) {
-      const double eps = (d * d - xsi * xsi) / (d + xsi);
+      const T eps = cr_div(d * d - xsi * xsi, d + xsi);
       if (eps < g) return 1;
     }
     toi += tl;
-    if (toi > 1.0) return 0;
-    tl = 0.9 * (d * d - xsi * xsi) / ((d + xsi) * l_p);
+    if (toi > T(1)) return 0;
+    tl = (T)(0.9 * (double)(d * d - xsi * xsi) / (double)((d + xsi) * l_p));
   }
   return 1;  // max_iter reached: the reference returns true (its assert is compiled out)
 }

@@ -20,7 +20,9 @@ struct SceneArgs {
 };
 
 // rec: per-vertex records (both time steps); v1 (may be null): V1 per vertex, v1_stride(dim) doubles each.
-void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1 = nullptr);
+// f32: round the coordinates to float (float-precision tree)
+void launch_pack(int dim, const double* V0, const double* V1, int nv, double* rec, hipStream_t s, double* v1 = nullptr,
+                 bool f32 = false);
 void launch_minmax_i32(const int* a, long long n, int* out2, hipStream_t s);
 void launch_fill_u8(uint8_t* p, long long n, uint8_t v, hipStream_t s);
 void launch_fill_i32(int* p, long long n, int v, hipStream_t s);
@@ -30,7 +32,7 @@ void launch_reptri(const int* P, int np, int pdim, int nv, int* owner, unsigned 
 int leaf_box_blocks(int np);
 void launch_leaf_boxes(int dim, const double* rec, const int* P, int np, int pdim, const int* active, double eta,
                        void* boxes, uint8_t* act, double* partial, unsigned long long* cbounds, const SceneArgs& S,
-                       hipStream_t s);
+                       hipStream_t s, bool f32 = false);  // f32: padded bounds rounded to float
 void launch_bounds_init(unsigned long long* b, int nscenes, int dim, unsigned long long lo0, unsigned long long hi0,
                         hipStream_t s);
 void launch_morton(int dim, const void* boxes, int np, const unsigned long long* cbounds, const SceneArgs& S,
@@ -112,6 +114,7 @@ struct AccdLaunch {
   int num_cus;
   int blocks_per_cu;
   int one_sided = 0;                 // VF: CTCD with the wrong-side test instead of ACCD
+  int f32 = 0;                       // float-precision tree: float ACCD; CTCD TOIs rounded to float
   // direct hit list: hits appended through counters[C_HITS] (zeroed by the
   // caller), records past hit_cap dropped.  Null, or a queue of 2^32 or
   // more stencils: only cand_toi / chunk_hits, and launch_hits builds the list.

@@ -45,10 +45,14 @@ int Ctx::kdbvh_from_leaves() {
   HIPCHECK_RC(hipMemcpy2DAsync(sw.data(), sizeof(double) * 2 * dim, d_boxes, pitch, sizeof(double) * 2 * dim,
                                (size_t)np, hipMemcpyDeviceToHost, stream));
   HIPCHECK_RC(hipStreamSynchronize(stream));
-  // AlignedBox::center(): (min + max) / 2
+  // AlignedBox::center(): (min + max) / 2, in T (a float tree's boxes hold floats)
   std::vector<double> cen((size_t)np * dim);
+  const bool f32 = precision == MCLG_PRECISION_F32;
   for (size_t i = 0; i < (size_t)np; ++i)
-    for (int k = 0; k < dim; ++k) cen[i * dim + k] = (sw[i * 2 * dim + k] + sw[i * 2 * dim + dim + k]) / 2.0;
+    for (int k = 0; k < dim; ++k) {
+      const double lo = sw[i * 2 * dim + k], hi = sw[i * 2 * dim + dim + k];
+      cen[i * dim + k] = f32 ? (double)(((float)lo + (float)hi) / 2.0f) : (lo + hi) / 2.0;
+    }
   if (dim == 3) kd_order<3>(cen.data(), np, kd_rank.data(), np > 1 ? kd_gapnode.data() : nullptr);
   else kd_order<2>(cen.data(), np, kd_rank.data(), np > 1 ? kd_gapnode.data() : nullptr);
   CHECK_RC(grow((void**)&d_rank, &cap_rank, sizeof(int) * (size_t)np, "rank"));
@@ -156,6 +160,26 @@ int mclg_get_orientation(const mclg_ctx* h, int* mode) {
   const Ctx* c = reinterpret_cast<const Ctx*>(h);
   if (!c || !mode) return MCLG_ERR_ARG;
   *mode = c->orient;
+  return MCLG_OK;
+}
+
+int mclg_set_precision(mclg_ctx* h, int precision) {
+  Ctx* c = reinterpret_cast<Ctx*>(h);
+  if (!c) return MCLG_ERR_ARG;
+  if (precision != MCLG_PRECISION_F64 && precision != MCLG_PRECISION_F32)
+    return c->fail(MCLG_ERR_ARG, "set_precision: unknown precision %d", precision);
+  // T is fixed for the life of a tree (a template parameter in the reference):
+  // the precision is chosen before the first update
+  if (precision != c->precision && c->tree_gen != 0)
+    return c->fail(MCLG_ERR_STATE, "set_precision: the context has already been updated; choose the precision first");
+  c->precision = precision;
+  return MCLG_OK;
+}
+
+int mclg_get_precision(const mclg_ctx* h, int* precision) {
+  const Ctx* c = reinterpret_cast<const Ctx*>(h);
+  if (!c || !precision) return MCLG_ERR_ARG;
+  *precision = c->precision;
   return MCLG_OK;
 }
 

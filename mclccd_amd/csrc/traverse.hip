@@ -15,6 +15,7 @@
 //   stencil queues.
 // k_accd: one lane per stencil, FP64 ACCD (NarrowPhase.cpp:634-711).
 #include <cmath>
+#include <type_traits>
 
 #include "ctcd.hpp"
 #include "pairs.hpp"
@@ -532,6 +533,7 @@ struct AccdArgs {
   uint8_t* hit_vf;
   double* hit_toi;
   unsigned long long hit_cap;
+  int f32;                  // float-precision tree (CTCD: TOI rounded to float)
 };
 
 constexpr int AC_WAVES = 4;
@@ -583,8 +585,23 @@ __device__ __forceinline__ void stage_hits(HitStage<N>& S, int w, int& cnt, bool
 // A.hit_sten the hits are also appended to the hit list (staged per wave,
 // one counter atomic per flush), so the compaction pass only runs when the
 // list overflowed hit_cap.
-template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
+template <typename T> struct VecOf;
+template <> struct VecOf<double> {
+  template <int D> using V = typename VLoad<D>::V;
+};
+template <> struct VecOf<float> {
+  template <int D> using V = typename std::conditional<D == 3, vec3<float>, vec2<float>>::type;
+};
+__device__ __forceinline__ void narrow(const d3& a, vec3<float>& o) { o = {(float)a.x, (float)a.y, (float)a.z}; }
+__device__ __forceinline__ void narrow(const d2& a, vec2<float>& o) { o = {(float)a.x, (float)a.y}; }
+__device__ __forceinline__ void narrow(const d3& a, d3& o) { o = a; }
+__device__ __forceinline__ void narrow(const d2& a, d2& o) { o = a; }
+
+// T = double: the FP64 path; T = float: BVHTree<float,DIM>'s float ACCD on the
+// float-valued vertex records of a float-precision tree (exact conversions).
+template <int D, bool VF, typename T> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
   using V = typename VLoad<D>::V;
+  using VT = typename VecOf<T>::template V<D>;
   __shared__ HitStage<ac_stage<D>()> S;
   const int w = threadIdx.x / kWave, lane = lane_id();
   long long evals = 0;
@@ -599,12 +616,19 @@ template <int D, bool VF> __global__ __launch_bounds__(AC_WAVES * kWave) void k_
     if (i < n) {
       const int4 sv = A.q[i];
       const int id[4] = {sv.x, sv.y, sv.z, sv.w};
-      V v0[D + 1], v1[D + 1];
+      VT v0[D + 1], v1[D + 1];
 #pragma unroll
-      for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
+      for (int k = 0; k < D + 1; ++k) {
+        V a, b;
+        VLoad<D>::load(A.rec, id[k], a, b);
+        narrow(a, v0[k]);
+        narrow(b, v1[k]);
+      }
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
       int ev = 0;
-      hit = accd<D, VF>(v0, v1, eta, toi, ev);
+      T t = T(-1);
+      hit = accd<D, VF, T>(v0, v1, (T)eta, t, ev);
+      toi = (double)t;
       if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
       evals += ev;
     }
@@ -649,6 +673,7 @@ template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(Accd
       for (int k = 0; k < D + 1; ++k) VLoad<D>::load(A.rec, id[k], v0[k], v1[k]);
       const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
       hit = ctcd::Query<D>::run(v0, v1, eta, toi, cosr, sinr) == 1;
+      if (A.f32 && hit) toi = (double)(float)toi;  // NarrowPhase.cpp:425-444: t_impact = t (a float)
       if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
     }
     const uint64_t m = __ballot(hit != 0);
@@ -749,7 +774,7 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   AccdArgs A;
   A.q = L.q; A.n_dev = L.n_dev; A.cap = L.cap; A.rec = L.rec; A.eta = L.eta;
   A.vert_scene = L.vert_scene; A.scene_eta = L.scene_eta; A.cand_toi = L.cand_toi;
-  A.chunk_hits = L.chunk_hits; A.counters = L.counters;
+  A.chunk_hits = L.chunk_hits; A.counters = L.counters; A.f32 = L.f32;
   const bool direct = L.hit_sten && accd_direct_hits(L.cap);
   A.hit_sten = direct ? L.hit_sten : nullptr; A.hit_vf = L.hit_vf; A.hit_toi = L.hit_toi; A.hit_cap = L.hit_cap;
   if (!direct && (!A.cand_toi || !A.chunk_hits)) return;  // caller error: no output at all
@@ -763,11 +788,20 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
     else k_ctcd<2><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
     return;
   }
+  if (L.f32) {
+    if (dim == 3) {
+      if (vf) k_accd<3, true, float><<<g, AC_WAVES * kWave, 0, s>>>(A);
+      else k_accd<3, false, float><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    } else {
+      k_accd<2, true, float><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    }
+    return;
+  }
   if (dim == 3) {
-    if (vf) k_accd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
-    else k_accd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    if (vf) k_accd<3, true, double><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    else k_accd<3, false, double><<<g, AC_WAVES * kWave, 0, s>>>(A);
   } else {
-    k_accd<2, true><<<g, AC_WAVES * kWave, 0, s>>>(A);
+    k_accd<2, true, double><<<g, AC_WAVES * kWave, 0, s>>>(A);
   }
 }
 

@@ -95,12 +95,19 @@ def _dev_check(t, dtype_name):
 
 
 class BVHTree:
-    """``mcl::BVHTree<double, dim>`` on one GPU."""
+    """``mcl::BVHTree<T, dim>`` on one GPU; T = double (default) or, with
+    ``dtype=numpy.float32``, float: coordinates and etas rounded to float,
+    boxes padded in float and ACCD in float arithmetic (mclg_set_precision).
+    Inputs are float64 either way (float32 host arrays are widened exactly)."""
 
-    def __init__(self, dim: int = 3):
+    def __init__(self, dim: int = 3, dtype=np.float64):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
+        dt = np.dtype(dtype)
+        if dt not in (np.dtype(np.float64), np.dtype(np.float32)):
+            raise ValueError("dtype must be float64 or float32")
         self.dim = dim
+        self.dtype = dt
         self.options = Options()
         self.append_pair = None
         self.append_discrete = None
@@ -111,6 +118,8 @@ class BVHTree:
         if rc != _lib.OK:
             raise _lib.MclgError(rc, lib.mclg_last_error(None).decode())
         self._h = h
+        if dt == np.dtype(np.float32):
+            self._check(lib.mclg_set_precision(h, _lib.PRECISION_F32))
         self.nv = 0
         self.np = 0
         self.serial = 0  # counts updates and traversals (the adjoint refers to the last traverse)

@@ -36,6 +36,9 @@
 //                               conservativeness along the path
 //   projections                 dense-sampling minimum distance
 //   tri-tri, 2D edge-edge       exact rational predicates (Python fractions)
+//   KdBVH partition             median-split property per node (tests/test_kdbvh_order.py)
+//   float tree (Options::f32)   ACCD vs a numpy float32 restatement, boxes vs
+//                               numpy float32 (tests/test_oracle_float.py)
 //   Eigen 3.4 reduction order   parity unpinned: no reference test fixes it
 #pragma once
 
@@ -49,6 +52,7 @@
 #include <limits>
 #include <set>
 #include <string>
+#include <type_traits>
 #include <unordered_set>
 #include <utility>
 #include <vector>
@@ -68,6 +72,7 @@ namespace mclo {
 // itself, a forward-mode dual number (oracle/dual.hpp) for the reference TOI
 // gradient.  With T = double every expression is the plain FP64 one.
 inline double osqrt(double x) { return std::sqrt(x); }
+inline float osqrt(float x) { return std::sqrt(x); }
 
 template <int D, typename T = double> inline T vdot(const T* a, const T* b) {
   if constexpr (D == 3) return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
@@ -189,18 +194,18 @@ inline void point_on_triangle(const T* q, const T* a, const T* b, const T* c, T*
       const T t0 = B + Dd, t1 = C + E;
       if (t1 > t0) {
         region = 6;
-        const T num = t1 - t0, den = A - 2.0 * B + C;
+        const T num = t1 - t0, den = A - T(2.0) * B + C;
         s = clamp01(num / den); t = T(1.0) - s;
       } else { region = 7; t = clamp01(-E / C); s = 0.0; }
     } else if (t < 0.0) {
       if (A + Dd > B + E) {
         region = 8;
-        const T num = C + E - B - Dd, den = A - 2.0 * B + C;
+        const T num = C + E - B - Dd, den = A - T(2.0) * B + C;
         s = clamp01(num / den); t = T(1.0) - s;
       } else { region = 9; s = clamp01(-E / C); t = 0.0; quirk = true; }
     } else {
       region = 10;
-      const T num = C + E - B - Dd, den = A - 2.0 * B + C;
+      const T num = C + E - B - Dd, den = A - T(2.0) * B + C;
       s = clamp01(num / den); t = T(1.0) - s;
     }
   }
@@ -295,11 +300,13 @@ struct AccdStats {
 template <int D, typename T = double>
 inline bool additive_ccd(const T (*v0)[D], const T (*v1)[D], double eta, bool is_vf, T& toi, AccdStats* st = nullptr) {
   constexpr int ns = D + 1;
-  const double s = 1, t_c = 1;
+  // s, t_c and 1/ns are T in the reference: float for the float tree
+  using S = std::conditional_t<std::is_same_v<T, float>, float, double>;
+  const S s = 1, t_c = 1;
   const T xsi = eta;
   T p[ns][D], x[ns][D], pbar[D];
   for (int k = 0; k < D; ++k) pbar[k] = 0.0;
-  const double inv_ns = 1.0 / double(ns);
+  const S inv_ns = S(1) / S(ns);
   for (int i = 0; i < ns; ++i)
     for (int k = 0; k < D; ++k) {
       p[i][k] = v1[i][k] - v0[i][k];
@@ -550,6 +557,10 @@ struct Options {
   double box_eta = 1e-6, vf_ccd_eta = 1e-6, ee_ccd_eta = 1e-6;
   bool reptri = true, threaded = true, discrete = true, continuous = true;
   bool vf_one_sided = false;  // CTCD for vertex-face stencils (ctcd_oracle.hpp)
+  // BVHTree<float,DIM> (BVHTree.cpp:504-507): float coordinates and etas, boxes
+  // padded in float, float KdBVH centres, NarrowPhaseACCD<float,DIM>; CTCD and
+  // the discrete tests widen to double (NarrowPhase.cpp:208-219,259-269,425-444)
+  bool f32 = false;
 };
 
 struct Stencil {
@@ -590,10 +601,18 @@ template <int D> class Tree {
   int nthreads = 0;  // 0 = OpenMP default
 
   // BVHTree.cpp:27-126
-  void update(const double* V0, const double* V1, const int* P, int np, int pdim, const int* active = nullptr) {
+  void update(const double* V0_, const double* V1_, const int* P, int np, int pdim, const int* active = nullptr) {
     if (np == 0) {
       leaves.clear(); vols.clear(); children.clear(); objects.clear(); rank.clear();
       return;
+    }
+    std::vector<double> r0, r1;
+    const double* V0 = as_t(V0_, P, np, pdim, r0);
+    const double* V1 = as_t(V1_, P, np, pdim, r1);
+    if (options.f32) {  // the float tree's options are floats
+      options.box_eta = (float)options.box_eta;
+      options.vf_ccd_eta = (float)options.vf_ccd_eta;
+      options.ee_ccd_eta = (float)options.ee_ccd_eta;
     }
     options.vf_ccd_eta = std::min(options.vf_ccd_eta, options.box_eta);
     options.ee_ccd_eta = std::min(options.ee_ccd_eta, options.box_eta);
@@ -618,12 +637,33 @@ template <int D> class Tree {
         L.box.sw.extend(a); L.box.sw.extend(b);
         if (active != nullptr && active[vi]) L.box.active = true;
       }
+      if (options.f32) {  // float - float, float + float
+        const float e = (float)eta;
+        for (int k = 0; k < D; ++k) {
+          L.box.t0.lo[k] = (float)L.box.t0.lo[k] - e; L.box.t1.lo[k] = (float)L.box.t1.lo[k] - e;
+          L.box.sw.lo[k] = (float)L.box.sw.lo[k] - e;
+          L.box.t0.hi[k] = (float)L.box.t0.hi[k] + e; L.box.t1.hi[k] = (float)L.box.t1.hi[k] + e;
+          L.box.sw.hi[k] = (float)L.box.sw.hi[k] + e;
+        }
+        continue;
+      }
       for (int k = 0; k < D; ++k) {
         L.box.t0.lo[k] -= eta; L.box.t1.lo[k] -= eta; L.box.sw.lo[k] -= eta;
         L.box.t0.hi[k] += eta; L.box.t1.hi[k] += eta; L.box.sw.hi[k] += eta;
       }
     }
     build_kdbvh();
+  }
+
+  // The coordinates the tree computes on: V itself, or for the float tree
+  // float(V) (a copy of the vertices P uses; others are never read).
+  const double* as_t(const double* V, const int* P, int np, int pdim, std::vector<double>& buf) const {
+    if (!options.f32 || !V) return V;
+    int nv = 0;
+    for (long long i = 0; i < (long long)np * pdim; ++i) nv = std::max(nv, P[i] + 1);
+    buf.resize((size_t)nv * D);
+    for (size_t i = 0; i < buf.size(); ++i) buf[i] = (float)V[i];
+    return buf.data();
   }
 
   // BVHTree.cpp:53-79 — first-seen vertex / edge ownership, in face order.
@@ -654,7 +694,9 @@ template <int D> class Tree {
     if (n < 2) { rank.assign(n, 0); return; }
     std::vector<VIPair> cent(n);
     for (int i = 0; i < n; ++i) {
-      for (int k = 0; k < D; ++k) cent[i].c[k] = (leaves[i].box.sw.lo[k] + leaves[i].box.sw.hi[k]) / 2.0;
+      for (int k = 0; k < D; ++k)  // AlignedBox::center() in T
+        cent[i].c[k] = options.f32 ? (double)(((float)leaves[i].box.sw.lo[k] + (float)leaves[i].box.sw.hi[k]) / 2.0f)
+                                   : (leaves[i].box.sw.lo[k] + leaves[i].box.sw.hi[k]) / 2.0;
       cent[i].second = i;
     }
     vols.reserve(n - 1);
@@ -734,9 +776,12 @@ template <int D> class Tree {
   };
 
   // BVHTree.cpp:129-159
-  void traverse(const double* V0, const double* V1, const int* P) {
+  void traverse(const double* V0_, const double* V1_, const int* P) {
     leaf_pairs.clear(); candidates.clear(); hits.clear(); discrete.clear(); events.clear(); stats = AccdStats();
     if (leaves.size() <= 1) return;
+    std::vector<double> r0, r1;
+    const double* V0 = as_t(V0_, P, (int)leaves.size(), D, r0);
+    const double* V1 = as_t(V1_, P, (int)leaves.size(), D, r1);
     std::vector<std::pair<Node, Node>> q;
     q.reserve(leaves.size());
     make_frontlist(Node((int)vols.size() - 1, false), q);
@@ -857,7 +902,17 @@ template <int D> class Tree {
       for (int k = 0; k < D; ++k) { a[i][k] = V0[s.v[i] * D + k]; b[i][k] = V1[s.v[i] * D + k]; }
     double toi = -1;
     const double eta = s.is_vf ? options.vf_ccd_eta : options.ee_ccd_eta;
-    if (s.is_vf && options.vf_one_sided) return ctcd::query_vf<D>(a, b, eta, toi) == 1 ? toi : -1;
+    if (s.is_vf && options.vf_one_sided) {  // the float specialisation widens to double, stores a float
+      if (ctcd::query_vf<D>(a, b, eta, toi) != 1) return -1;
+      return options.f32 ? (double)(float)toi : toi;
+    }
+    if (options.f32) {  // NarrowPhaseACCD<float,DIM>
+      float fa[D + 1][D], fb[D + 1][D], ft = -1;
+      for (int i = 0; i < D + 1; ++i)
+        for (int k = 0; k < D; ++k) { fa[i][k] = (float)a[i][k]; fb[i][k] = (float)b[i][k]; }
+      if (additive_ccd<D, float>(fa, fb, eta, s.is_vf, ft, st)) return ft;
+      return -1;
+    }
     if (additive_ccd<D>(a, b, eta, s.is_vf, toi, st)) return toi;
     return -1;
   }

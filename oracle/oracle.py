@@ -19,7 +19,7 @@ _lib = None
 class OracleOptions(C.Structure):
     _fields_ = [("box_eta", C.c_double), ("vf_ccd_eta", C.c_double), ("ee_ccd_eta", C.c_double),
                 ("reptri", C.c_int), ("threaded", C.c_int), ("discrete", C.c_int), ("continuous", C.c_int),
-                ("vf_one_sided", C.c_int)]
+                ("vf_one_sided", C.c_int), ("f32", C.c_int)]
 
 
 def build() -> str:
@@ -56,6 +56,7 @@ def lib():
         L.oracle_brute_force_pairs.restype = LL
         L.oracle_brute_force_pairs.argtypes = [P, P, LL]
         L.oracle_accd.argtypes = [I, P, P, D, I, P, P]
+        L.oracle_accd_f32.argtypes = [I, P, P, D, I, P, P]
         L.oracle_pair_distance.restype = D
         L.oracle_pair_distance.argtypes = [I, P, I]
         L.oracle_point_on_triangle.argtypes = [P, P, P, P, P]
@@ -106,9 +107,9 @@ class Oracle:
             self._h = None
 
     def set_options(self, box_eta=1e-6, vf_ccd_eta=1e-6, ee_ccd_eta=1e-6, reptri=True, threaded=True,
-                    discrete=True, continuous=True, vf_one_sided=False, nthreads=0):
+                    discrete=True, continuous=True, vf_one_sided=False, nthreads=0, f32=False):
         o = OracleOptions(box_eta, vf_ccd_eta, ee_ccd_eta, int(reptri), int(threaded), int(discrete),
-                          int(continuous), int(vf_one_sided))
+                          int(continuous), int(vf_one_sided), int(f32))
         lib().oracle_set_options(self._h, C.byref(o), int(nthreads))
 
     def etas(self):
@@ -203,6 +204,16 @@ def accd(dim, verts0, verts1, eta, is_vf=True):
     toi = C.c_double(-1.0)
     ev = C.c_longlong(0)
     h = lib().oracle_accd(dim, _p(v0), _p(v1), float(eta), int(is_vf), C.byref(toi), C.byref(ev))
+    return bool(h), toi.value, ev.value
+
+
+def accd_f32(dim, verts0, verts1, eta, is_vf=True):
+    """(hit, toi, distance evaluations) of ACCD at T = float on float(verts)
+    (NarrowPhaseACCD<float,DIM>)."""
+    v0, v1 = _f64(verts0), _f64(verts1)
+    toi = C.c_double(-1.0)
+    ev = C.c_longlong(0)
+    h = lib().oracle_accd_f32(dim, _p(v0), _p(v1), float(eta), int(is_vf), C.byref(toi), C.byref(ev))
     return bool(h), toi.value, ev.value
 
 

@@ -29,6 +29,7 @@ struct oracle_options {
   double box_eta, vf_ccd_eta, ee_ccd_eta;
   int reptri, threaded, discrete, continuous;
   int vf_one_sided;
+  int f32;  // BVHTree<float,DIM> arithmetic
 };
 
 template <int D> static void set_opts(OracleCtx* c, const oracle_options* o, int nthreads) {
@@ -41,6 +42,7 @@ template <int D> static void set_opts(OracleCtx* c, const oracle_options* o, int
   t.options.discrete = o->discrete != 0;
   t.options.continuous = o->continuous != 0;
   t.options.vf_one_sided = o->vf_one_sided != 0;
+  t.options.f32 = o->f32 != 0;
   t.nthreads = nthreads;
 }
 
@@ -238,6 +240,21 @@ int oracle_accd(int dim, const double* verts0, const double* verts1, double eta,
   if (dim == 3) hit = mclo::additive_ccd<3>(reinterpret_cast<const double(*)[3]>(verts0), reinterpret_cast<const double(*)[3]>(verts1), eta, is_vf != 0, t, &st);
   else hit = mclo::additive_ccd<2>(reinterpret_cast<const double(*)[2]>(verts0), reinterpret_cast<const double(*)[2]>(verts1), eta, true, t, &st);
   if (toi) *toi = hit ? t : -1.0;
+  if (dist_evals) *dist_evals = st.dist_evals;
+  return hit ? 1 : 0;
+}
+
+// additive_ccd at T = float (NarrowPhaseACCD<float,DIM>) on float(verts);
+// returns hit, toi written as the float value.
+int oracle_accd_f32(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* toi,
+                    long long* dist_evals) {
+  mclo::AccdStats st;
+  float a[12], b[12], t = -1;
+  for (int i = 0; i < (dim + 1) * dim; ++i) { a[i] = (float)verts0[i]; b[i] = (float)verts1[i]; }
+  bool hit;
+  if (dim == 3) hit = mclo::additive_ccd<3, float>(reinterpret_cast<const float(*)[3]>(a), reinterpret_cast<const float(*)[3]>(b), (float)eta, is_vf != 0, t, &st);
+  else hit = mclo::additive_ccd<2, float>(reinterpret_cast<const float(*)[2]>(a), reinterpret_cast<const float(*)[2]>(b), (float)eta, true, t, &st);
+  if (toi) *toi = hit ? (double)t : -1.0;
   if (dist_evals) *dist_evals = st.dist_evals;
   return hit ? 1 : 0;
 }

@@ -22,6 +22,7 @@ struct oracle_options {
   double box_eta, vf_ccd_eta, ee_ccd_eta;
   int reptri, threaded, discrete, continuous;
   int vf_one_sided;
+  int f32;
 };
 
 extern "C" {
@@ -42,6 +43,8 @@ int oracle_copy_boxes(void* h, double* out, unsigned char* active);
 long long oracle_brute_force_pairs(void* h, int* out, long long cap);
 int oracle_accd(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* toi,
                 long long* dist_evals);
+int oracle_accd_f32(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* toi,
+                    long long* dist_evals);
 int oracle_accd_grad(int dim, const double* verts0, const double* verts1, double eta, int is_vf, double* g0,
                      double* g1, double* toi);
 void oracle_toi_list(int dim, const double* V0, const double* V1, const int* sten, const unsigned char* is_vf,
@@ -81,7 +84,7 @@ struct Scene {
   std::vector<double> V0, V1;
   std::vector<int> F;
   double eta;
-  int reptri = 1, one_sided = 0;
+  int reptri = 1, one_sided = 0, f32 = 0;
   int nprims() const { return (int)F.size() / dim; }
   int nverts() const { return (int)V0.size() / dim; }
 };
@@ -167,7 +170,7 @@ struct Run {
 
 Run run_oracle(const Scene& s, int threaded, int nthreads, bool cand) {
   void* h = oracle_create(s.dim);
-  oracle_options o{s.eta, s.eta, s.eta, s.reptri, threaded, 1, 1, s.one_sided};
+  oracle_options o{s.eta, s.eta, s.eta, s.reptri, threaded, 1, 1, s.one_sided, s.f32};
   oracle_set_options(h, &o, nthreads);
   const int np = s.nprims();
   oracle_update(h, s.V0.data(), s.V1.data(), s.F.data(), np, s.dim, nullptr);
@@ -232,7 +235,9 @@ void check_replay(const Scene& s, const Run& r, bool cand, Rng& rng) {
   const int D = s.dim;
   for (int i = 0; i < np; ++i)
     for (int k = 0; k < D; ++k)
-      cen[(size_t)i * D + k] = (r.boxes[(size_t)i * 6 * D + k] + r.boxes[(size_t)i * 6 * D + D + k]) / 2.0;
+      cen[(size_t)i * D + k] = s.f32 ? (double)(((float)r.boxes[(size_t)i * 6 * D + k] +
+                                                 (float)r.boxes[(size_t)i * 6 * D + D + k]) / 2.0f)
+                                     : (r.boxes[(size_t)i * 6 * D + k] + r.boxes[(size_t)i * 6 * D + D + k]) / 2.0;
   if (D == 3) mclg::host::kd_order<3>(cen.data(), np, rank.data(), np > 1 ? gap.data() : nullptr);
   else mclg::host::kd_order<2>(cen.data(), np, rank.data(), np > 1 ? gap.data() : nullptr);
   expect(rank == r.rank, s.name, "host KdBVH rank == oracle rank");
@@ -323,14 +328,27 @@ void check_narrow(const Scene& s, const Run& serial) {
     expect(nvf_hits == want, s.name, "CTCD list hits == traverse's VF hits");
     return;
   }
-  oracle_toi_list(D, s.V0.data(), s.V1.data(), serial.csten.data(), serial.cvf.data(), (long long)serial.cvf.size(),
-                  s.eta, s.eta, toi.data(), nullptr);
+  if (s.f32) {  // the float tree: float ACCD per candidate on float(V)
+    for (size_t i = 0; i < toi.size(); ++i) {
+      double a[12], b[12];
+      for (int j = 0; j <= D; ++j)
+        for (int k = 0; k < D; ++k) {
+          a[j * D + k] = s.V0[(size_t)serial.csten[4 * i + j] * D + k];
+          b[j * D + k] = s.V1[(size_t)serial.csten[4 * i + j] * D + k];
+        }
+      if (!oracle_accd_f32(D, a, b, (float)s.eta, serial.cvf[i] || D == 2, &toi[i], nullptr)) toi[i] = -1;
+    }
+  } else {
+    oracle_toi_list(D, s.V0.data(), s.V1.data(), serial.csten.data(), serial.cvf.data(), (long long)serial.cvf.size(),
+                    s.eta, s.eta, toi.data(), nullptr);
+  }
   std::multiset<HitKey> from_list;
   for (size_t i = 0; i < toi.size(); ++i)
     if (toi[i] >= 0)
       from_list.insert({serial.csten[4 * i], serial.csten[4 * i + 1], serial.csten[4 * i + 2], serial.csten[4 * i + 3],
                         serial.cvf[i], toi[i]});
   expect(from_list == hit_set(serial), s.name, "toi list over candidates == traverse hits");
+  if (s.f32) return;  // the single-stencil and gradient entries are FP64
   bool same = true, grad_ok = true;
   const size_t step = std::max<size_t>(1, serial.hvf.size() / 64);
   for (size_t i = 0; i < serial.hvf.size(); i += step) {
@@ -387,6 +405,12 @@ int main() {
   Scene ctcd = soup("soup3-ctcd", 3, 300, 13, 0.2);
   ctcd.one_sided = 1;
   scenes.push_back(ctcd);
+  Scene f3 = soup("soup3-f32", 3, 300, 19, 0.2);
+  f3.f32 = 1;
+  scenes.push_back(f3);
+  Scene f2 = soup("edges2-f32", 2, 400, 23, 0.2);
+  f2.f32 = 1;
+  scenes.push_back(f2);
   Scene ctcd2 = soup("edges2-ctcd", 2, 300, 17, 0.2);
   ctcd2.one_sided = 1;
   scenes.push_back(ctcd2);

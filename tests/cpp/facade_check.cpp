@@ -173,30 +173,25 @@ int main(int argc, char** argv) {
   StopFirst sf;
   tree.traverse(&sf);
   std::fprintf(out, "visit_stop %ld\n", sf.objects);
-  // 7. the float instantiation: vertices rounded to float, then the float tree
-  // vs the double tree on the same (widened) coordinates
+  // 7. the float instantiation (MCLG_PRECISION_F32): float vertices, the
+  // serial order (threaded = false), every callback logged for comparison
+  // with the oracle's float tree; leaf boxes checked against float arithmetic
   {
     std::vector<float> F0(V0.begin(), V0.end()), F1(V1.begin(), V1.end());
-    std::vector<double> D0(F0.begin(), F0.end()), D1(F1.begin(), F1.end());
     mcl::BVHTree<float, 3> tf;
-    mcl::BVHTree<double, 3> td;
     tf.options.box_eta = tf.options.vf_ccd_eta = tf.options.ee_ccd_eta = (float)eta;
-    td.options.box_eta = td.options.vf_ccd_eta = td.options.ee_ccd_eta = (double)(float)eta;
-    std::map<std::vector<int>, std::vector<double>> hf, hd;
+    tf.options.threaded = false;
+    long nf_hits = 0;
     tf.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const float& toi) {
-      hf[{s[0], s[1], s[2], s[3], (int)is_vf}].push_back(toi);
+      std::fprintf(out, "fseq h %d %d %d %d %d %.9g\n", s[0], s[1], s[2], s[3], (int)is_vf, (double)toi);
+      ++nf_hits;
     };
-    td.append_pair = [&](const mcl::Vector4i& s, bool is_vf, const double& toi) {
-      hd[{s[0], s[1], s[2], s[3], (int)is_vf}].push_back((double)(float)toi);
+    tf.append_discrete = [&](int p0, int p1) {
+      std::fprintf(out, "fseq d %d %d\n", p0, p1);
+      return false;
     };
     tf.update(F0.data(), F1.data(), F.data(), nf, 3);
-    td.update(D0.data(), D1.data(), F.data(), nf, 3);
     tf.traverse(F0.data(), F1.data(), F.data());
-    td.traverse(D0.data(), D1.data(), F.data());
-    for (auto& kv : hf) std::sort(kv.second.begin(), kv.second.end());
-    for (auto& kv : hd) std::sort(kv.second.begin(), kv.second.end());
-    long nf_hits = 0;
-    for (auto& kv : hf) nf_hits += (long)kv.second.size();
     // leaf boxes equal the float arithmetic of the reference's float tree
     // (BVHTree.cpp:83-124 with T = float): min/max of float vertices, padded in float
     tf.sync_leaves();
@@ -215,7 +210,7 @@ int main(int argc, char** argv) {
         box_bad += tf.leaves[i].box.max()[k] != hi + ef;
         box_bad += tf.leaves[i].box.t1.max()[k] != hi1 + ef;
       }
-    std::fprintf(out, "float %ld %d %ld %g\n", nf_hits, (int)(hf == hd), box_bad, (double)tf.options.vf_ccd_eta);
+    std::fprintf(out, "float %ld %ld %g\n", nf_hits, box_bad, (double)tf.options.vf_ccd_eta);
   }
   // 8. refit: the same frame through refit gives the rebuilt tree's hits
   {

@@ -36,7 +36,7 @@ def test_facade_callbacks(gpu, tmp_path):
     assert r.returncode == 0, r.stderr
     lines = open(tmp_path / "out.txt").read().splitlines()
     rec = {l.split()[0]: list(map(float, l.split()[1:])) for l in lines
-           if not l.startswith(("hit ", "disc ", "seqfull ", "seqstop "))}
+           if not l.startswith(("hit ", "disc ", "seqfull ", "seqstop ", "fseq "))}
     hits = [l.split()[1:] for l in lines if l.startswith("hit ")]
     disc = [tuple(map(int, l.split()[1:])) for l in lines if l.startswith("disc ")]
 
@@ -71,10 +71,27 @@ def test_facade_callbacks(gpu, tmp_path):
     npts, mism, found = rec["visit_points"]
     assert npts > 0 and mism == 0 and found >= npts  # each point lies in its own triangles' boxes
     assert rec["visit_stop"][0] == 1
-    # the float tree: the double tree's hits on the same (float) coordinates,
-    # TOIs rounded to float; leaf boxes equal to float arithmetic
-    n_float_hits, same, box_bad, eta_f = rec["float"]
-    assert n_float_hits > 0 and same == 1 and box_bad == 0 and eta_f == np.float32(eta)
+    # the float tree (threaded = false): the oracle's float tree's serial
+    # callback sequence, float TOIs bit for bit; leaf boxes equal to float arithmetic
+    n_float_hits, box_bad, eta_f = rec["float"]
+    assert n_float_hits > 0 and box_bad == 0 and eta_f == np.float32(eta)
+    of = run_scene(V0, V1, F, eta, threaded=False, candidates=False, f32=True)
+    fs, fv, ft = of.hits()
+    fd = of.discrete()
+    fexp = []
+    for _, kind, idx in zip(*of.events()):
+        if kind == 1:
+            fexp.append(("h",) + tuple(int(x) for x in fs[idx]) + (int(fv[idx]), float(np.float32(ft[idx]))))
+        elif kind == 2:
+            fexp.append(("d", int(fd[idx, 0]), int(fd[idx, 1])))
+    fgot = []
+    for l in lines:
+        if l.startswith("fseq h "):
+            x = l.split()[2:]
+            fgot.append(("h",) + tuple(int(v) for v in x[:5]) + (float(np.float32(float(x[5]))),))
+        elif l.startswith("fseq d "):
+            fgot.append(("d",) + tuple(int(v) for v in l.split()[2:]))
+    assert fgot == fexp and sum(e[0] == "h" for e in fexp) == n_float_hits
     # refit on the same frame: the rebuilt tree's hits
     n_a, n_b = rec["refit"]
     assert n_a == n_b == len(rs)
