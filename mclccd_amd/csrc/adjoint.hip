@@ -287,7 +287,7 @@ __device__ void toi_adjoint(const int* sv, const double* rec, double xsi, double
 // summed with LDS atomics; flushed with one global atomic per component and
 // distinct vertex.  A crowded table sends the contribution to global memory.
 constexpr int ADJ_TAB_BITS = 10, ADJ_TAB = 1 << ADJ_TAB_BITS, ADJ_PROBES = 16;
-constexpr int ADJ_R = 4;  // hits per thread per block of the one-step kernel
+constexpr int ADJ_R = 16;  // hits per thread per block of the one-step kernel
 
 struct AdjTable {
   int* key;     // ADJ_TAB entries, -1 = empty
@@ -298,7 +298,10 @@ template <int D>
 __device__ __forceinline__ void adj_accum(const AdjTable& tab, int v, const double* val, double* gV0, double* gV1) {
   unsigned h = ((unsigned)v * 2654435761u) >> (32 - ADJ_TAB_BITS);
   for (int probe = 0; probe < ADJ_PROBES; ++probe) {
-    const int k = atomicCAS(&tab.key[h], -1, v);
+    // a slot's key never changes once set: read it first (a broadcast when
+    // many lanes look up one vertex), claim an empty slot with a CAS
+    int k = __hip_atomic_load(&tab.key[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (k == -1) k = atomicCAS(&tab.key[h], -1, v);
     if (k == -1 || k == v) {
 #pragma unroll
       for (int c = 0; c < 2 * D; ++c) atomicAdd(&tab.acc[c * ADJ_TAB + h], val[c]);
@@ -461,13 +464,37 @@ __global__ __launch_bounds__(ADJ_T) void k_toi_adjoint_k1(const int4* __restrict
   }
 }
 
-// Sort keys of the hits: (kind, first vertex) with kind = 1 for EE, 0 else.
-__global__ void k_adj_keys(const int4* __restrict__ sten, const uint8_t* __restrict__ is_vf, long long nh, int dim,
-                           int kind_shift, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// Sort keys of the hits: (kind, cell of the first vertex) with kind = 1 for
+// EE, 0 else, at bit D * ADJ_CELL_BITS<D>; the cell is the Morton code of the
+// vertex's t0 position on a 2^ADJ_CELL_BITS grid per axis over the bounds of
+// the leaf-box centres (cbounds, ordered encoding), clamped.
+template <int D> constexpr int adj_cell_bits() { return D == 3 ? 5 : 7; }
+
+template <int D>
+__global__ void k_adj_keys(const int4* __restrict__ sten, const uint8_t* __restrict__ is_vf, long long nh,
+                           const double* __restrict__ rec, const unsigned long long* __restrict__ cbounds,
+                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  constexpr int B = adj_cell_bits<D>();
+  constexpr double cells = (double)(1 << B);
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= nh) return;
-  const uint32_t kind = (dim == 3 && !is_vf[i]) ? 1u : 0u;
-  keys[i] = (kind << kind_shift) | (uint32_t)sten[i].x;
+  const uint32_t kind = (D == 3 && !is_vf[i]) ? 1u : 0u;
+  const double* r = rec + (size_t)sten[i].x * VRec<D>::stride;
+  uint32_t q[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const double lo = ord_to_dbl(cbounds[k]), hi = ord_to_dbl(cbounds[D + k]);
+    const double ext = hi - lo;
+    double u = ext > 0 ? (r[k] - lo) / ext * cells : 0.0;
+    u = u < 0 ? 0.0 : (u > cells - 1 ? cells - 1 : u);  // NaN -> cells - 1
+    q[k] = (uint32_t)u;
+  }
+  uint32_t code = 0;
+#pragma unroll
+  for (int b = B - 1; b >= 0; --b)
+#pragma unroll
+    for (int k = 0; k < D; ++k) code = (code << 1) | ((q[k] >> b) & 1u);
+  keys[i] = (kind << (D * B)) | code;
   vals[i] = (uint32_t)i;
 }
 
@@ -509,13 +536,6 @@ static void launch_adjoint(Ctx* c, const double* gw, const uint32_t* order, cons
                                                       c->d_rec, eta, g0, g1, S.queue, S.qcap, S.qn);
 }
 
-// Bits of the vertex index in the visiting-order sort key (kind above them).
-static int adj_vertex_bits(int nv) {
-  int vbits = 1;
-  while (vbits < 31 && (1ll << vbits) < (long long)nv) ++vbits;
-  return vbits;
-}
-
 // The adjoint of the last traverse's hits into device gradients g0, g1
 // (dim*nv each, zeroed here) from device weights gw; S.keys null: no sort.
 // Synchronises the context's stream (reads the multi-step count).
@@ -526,13 +546,22 @@ static int toi_grad_run(Ctx* c, const double* gw, double* g0, double* g1, const 
   c->adj_multi = 0;
   if (nh == 0) return MCLG_OK;
   launch_fill_u64(S.qn, 1, 0ull, c->stream);
-  // Visit hits sorted by (kind, first vertex) so that each block's LDS table
-  // merges most of its contributions; the caller's hit order is untouched.
-  const int kind_shift = adj_vertex_bits(c->nv);
+  // Visit hits grouped by (kind, spatial cell of the first vertex): a 16-bit
+  // key (two radix passes), 32^3 cells in 3D.  The sort is stable, so within
+  // a cell the hits keep the hit list's order, in which the hits of one leaf
+  // pair sit together, and a block's LDS table sees the same few vertices
+  // again and again.  Measured at 1M triangles: 81 ms, against 92 ms for
+  // groups of 32 consecutive first-vertex indices and 152 ms for the full
+  // (kind, first vertex) order, whose blocks see one vertex with all of its
+  // partners.  The caller's hit order is untouched.
+  const int kind_shift = c->dim * (c->dim == 3 ? adj_cell_bits<3>() : adj_cell_bits<2>());
   const uint32_t *order = nullptr, *skey = nullptr;
   if (S.keys) {
-    k_adj_keys<<<(unsigned)((nh + 255) / 256), 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->dim,
-                                                                     kind_shift, S.keys, S.vals);
+    const unsigned gk = (unsigned)((nh + 255) / 256);
+    if (c->dim == 3)
+      k_adj_keys<3><<<gk, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->d_rec, c->d_cbounds, S.keys, S.vals);
+    else
+      k_adj_keys<2><<<gk, 256, 0, c->stream>>>(c->d_hit_sten, c->d_hit_vf, nh, c->d_rec, c->d_cbounds, S.keys, S.vals);
     launch_radix_sort(S.keys, S.vals, S.ktmp, S.vtmp, S.hist, (int)nh, kind_shift + 1, c->stream);
     order = S.vals;
     skey = S.keys;
@@ -670,7 +699,7 @@ extern "C" int mclg_toi_grad(mclg_ctx* h, const double* w, double* dV0, double* 
     S.queue = c->d_adj_q;
     S.qcap = (long long)c->cap_adj_q;
     S.qn = c->d_adj_n;
-    if (nh <= (long long)INT32_MAX && adj_vertex_bits(c->nv) <= 30) {
+    if (nh <= (long long)INT32_MAX) {  // the visiting order holds 32-bit hit indices
       const size_t b4 = sizeof(uint32_t) * (size_t)nh;
       CHECK(c->grow((void**)&c->d_adj_keys, &c->cap_adj_keys, b4, "adjoint"));
       CHECK(c->grow((void**)&c->d_adj_vals, &c->cap_adj_vals, b4, "adjoint"));
@@ -707,8 +736,6 @@ extern "C" int mclg_toi_grad_noalloc(mclg_ctx* h, const double* w, double* dV0, 
     return c->fail(MCLG_ERR_CAPACITY,
                    "toi_grad_noalloc: %lld hits exceed the static scratch's maximum of %lld; use mclg_toi_grad",
                    nh, kNoAllocHits);
-  if (adj_vertex_bits(c->nv) > 30)
-    return c->fail(MCLG_ERR_CAPACITY, "toi_grad_noalloc: %d vertices exceed the sort key's 2^30", c->nv);
   AdjScratch S;
   CHECK(noalloc_scratch(c, S));
   c->timing = true;

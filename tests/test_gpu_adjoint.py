@@ -68,6 +68,31 @@ def test_adjoint_matches_dual_numbers(gpu, case):
     assert grad_close(got, ref) <= 1.0
 
 
+def test_adjoint_first_and_later_frames(gpu):
+    # 66 % of this soup's candidates hit, more than a fresh context's first hit
+    # list holds (half the candidates): the first traverse builds its list in
+    # the compaction pass (hits in candidate order), the second appends it
+    # directly from the ACCD waves (another order).  Both gradients match the
+    # oracle's.
+    from mclccd_amd.autograd import toi_grad
+
+    V0, V1, F, _ = scenes.random_soup(300, seed=1, share=True, motion=0.2)
+    th = 1e-2
+    o = orc.run_scene(V0, V1, F, th)
+    t = gpu_run(V0, V1, F, th, strict_rank=o.rank())
+    c = t.counts()
+    assert c["hits"] > c["candidates"] // 2
+    for frame in range(2):
+        if frame:
+            t.run(V0, V1, F)
+        s, v, _ = t.hits()
+        w = np.random.default_rng(6).uniform(0, 1, len(s))
+        g0, g1 = toi_grad(t, w)
+        r0, r1 = orc.toi_grad_list(V0, V1, s, v, w, th, th)
+        ref, got = np.concatenate([r0.ravel(), r1.ravel()]), np.concatenate([g0.ravel(), g1.ravel()])
+        assert grad_close(got, ref) <= 1.0, frame
+
+
 def test_adjoint_central_differences(gpu):
     # w is non-zero on a vertex-disjoint set of hits, so each coordinate's
     # gradient comes from one hit; FD re-evaluates that hit's TOI at +-h
