@@ -43,11 +43,13 @@ template <int D> __device__ __forceinline__ Swept<D> swept(const NodeF<D>& b) {
   }
   return s;
 }
-template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const Swept<D>& qs, const NodeF<D>& b) {
+// Internal-node cull: active, and no axis separated at both t0 and t1.  The
+// swept-box overlap the leaf test also checks is implied: disjoint swept
+// boxes (unions of the t0 and t1 boxes) on an axis separate that axis at both
+// times.  A leaf with an empty t0 or t1 box holds its swept box in both FP32
+// boxes, so no node above it separates from a pair it would accept.
+template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const NodeF<D>& b) {
   bool ok = ((q.meta | b.meta) & 0x80000000u) != 0;
-  const Swept<D> bs = swept<D>(b);
-#pragma unroll
-  for (int k = 0; k < D; ++k) ok &= (qs.lo[k] <= bs.hi[k]) & (bs.lo[k] <= qs.hi[k]);
 #pragma unroll
   for (int k = 0; k < D; ++k)
     ok &= !(((q.t0_lo[k] > b.t0_hi[k]) & (q.t1_lo[k] > b.t1_hi[k])) |
@@ -128,18 +130,17 @@ template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>
 
 // leaf_test(query, leaf) in three-valued logic on the FP32 bounds; qemp: the
 // query's t0 or t1 box is empty.  A pair that cull_f32 would reject is
-// certainly false here, so leaves need no separate cull.
+// certainly false here, so leaves need no separate cull.  With both t0 and t1
+// boxes non-empty the swept overlap is implied by the separation test (see
+// cull_f32) and is only evaluated for the rare pairs with an empty box.
 template <int D>
 __device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const Swept<D>& qs, const QSteps<D>& qp, bool qa,
                                              bool qemp, const NodeF<D>& b, bool ba) {
-  const Swept<D> bs = swept<D>(b);
   const bool act = qa | ba;
   Tri ov = {act, !act};
   Tri sep = {false, true};
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < bs.hi[k], qs.lo[k] > bs.hi[k]});  // q.sw_lo <= b.sw_hi
-    ov = tri_and(ov, Tri{bs.lo[k] < qp.sw_hi_m[k], bs.lo[k] > qs.hi[k]});  // b.sw_lo <= q.sw_hi
     // q above b at t0 / t1, b above q at t0 / t1 (lo > hi: certain iff a > b, excluded iff a+ < b)
     const Tri a0 = {q.t0_lo[k] > b.t0_hi[k], qp.t0_lo_p[k] < b.t0_hi[k]};
     const Tri a1 = {q.t1_lo[k] > b.t1_hi[k], qp.t1_lo_p[k] < b.t1_hi[k]};
@@ -148,7 +149,15 @@ __device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const Swept<D>& 
     sep = tri_or(sep, tri_or(tri_and(a0, a1), tri_and(b0, b1)));
   }
   const bool emp = qemp | (b.child != 0);
-  return emp ? ov : tri_and(ov, Tri{sep.f, sep.t});
+  if (!emp) return tri_and(ov, Tri{sep.f, sep.t});
+  // an empty t0 or t1 box: the swept boxes decide (rare)
+  const Swept<D> bs = swept<D>(b);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < bs.hi[k], qs.lo[k] > bs.hi[k]});  // q.sw_lo <= b.sw_hi
+    ov = tri_and(ov, Tri{bs.lo[k] < qp.sw_hi_m[k], bs.lo[k] > qs.hi[k]});  // b.sw_lo <= q.sw_hi
+  }
+  return ov;
 }
 
 // Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
@@ -226,7 +235,7 @@ __device__ __forceinline__ Visit visit_child(const NodeF<D>& c, int n, int s, bo
       }
     }
   } else {
-    v.want = valid && c.last > s && c.first <= last && cull_f32<D>(q, qs, c);
+    v.want = valid && c.last > s && c.first <= last && cull_f32<D>(q, c);
   }
   return v;
 }
