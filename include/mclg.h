@@ -330,6 +330,14 @@ const char* mclg_batch_last_error(const mclg_batch* b);
 int mclg_batch_set_scenes(mclg_batch* b, int nscenes, const int* voff, const int* poff, const int* P,
                           const mclg_options* opts, int nopts);
 
+/* The active argument of BVHTree::update (BVHTree.cpp:27,88,114-115) for every
+ * scene at once: one int per concatenated vertex (voff[n] of them), a leaf
+ * active when any of its vertices is; a scene without a mask of its own takes
+ * all ones.  Used by every later frame until changed; NULL: no mask (every
+ * leaf active).  Host pointer, or a device pointer when device_ptrs != 0;
+ * the values are copied (a later change needs another call). */
+int mclg_batch_set_active(mclg_batch* b, const int* active, int device_ptrs);
+
 /* One frame (update + traverse of every scene).  V0, V1: voff[n] x dim,
  * concatenated; host pointers, or device pointers when device_ptrs != 0. */
 int mclg_batch_run(mclg_batch* b, const double* V0, const double* V1, int device_ptrs);

@@ -111,6 +111,7 @@ def _load():
         "mclg_batch_copy_hits": (I, [P, P, P, P, P, I64]),
         "mclg_batch_copy_discrete": (I, [P, P, P, I64]),
         "mclg_batch_set_params": (I, [P, I, I, I, I]),
+        "mclg_batch_set_active": (I, [P, P, I]),
         "mclg_batch_get_params": (I, [P, P]),
         "mclg_batch_autotune": (I, [P, C.c_char_p, I]),
         "mclg_set_rank": (I, [P, P, I]),

@@ -51,6 +51,21 @@ class Batch:
         self._check(lib.mclg_batch_set_scenes(self._h, n, _ptr(voff), _ptr(poff), _ptr(P), arr, len(opts)))
         self.nscenes, self.voff, self.poff = n, voff, poff
 
+    def set_active(self, active):
+        """BVHTree::update's active mask for every scene: one int per
+        concatenated vertex (scenes without a mask: all ones), or None for no
+        mask.  Kept for later frames (mclg_batch_set_active)."""
+        if active is None:
+            self._check(lib.mclg_batch_set_active(self._h, None, 0))
+            return
+        dev = _is_device(active)
+        if not dev:
+            active = np.ascontiguousarray(active, np.int32)
+        if len(active) != int(self.voff[-1]):
+            raise ValueError(f"active needs {int(self.voff[-1])} entries, got {len(active)}")
+        self._check(lib.mclg_batch_set_active(self._h, _ptr(active), int(dev)))
+        self._active = active  # a device mask must outlive the copy
+
     def run(self, V0, V1):
         """One frame on concatenated vertices (numpy, or CUDA float64 tensors)."""
         dev = _is_device(V0)

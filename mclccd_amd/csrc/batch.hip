@@ -50,6 +50,10 @@ struct BatchState {
   double* d_V0 = nullptr; size_t cap_V0 = 0;
   double* d_V1 = nullptr; size_t cap_V1 = 0;
   unsigned long long* d_scene_counts = nullptr; size_t cap_scene_counts = 0;
+  // per-vertex active mask of the concatenated vertices (BVHTree::update's
+  // active, every scene at once); has_active false: every leaf active
+  int* d_active = nullptr; size_t cap_active = 0;
+  bool has_active = false;
   // graph
   bool use_graph = true;
   hipGraph_t graph = nullptr;
@@ -149,7 +153,8 @@ static void batch_enqueue(Ctx* c) {
   const mclg_options& o = b->opts[0];
   launch_pack(dim, b->d_V0, b->d_V1, b->nv, c->d_rec, s, c->d_v1);
   launch_fill_u64(c->d_counters, C_COUNT, 0ull, s);
-  launch_leaf_boxes(dim, c->d_rec, b->d_P, np, dim, nullptr, 0.0, c->d_boxes, c->d_act, c->d_partial, c->d_cbounds, S, s);
+  launch_leaf_boxes(dim, c->d_rec, b->d_P, np, dim, b->has_active ? b->d_active : nullptr, 0.0, c->d_boxes, c->d_act,
+                    c->d_partial, c->d_cbounds, S, s);
   launch_morton(dim, c->d_boxes, np, c->d_cbounds, S, c->d_keys, c->d_order, b->d_mort, s);
   launch_radix_sort(c->d_keys, c->d_order, c->d_ktmp, c->d_vtmp, c->d_hist, np, 32, s);
   launch_scene_keys(c->d_order, np, b->d_prim_scene, b->d_khi, s);
@@ -407,6 +412,23 @@ int mclg_batch_get_params(const mclg_batch* h, int* out4) {
   out4[1] = c->tr_waves;
   out4[2] = c->blocks_per_cu;
   out4[3] = c->chunk;
+  return MCLG_OK;
+}
+
+int mclg_batch_set_active(mclg_batch* h, const int* active, int device_ptrs) {
+  Ctx* c = B(h);
+  if (!c || !c->batch) return MCLG_ERR_ARG;
+  BatchState* b = c->batch;
+  if (b->nscenes == 0) return c->fail(MCLG_ERR_STATE, "batch: no scenes set");
+  const bool on = active != nullptr;
+  if (on) {
+    CHECK(c->grow((void**)&b->d_active, &b->cap_active, sizeof(int) * std::max<size_t>((size_t)b->nv, 1), "active"));
+    if (b->nv > 0)
+      HIPCHECK(c, hipMemcpyAsync(b->d_active, active, sizeof(int) * (size_t)b->nv,
+                                 device_ptrs ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream));
+  }
+  if (on != b->has_active) drop_graph(b);  // the graph holds the mask's pointer (or none), not its contents
+  b->has_active = on;
   return MCLG_OK;
 }
 

@@ -21,12 +21,20 @@ def small_batch(count=24, seed=5):
     return out
 
 
-def check_batch_vs_oracle(b, sc_list):
+def check_batch_vs_oracle(b, sc_list, masks=None):
     sc_h, s_h, v_h, t_h = b.hits()
     sc_d, p_d = b.discrete()
     per_h, per_d = b.scene_counts()
     for k, (V0, V1, F, th) in enumerate(sc_list):
-        o = run_scene(V0, V1, F, th)
+        if masks is not None and masks[k] is not None:
+            from oracle.oracle import Oracle
+
+            o = Oracle(V0.shape[1])
+            o.set_options(box_eta=th, vf_ccd_eta=th, ee_ccd_eta=th)
+            o.update(V0, V1, F, active=masks[k])
+            o.traverse()
+        else:
+            o = run_scene(V0, V1, F, th)
         rs, rv, rt = o.hits()
         m = sc_h == k
         gm, rm = hit_map(s_h[m], v_h[m], t_h[m]), hit_map(rs, rv, rt)
@@ -53,6 +61,30 @@ def test_batch_matches_oracle_per_scene(gpu, graph):
     c1 = b.counts()
     b.run(V0, V1)
     assert b.counts() == c1
+    check_batch_vs_oracle(b, sc_list)
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_batch_active_masks(gpu, graph):
+    # BVHTree::update's active mask per scene (BVHTree.cpp:27,88,114-115):
+    # every third scene unmasked (all ones), the others with random vertex
+    # masks; each scene equals the oracle updated with its own mask.  The mask
+    # then changes between graph replays, and is finally removed.
+    from mclccd_amd.batch import from_scenes
+
+    sc_list = small_batch(count=12, seed=8)
+    b, V0, V1 = from_scenes(sc_list)
+    b.set_params(use_graph=graph)
+    rng = np.random.default_rng(3)
+    for frame in range(2):
+        masks = [None if k % 3 == 0 else (rng.uniform(size=len(s[0])) < 0.3).astype(np.int32)
+                 for k, s in enumerate(sc_list)]
+        full = np.concatenate([np.ones(len(s[0]), np.int32) if m is None else m for m, s in zip(masks, sc_list)])
+        b.set_active(full)
+        b.run(V0, V1)
+        check_batch_vs_oracle(b, sc_list, masks)
+    b.set_active(None)
+    b.run(V0, V1)
     check_batch_vs_oracle(b, sc_list)
 
 
