@@ -240,6 +240,14 @@ __device__ __forceinline__ Visit visit_child(const NodeF<D>& c, int n, int s, bo
   return v;
 }
 
+// Internal children covering at most TR_BUCKET sorted leaves are not walked:
+// their leaves (contiguous in the node array) are tested in the same step,
+// one scalar-loaded leaf record each.  A step's fixed work (the child-pair
+// load, ballots, stack) costs more than a few leaf tests: at 1M triangles the
+// traversal took 11.7 ms without buckets, 10.8 / 10.2 / 10.1 / 10.1 ms with
+// buckets of 4 / 8 / 16 / 32 leaves (12.1 ms with 2).
+constexpr int TR_BUCKET = 16;
+
 // Stages the lanes' pairs (s, k) of one leaf k; flushes a full buffer.
 __device__ __forceinline__ void stage_pairs(bool emit, int s, int k, int2* buf, int& cnt, int2* __restrict__ pairs,
                                             unsigned long long cap, unsigned long long* counter) {
@@ -302,7 +310,26 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
     stage_pairs(va.emit, s, va.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
     stage_pairs(vb.emit, s, vb.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
     const int ia = va.idx, ib = vb.idx;
-    const bool ga = __ballot(va.want) != 0, gb = __ballot(vb.want) != 0;
+    bool ga = __ballot(va.want) != 0, gb = __ballot(vb.want) != 0;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const NodeF<D>& c = pr.c[j];
+      const bool g = j == 0 ? ga : gb;
+      if (!g || c.last - c.first >= TR_BUCKET) continue;
+      // a small subtree: test its leaves here
+      for (int k = c.first; k <= c.last; ++k) {
+        const NodeF<D> L = nodes[(n - 1) + k];
+        bool emit = false;
+        if (valid && k > s && k <= last) {
+          const bool ba = (L.meta >> 31) != 0;
+          const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, L, ba);
+          emit = (r.t | r.f) ? r.t : leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
+        }
+        stage_pairs(emit, s, k, s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
+      }
+      if (j == 0) ga = false;
+      else gb = false;
+    }
     if (ga) {
       if (gb) {  // enter the first child, the second waits on the stack
         if (sp == TR_STACK) {
