@@ -129,7 +129,7 @@ def roofline(c, avg, n, nv, dim):
         head_b, vert_b = (24, 32) if dim == 3 else (16, 16)
         byts = 16 * c["leaf_pairs"] + head_b * n + vert_b * nv + 16 * c["candidates"] + 8 * c["discrete"]
         achieved = byts / (avg["expand"] * 1e-3) / 1e9
-        kernel = f"k_expand<{dim}>"
+        kernel = f"k_expand<{dim}, false>"
         return {"kernel": kernel + " (+ count pass and scans)", "bound": "hbm", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": profiled_traffic(kernel) if dim == 3 else None, "algorithmic_bytes": byts,
@@ -144,7 +144,7 @@ def roofline(c, avg, n, nv, dim):
     per_eval = FLOPS_PER_EVAL["vf2" if dim == 2 else ("vf3" if vf else "ee3")]
     ms = avg[dom]
     achieved = flops / (ms * 1e-3) / 1e12
-    kernel = f"k_accd<{dim}, true>" if vf else f"k_accd<{dim}, false>"
+    kernel = f"k_accd<{dim}, true, double>" if vf else f"k_accd<{dim}, false, double>"
     return {"kernel": f"{kernel} (FP64 ACCD, {'VF' if vf else 'EE'} stencils)", "bound": "valu",
             "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": profiled_traffic(kernel) if dim == 3 else None,
@@ -491,6 +491,7 @@ def main():
                               "no copy to the host and no callback replay inside the timed region",
                    "leaf_pairs": c["leaf_pairs"], "candidates": c["candidates"], "hits": c["hits"],
                    "discrete": c["discrete"], "dist_evals": c["dist_evals"],
+                   "traverse_wave_steps": c["traverse_steps"],
                    "phase_ms": {k: round(v, 4) for k, v in avg.items()}},
         "roofline": roof,
         "kernels": others,

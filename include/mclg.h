@@ -124,6 +124,7 @@ typedef struct mclg_counts {
   int64_t vf_candidates;  /* of candidates: vertex-face stencils (the rest are edge-edge) */
   int64_t ee_dist_evals;  /* of dist_evals: those of edge-edge stencils */
   int64_t order_dependent; /* leaf pairs whose discrete result depends on the argument order (MCLG_CHECK_ORDER) */
+  int64_t traverse_steps;  /* traversal wave-steps: child-pair records read, summed over the 64-leaf packets */
 } mclg_counts;
 int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
 

@@ -258,6 +258,7 @@ static int batch_frame(Ctx* c) {
   c->counts.dist_evals = (int64_t)h[C_DIST_EVALS];
   c->counts.vf_candidates = (int64_t)h[C_CAND_VF];
   c->counts.ee_dist_evals = (int64_t)h[C_EE_EVALS];
+  c->counts.traverse_steps = (int64_t)h[C_TR_STEPS];
   c->keep_cand = false;
   return MCLG_OK;
 }

@@ -296,7 +296,7 @@ int Ctx::read_counters() {
 
 // BVHTree.cpp:129-159 (+ collide / get_candidates / narrow phase / discrete)
 int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned flags) {
-  counts = mclg_counts{0, 0, 0, 0, 0, 0, 0};
+  counts = mclg_counts{};
   hits_gen = kNoGen;
   hits_one_sided = opt.vf_one_sided != 0;
   hits_f32 = precision == MCLG_PRECISION_F32;
@@ -336,6 +336,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
   }
   const long long npairs = (long long)h_counters[C_LEAF_PAIRS];
   counts.leaf_pairs = npairs;
+  counts.traverse_steps = (int64_t)h_counters[C_TR_STEPS];
   n_vf = n_ee = 0;
   if (npairs > 0 && (opt.continuous || opt.discrete)) {
     // candidates + discrete (BVHTree.cpp:252-283)

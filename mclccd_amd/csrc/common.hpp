@@ -91,7 +91,8 @@ enum Counter : int {
   C_ERRORS,     // kernels that detected an inconsistent state
   C_EE_EVALS,   // distance evaluations of edge-edge stencils (also counted in C_DIST_EVALS)
   C_ORDER_DEP,  // leaf pairs whose discrete result depends on the argument order (when checked)
-  C_COUNT = 9
+  C_TR_STEPS,   // traversal wave-steps (child-pair records visited, summed over waves)
+  C_COUNT = 10
 };
 
 // ---- wave helpers ---------------------------------------------------------

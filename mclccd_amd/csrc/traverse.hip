@@ -298,7 +298,9 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
   // each internal node is entered at most once: more steps means a malformed tree
   int budget = n;
   int cnt = 0;  // staged pairs (wave-uniform)
+  unsigned steps = 0;
   for (;;) {
+    ++steps;
     if (--budget < 0) {
       if (lane == 0) atomicAdd(counters + C_ERRORS, 1ull);
       break;
@@ -350,6 +352,7 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
     }
   }
   if (cnt > 0) flush_pairs(s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
+  if (lane == 0) atomicAdd(counters + C_TR_STEPS, (unsigned long long)steps);
 }
 
 void launch_traverse(int dim, const void* nodes, const void* cp, const int* root, const void* sboxes, int n,
