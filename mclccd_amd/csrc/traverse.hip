@@ -4,10 +4,9 @@
 //   wave walking the LBVH together (child-pair records, wave-uniform stack);
 //   lane s pairs only with leaves after it, so every unordered leaf pair is
 //   found once.  Internal nodes are culled with the FP32 outward-rounded boxes;
-//   at a leaf the reference's exact test (BVHTree.cpp:401-426) is decided on the
-//   FP32 bounds when they settle it and on the FP64 boxes (kept in sorted
-//   order) otherwise.  Pairs (sorted positions) are staged per wave in LDS and
-//   appended in batches.
+//   leaves are decided by the reference's exact test (BVHTree.cpp:401-426) on
+//   the FP64 boxes (kept in sorted order).  Pairs (sorted positions) are staged
+//   per wave in LDS and appended in batches.
 // k_face_records: per sorted leaf, the compact head (P, reptri flags), rank
 //   and primitive index.
 // k_expand: one lane per leaf pair: orientation, the discrete test and the
@@ -27,27 +26,12 @@ namespace mclg {
 // ---------------------------------------------------------------------------
 // Leaf-pair tests
 // ---------------------------------------------------------------------------
-// The tests below are branch-free: every field is used unconditionally, so a
-// node's or box's loads are all issued at once instead of one dependent load
-// per early exit.
-// A node's FP32 swept box: the union of its t0 and t1 boxes.
-template <int D> struct Swept {
-  float lo[D], hi[D];
-};
-template <int D> __device__ __forceinline__ Swept<D> swept(const NodeF<D>& b) {
-  Swept<D> s;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    s.lo[k] = fminf(b.t0_lo[k], b.t1_lo[k]);
-    s.hi[k] = fmaxf(b.t0_hi[k], b.t1_hi[k]);
-  }
-  return s;
-}
-// Internal-node cull: active, and no axis separated at both t0 and t1.  The
-// swept-box overlap the leaf test also checks is implied: disjoint swept
-// boxes (unions of the t0 and t1 boxes) on an axis separate that axis at both
-// times.  A leaf with an empty t0 or t1 box holds its swept box in both FP32
-// boxes, so no node above it separates from a pair it would accept.
+// Node cull on the FP32 outward-rounded boxes: active, and no axis
+// separated at both t0 and t1.  It never rejects a pair the exact test accepts:
+// FP32 boxes that separate contain FP64 boxes that separate, and a leaf with an
+// empty t0 or t1 box holds its swept box in both FP32 boxes (the exact test then
+// asks only for overlapping swept boxes, the unions of the t0 and t1 boxes).
+// Branch-free, so the node's loads are all issued at once.
 template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, const NodeF<D>& b) {
   bool ok = ((q.meta | b.meta) & 0x80000000u) != 0;
 #pragma unroll
@@ -57,33 +41,11 @@ template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, con
   return ok;
 }
 
-template <int D> __device__ __forceinline__ bool empty_box(const double* lo, const double* hi) {
-  bool e = false;
-#pragma unroll
-  for (int k = 0; k < D; ++k) e |= lo[k] > hi[k];
-  return e;
-}
-
-// BVHTree.cpp:401-426 on two leaves (symmetric in l and r): false if both
-// inactive or the swept boxes are disjoint; true if a t0 or t1 box is empty;
-// else false iff some axis separates both at t0 and at t1.
-template <int D>
-__device__ __forceinline__ bool leaf_test(const LeafBox<D>& l, bool la, const LeafBox<D>& r, bool ra) {
-  bool ov = la | ra;
-#pragma unroll
-  for (int k = 0; k < D; ++k) ov &= (l.sw_lo[k] <= r.sw_hi[k]) & (r.sw_lo[k] <= l.sw_hi[k]);
-  const bool emp = empty_box<D>(l.t0_lo, l.t0_hi) | empty_box<D>(r.t0_lo, r.t0_hi) |
-                   empty_box<D>(l.t1_lo, l.t1_hi) | empty_box<D>(r.t1_lo, r.t1_hi);
-  bool sep = false;
-#pragma unroll
-  for (int k = 0; k < D; ++k)
-    sep |= ((l.t0_lo[k] > r.t0_hi[k]) & (l.t1_lo[k] > r.t1_hi[k])) | ((l.t0_hi[k] < r.t0_lo[k]) & (l.t1_hi[k] < r.t1_lo[k]));
-  return ov & (emp | !sep);
-}
-
-// The same test one axis at a time, reading the FP64 boxes in place: the rare
-// fallback of the FP32 test below, kept small in registers so that it does not
-// set the register budget of the traversal loop.
+// BVHTree.cpp:401-426 on two leaves (symmetric in l and r), reading the FP64
+// boxes in place one axis at a time: false if both inactive or the swept boxes
+// are disjoint; true if a t0 or t1 box is empty; else false iff some axis
+// separates both at t0 and at t1.  The traversal's fallback for pairs with an
+// empty box, kept out of line and small in registers.
 template <int D>
 __device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bool la, const LeafBox<D>* r, bool ra) {
   bool ov = la | ra, emp = false, sep = false;
@@ -99,65 +61,23 @@ __device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bo
   return ov & (emp | !sep);
 }
 
-// The same test decided from the FP32 node boxes when it can be.  A node's
-// lower bound a = f32_down(lo) brackets the FP64 value as lo in [a, a+) and an
-// upper bound b = f32_up(hi) as hi in (b-, b] (a+ and b- one float step up and
-// down), so lo <= hi is certainly true iff a+ < b (equivalently a < b-) and
-// certainly false iff a > b.  Only the query's bounds are ever stepped, once
-// per lane (QSteps).  Emptiness of the t0 / t1 boxes comes exactly from the
-// FP64 boxes (the leaf's child flag).  The FP64 boxes are read only when the
-// combination is undecided.
-struct Tri {
-  bool t, f;  // certainly true / certainly false
+// The t0 and t1 FP64 boxes of a sorted leaf (the tail of its LeafBox).  With
+// both boxes non-empty the exact test is (la | ra) & !sep: disjoint swept boxes
+// on an axis separate that axis at t0 and at t1, so the swept overlap is implied.
+// FP64 boxes hold no NaN (box extension skips NaN coordinates; an axis of NaNs
+// leaves the box empty, which takes the fallback).
+template <int D> struct Box01 {
+  double t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D];
 };
-__device__ __forceinline__ Tri tri_and(Tri x, Tri y) { return {bool(x.t & y.t), bool(x.f | y.f)}; }
-__device__ __forceinline__ Tri tri_or(Tri x, Tri y) { return {bool(x.t | y.t), bool(x.f & y.f)}; }
-
-// a+ of the query's lower bounds and b- of its upper bounds
-template <int D> struct QSteps {
-  float sw_lo_p[D], sw_hi_m[D], t0_lo_p[D], t0_hi_m[D], t1_lo_p[D], t1_hi_m[D];
-};
-template <int D> __device__ __forceinline__ QSteps<D> query_steps(const NodeF<D>& q, const Swept<D>& qs) {
-  QSteps<D> p;
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    p.sw_lo_p[k] = step_up(qs.lo[k]); p.sw_hi_m[k] = step_down(qs.hi[k]);
-    p.t0_lo_p[k] = step_up(q.t0_lo[k]); p.t0_hi_m[k] = step_down(q.t0_hi[k]);
-    p.t1_lo_p[k] = step_up(q.t1_lo[k]); p.t1_hi_m[k] = step_down(q.t1_hi[k]);
-  }
-  return p;
+template <int D> __device__ __forceinline__ const Box01<D>* box01(const LeafBox<D>* b) {
+  return reinterpret_cast<const Box01<D>*>(b->t0_lo);
 }
-
-// leaf_test(query, leaf) in three-valued logic on the FP32 bounds; qemp: the
-// query's t0 or t1 box is empty.  A pair that cull_f32 would reject is
-// certainly false here, so leaves need no separate cull.  With both t0 and t1
-// boxes non-empty the swept overlap is implied by the separation test (see
-// cull_f32) and is only evaluated for the rare pairs with an empty box.
-template <int D>
-__device__ __forceinline__ Tri leaf_test_f32(const NodeF<D>& q, const Swept<D>& qs, const QSteps<D>& qp, bool qa,
-                                             bool qemp, const NodeF<D>& b, bool ba) {
-  const bool act = qa | ba;
-  Tri ov = {act, !act};
-  Tri sep = {false, true};
+template <int D> __device__ __forceinline__ bool separated(const Box01<D>& q, const Box01<D>& b) {
+  bool sep = false;
 #pragma unroll
-  for (int k = 0; k < D; ++k) {
-    // q above b at t0 / t1, b above q at t0 / t1 (lo > hi: certain iff a > b, excluded iff a+ < b)
-    const Tri a0 = {q.t0_lo[k] > b.t0_hi[k], qp.t0_lo_p[k] < b.t0_hi[k]};
-    const Tri a1 = {q.t1_lo[k] > b.t1_hi[k], qp.t1_lo_p[k] < b.t1_hi[k]};
-    const Tri b0 = {b.t0_lo[k] > q.t0_hi[k], b.t0_lo[k] < qp.t0_hi_m[k]};
-    const Tri b1 = {b.t1_lo[k] > q.t1_hi[k], b.t1_lo[k] < qp.t1_hi_m[k]};
-    sep = tri_or(sep, tri_or(tri_and(a0, a1), tri_and(b0, b1)));
-  }
-  const bool emp = qemp | (b.child != 0);
-  if (!emp) return tri_and(ov, Tri{sep.f, sep.t});
-  // an empty t0 or t1 box: the swept boxes decide (rare)
-  const Swept<D> bs = swept<D>(b);
-#pragma unroll
-  for (int k = 0; k < D; ++k) {
-    ov = tri_and(ov, Tri{qp.sw_lo_p[k] < bs.hi[k], qs.lo[k] > bs.hi[k]});  // q.sw_lo <= b.sw_hi
-    ov = tri_and(ov, Tri{bs.lo[k] < qp.sw_hi_m[k], bs.lo[k] > qs.hi[k]});  // b.sw_lo <= q.sw_hi
-  }
-  return ov;
+  for (int a = 0; a < D; ++a)
+    sep |= ((q.t0_lo[a] > b.t0_hi[a]) & (q.t1_lo[a] > b.t1_hi[a])) | ((q.t0_hi[a] < b.t0_lo[a]) & (q.t1_hi[a] < b.t1_lo[a]));
+  return sep;
 }
 
 // Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
@@ -205,47 +125,19 @@ __device__ __forceinline__ void flush_pairs(const int2* buf, int cnt, int2* __re
 // (queries) and walk the tree together, depth first from the root with a
 // wave-uniform stack.  Each step reads the child-pair record of one internal
 // node (both children, one 128 B line, at a wave-uniform address: scalar
-// loads) and tests each lane's query against both children; a child is
+// loads) and culls each lane's query against both children; a child is
 // entered if any lane wants it.  Lane j takes part only where a child covers
 // a sorted leaf in (s_j, last_j] (last_j: the last leaf of its scene, n-1 for
 // one scene), so each unordered leaf pair is found once, by its lower leaf.
 constexpr int TR_STACK = 96;  // deeper than any LBVH over 32-bit keys and int indices
 
-// One child of the current node: a leaf is tested by every lane that wants it
-// (emit), an internal node is wanted if any lane's query may meet it.
-struct Visit {
-  int idx;
-  bool emit, want;
-};
-template <int D>
-__device__ __forceinline__ Visit visit_child(const NodeF<D>& c, int n, int s, bool valid, int last, int wave_last,
-                                             const NodeF<D>& q, const Swept<D>& qs, const QSteps<D>& qp, bool qa,
-                                             bool qemp, const LeafBox<D>* __restrict__ sboxes) {
-  Visit v = {(int)(c.meta & 0x7fffffffu) - 1, false, false};
-  if (c.first > wave_last) return v;  // past every lane's scene
-  if (v.idx >= n - 1) {
-    const int k = v.idx - (n - 1);
-    if (valid && k > s && k <= last) {
-      const bool ba = (c.meta >> 31) != 0;
-      const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, c, ba);
-      if (r.t | r.f) {
-        v.emit = r.t;
-      } else {  // undecided on the FP32 bounds: the exact test on the FP64 boxes
-        v.emit = leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
-      }
-    }
-  } else {
-    v.want = valid && c.last > s && c.first <= last && cull_f32<D>(q, c);
-  }
-  return v;
-}
-
-// Internal children covering at most TR_BUCKET sorted leaves are not walked:
-// their leaves (contiguous in the node array) are tested in the same step,
-// one scalar-loaded leaf record each.  A step's fixed work (the child-pair
-// load, ballots, stack) costs more than a few leaf tests: at 1M triangles the
-// traversal took 11.7 ms without buckets, 10.8 / 10.2 / 10.1 / 10.1 ms with
-// buckets of 4 / 8 / 16 / 32 leaves (12.1 ms with 2).
+// Leaf children and internal children covering at most TR_BUCKET sorted leaves
+// are not walked: their leaves (contiguous in sorted order) are tested in the
+// same step, one scalar-loaded FP64 box each.  A step's fixed work (the
+// child-pair load, ballots, stack) costs more than a few leaf tests: at 1M
+// triangles the traversal (then testing leaves on FP32 bounds) took 11.7 ms
+// without buckets, 10.8 / 10.2 / 10.1 / 10.1 ms with buckets of 4 / 8 / 16 /
+// 32 leaves (12.1 ms with 2).
 constexpr int TR_BUCKET = 16;
 
 // Stages the lanes' pairs (s, k) of one leaf k; flushes a full buffer.
@@ -274,16 +166,14 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
   const bool valid = s < n - 1;
   if (__ballot(valid) == 0) return;  // wave-uniform: the wave starts at or after the last leaf
   NodeF<D> q;
-  Swept<D> qs;
-  QSteps<D> qp;
+  Box01<D> qd;
   bool qa = false, qemp = false;
   int last = -1;
   if (valid) {
     q = load16(nodes + (n - 1) + s);
+    qd = load16(box01<D>(sboxes + s));
     qa = (q.meta >> 31) != 0;
     qemp = q.child != 0;
-    qs = swept<D>(q);
-    qp = query_steps<D>(q, qs);
     last = scene_off ? scene_off[scene_of_sorted[s] + 1] - 1 : n - 1;
   }
   int wave_last = last;
@@ -306,34 +196,38 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
       break;
     }
     const ChildPair<D> pr = cp[p];
-    const Visit va = visit_child<D>(pr.c[0], n, s, valid, last, wave_last, q, qs, qp, qa, qemp, sboxes);
-    const Visit vb = visit_child<D>(pr.c[1], n, s, valid, last, wave_last, q, qs, qp, qa, qemp, sboxes);
-    // pairs of leaf children (leaf k = index - (n-1))
-    stage_pairs(va.emit, s, va.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
-    stage_pairs(vb.emit, s, vb.idx - (n - 1), s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
-    const int ia = va.idx, ib = vb.idx;
-    bool ga = __ballot(va.want) != 0, gb = __ballot(vb.want) != 0;
+    bool g[2];
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const NodeF<D>& c = pr.c[j];
-      const bool g = j == 0 ? ga : gb;
-      if (!g || c.last - c.first >= TR_BUCKET) continue;
-      // a small subtree: test its leaves here
+      const bool want = valid && c.first <= wave_last && c.last > s && c.first <= last && cull_f32<D>(q, c);
+      g[j] = __ballot(want) != 0;
+    }
+    // leaves and small subtrees: their sorted leaves are tested here, on the
+    // FP64 t0 / t1 boxes (exact; the swept overlap is implied unless a box is empty)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const NodeF<D>& c = pr.c[j];
+      if (!g[j] || c.last - c.first >= TR_BUCKET) continue;
       for (int k = c.first; k <= c.last; ++k) {
-        const NodeF<D> L = nodes[(n - 1) + k];
-        bool emit = false;
-        if (valid && k > s && k <= last) {
-          const bool ba = (L.meta >> 31) != 0;
-          const Tri r = leaf_test_f32<D>(q, qs, qp, qa, qemp, L, ba);
-          emit = (r.t | r.f) ? r.t : leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
+        const Box01<D> B = *box01<D>(sboxes + k);
+        const NodeF<D>& Ln = nodes[(n - 1) + k];
+        const uint32_t bm = Ln.meta;
+        const int bemp = Ln.child;
+        const bool in = valid & (k > s) & (k <= last);
+        const bool ba = (bm >> 31) != 0;
+        bool emit = in & (qa | ba) & !separated<D>(qd, B);
+        const bool slow = in & (qemp | (bemp != 0));
+        if (__ballot(slow) != 0) {
+          if (slow) emit = leaf_test_fp64<D>(sboxes + s, qa, sboxes + k, ba);
         }
         stage_pairs(emit, s, k, s_buf[w], cnt, pairs, cap, counters + C_LEAF_PAIRS);
       }
-      if (j == 0) ga = false;
-      else gb = false;
+      g[j] = false;
     }
-    if (ga) {
-      if (gb) {  // enter the first child, the second waits on the stack
+    const int ia = (int)(pr.c[0].meta & 0x7fffffffu) - 1, ib = (int)(pr.c[1].meta & 0x7fffffffu) - 1;
+    if (g[0]) {
+      if (g[1]) {  // enter the first child, the second waits on the stack
         if (sp == TR_STACK) {
           if (lane == 0) atomicAdd(counters + C_ERRORS, 1ull);
           break;
@@ -342,7 +236,7 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
         ++sp;
       }
       p = ia;
-    } else if (gb) {
+    } else if (g[1]) {
       p = ib;
     } else {
       if (sp == 0) break;
