@@ -191,14 +191,15 @@ static void batch_enqueue(Ctx* c) {
   // the ACCD kernels append the hits (counters[C_HITS] was cleared with the
   // others); an overflow reruns the frame with a larger list
   const bool direct = accd_direct_hits(c->vf_cap) && accd_direct_hits(c->ee_cap);
-  if (direct) {
+  if (direct) {  // per-candidate TOIs and per-chunk counts only serve the compaction pass
     L.hit_sten = c->d_hit_sten; L.hit_vf = c->d_hit_vf; L.hit_toi = c->d_hit_toi; L.hit_cap = c->hit_cap;
+    L.cand_toi = nullptr; L.chunk_hits = nullptr;
   }
   launch_accd(dim, true, L, s);
   L.one_sided = 0;
   if (dim == 3) {
     L.q = c->d_eeq; L.n_dev = c->d_counters + C_CAND_EE; L.cap = c->ee_cap; L.scene_eta = S.ee_eta;
-    L.cand_toi = c->d_ee_toi; L.chunk_hits = c->d_hit_off + ncv;
+    if (!direct) { L.cand_toi = c->d_ee_toi; L.chunk_hits = c->d_hit_off + ncv; }
     launch_accd(dim, false, L, s);
   }
   if (direct) return;

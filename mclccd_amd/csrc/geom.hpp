@@ -235,39 +235,103 @@ template <typename T> struct Stencil<2, true, T> {
   }
 };
 
+// The iteration is split so that lock-step waves need not wait for their
+// longest stencil: start<STOP>() runs the setup, d0 and iterations [0, STOP);
+// a stencil still running at the top of iteration STOP = 3 can be suspended
+// with (t1, t2, t3, g) — the steps of iterations 0..2 after the skipped
+// iteration 0, and the break threshold — and resumed later: resume() rebuilds
+// p and l_p (the same arithmetic), replays the two position updates
+// x0 + t1*p, then + t2*p (the same roundings), and toi = ((0 + 0) + t1) + t2,
+// so run() continues with exactly the state the uninterrupted loop has.  Three
+// evaluations are the common case (d0, d1, and d2 where the break test fires).
+enum : int { ACCD_MISS = 0, ACCD_HIT = 1, ACCD_RUN = 2, ACCD_SUSPENDED = 3 };
+constexpr int kAccdSuspendIter = 3;
+
+template <int D, bool VF, typename T> struct Accd {
+  Stencil<D, VF, T> st;
+  T xsi, l_p, g, toi, tl, t1, t2;
+  int iter, evals;
+  bool skip, fin;
+  bool replayable;  // iteration 0 was the skipped no-op step: (t1, t2) rebuild x
+
+  // ACCD_HIT (toi set), ACCD_MISS, or ACCD_RUN at the top of iteration STOP.
+  template <int STOP, typename V> __device__ __forceinline__ int start(const V* v0, const V* v1, T xsi_) {
+    xsi = xsi_;
+    l_p = st.setup(v0, v1, fin);
+    evals = 0;
+    replayable = false;
+    if (l_p <= 0) return ACCD_MISS;
+    T d = st.dist();
+    evals = 1;
+    toi = T(0);
+    if (d <= xsi) return ACCD_HIT;
+    g = cr_div(T(1) * (d * d - xsi * xsi), d - xsi);
+    tl = cr_div((T(1) - T(1)) * (d * d - xsi * xsi), (d + xsi) * l_p);
+    skip = tl == T(0) && fin;  // iteration 0 leaves x unchanged: its distance is d0
+    replayable = skip;
+    iter = 0;
+    return loop<STOP>(d);
+  }
+
+  template <typename V>
+  __device__ __forceinline__ void resume(const V* v0, const V* v1, T xsi_, T t1_, T t2_, T t3_, T g_) {
+    xsi = xsi_;
+    l_p = st.setup(v0, v1, fin);
+    st.step(t1_);
+    st.step(t2_);
+    toi = T(0);
+    toi += T(0);
+    toi += t1_;
+    toi += t2_;
+    tl = t3_;
+    g = g_;
+    iter = kAccdSuspendIter;
+    skip = false;
+    evals = 0;
+  }
+
+  // iterations [iter, 1000): ACCD_HIT or ACCD_MISS
+  __device__ __forceinline__ int run() {
+    T d = T(0);  // assigned before use: skip is false from iteration 1 on
+    return loop<1000>(d);
+  }
+
+  // One distance evaluation site per iteration (reference loop NarrowPhase.cpp:684-707).
+  template <int STOP> __device__ __forceinline__ int loop(T& d) {
+    for (; iter < STOP; ++iter) {
+      const bool can_break = toi > T(0) && iter > 0;
+      if (!can_break && toi + tl > T(1)) return ACCD_MISS;  // the reference's distance here is unused
+      if (!skip) {
+        st.step(tl);
+        d = st.dist();
+        ++evals;
+      }
+      skip = false;
+      if (can_break) {
+        const T eps = cr_div(d * d - xsi * xsi, d + xsi);
+        if (eps < g) return ACCD_HIT;
+      }
+      toi += tl;
+      if (toi > T(1)) return ACCD_MISS;
+      tl = (T)(0.9 * (double)(d * d - xsi * xsi) / (double)((d + xsi) * l_p));
+      if (STOP <= kAccdSuspendIter) {
+        if (iter == 0) t1 = tl;
+        else if (iter == 1) t2 = tl;
+      }
+    }
+    // max_iter reached: the reference returns true (its assert is compiled out)
+    return STOP >= 1000 ? ACCD_HIT : ACCD_RUN;
+  }
+};
+
 template <int D, bool VF, typename T, typename V>
 __device__ __forceinline__ int accd(const V* v0, const V* v1, T xsi, T& toi, int& evals) {
-  Stencil<D, VF, T> st;
-  bool fin;
-  const T l_p = st.setup(v0, v1, fin);
-  evals = 0;
-  if (l_p <= 0) return 0;
-  T d = st.dist();
-  evals = 1;
-  toi = T(0);
-  if (d <= xsi) return 1;
-  const T g = cr_div(T(1) * (d * d - xsi * xsi), d - xsi);
-  T tl = cr_div((T(1) - T(1)) * (d * d - xsi * xsi), (d + xsi) * l_p);
-  bool skip = tl == T(0) && fin;  // iteration 0 leaves x unchanged: its distance is d0
-  // One distance evaluation site per iteration (reference loop NarrowPhase.cpp:684-707).
-  for (int iter = 0; iter < 1000; ++iter) {
-    const bool can_break = toi > T(0) && iter > 0;
-    if (!can_break && toi + tl > T(1)) return 0;  // the reference's distance here is unused
-    if (!skip) {
-      st.step(tl);
-      d = st.dist();
-      ++evals;
-    }
-    skip = false;
-    if (can_break) {
-      const T eps = cr_div(d * d - xsi * xsi, d + xsi);
-      if (eps < g) return 1;
-    }
-    toi += tl;
-    if (toi > T(1)) return 0;
-    tl = (T)(0.9 * (double)(d * d - xsi * xsi) / (double)((d + xsi) * l_p));
-  }
-  return 1;  // max_iter reached: the reference returns true (its assert is compiled out)
+  Accd<D, VF, T> a;
+  int s = a.template start<kAccdSuspendIter>(v0, v1, xsi);
+  if (s == ACCD_RUN) s = a.run();
+  toi = a.toi;
+  evals = a.evals;
+  return s == ACCD_HIT ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -122,6 +122,10 @@ struct AccdLaunch {
   uint8_t* hit_vf = nullptr;
   double* hit_toi = nullptr;
   unsigned long long hit_cap = 0;
+  // ACCD with the direct hit list and no chunk_hits suspends the stencils still
+  // running after three iterations and resumes them in dense chunks (same
+  // results); no_suspend runs every stencil in place.
+  int no_suspend = 0;
 };
 // Whether launch_accd appends hits directly for a queue of this capacity.
 inline bool accd_direct_hits(unsigned long long cap) { return cap < (1ull << 32); }

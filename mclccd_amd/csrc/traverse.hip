@@ -467,28 +467,40 @@ struct AccdArgs {
   double* hit_toi;
   unsigned long long hit_cap;
   int f32;                  // float-precision tree (CTCD: TOI rounded to float)
+  int suspend;               // suspend stencils still running after three iterations (see k_accd)
 };
 
 constexpr int AC_WAVES = 4;
 
 // Direct hit list: each wave stages its hits (queue index, toi) in LDS and
 // appends them with one counter atomic per flush of up to N hits.  3D:
-// 4 x 768 x 12 B = 36 KB per block keeps the four blocks per CU that the
-// kernels' registers allow; 2D (fewer registers, six blocks) stages 512.
-// Queue indices are 32-bit: the host only asks for the direct list when
-// both queues are shorter than 2^32.
-template <int D> constexpr int ac_stage() { return D == 3 ? 768 : 512; }  // multiples of 64
+// 4 x 448 x 12 B = 21 KB per block, which with the suspended-stencil stacks
+// (18 KB) keeps the four blocks per CU that the kernels' registers allow; 2D
+// (fewer registers, five blocks) stages 256.  Queue indices are 32-bit: the
+// host only asks for the direct list when both queues are shorter than 2^32.
+template <int D> constexpr int ac_stage() { return D == 3 ? 448 : 256; }  // multiples of 64
+
+// Per-wave stack of suspended stencils (Accd::resume state), SoA in LDS.
+// A wave resumes 64 of them as soon as it holds 64, so it never holds more
+// than 63 + 64.
+constexpr int kSuspendCap = 2 * kWave;
+struct SuspendStack {
+  uint32_t idx[AC_WAVES][kSuspendCap];
+  double t1[AC_WAVES][kSuspendCap], t2[AC_WAVES][kSuspendCap], t3[AC_WAVES][kSuspendCap], g[AC_WAVES][kSuspendCap];
+};
 
 template <int N> struct HitStage {
   uint32_t idx[AC_WAVES][N];
   double toi[AC_WAVES][N];
 };
 
+template <bool VF, int N>
+__device__ __forceinline__ void flush_hits(HitStage<N>& S, int w, int& cnt, const AccdArgs& A);
+
 // Called by the whole wave (converged) after each 64-stencil chunk.
 template <bool VF, int N>
 __device__ __forceinline__ void stage_hits(HitStage<N>& S, int w, int& cnt, bool hit, long long i, double toi,
                                            const AccdArgs& A, bool last) {
-  const int lane = lane_id();
   const uint64_t m = __ballot(hit);
   if (hit) {
     const int p = cnt + popc_below(m);
@@ -497,6 +509,12 @@ __device__ __forceinline__ void stage_hits(HitStage<N>& S, int w, int& cnt, bool
   }
   cnt += __popcll(m);
   if (cnt == 0 || (cnt <= N - kWave && !last)) return;
+  flush_hits<VF>(S, w, cnt, A);
+}
+
+template <bool VF, int N>
+__device__ __forceinline__ void flush_hits(HitStage<N>& S, int w, int& cnt, const AccdArgs& A) {
+  const int lane = lane_id();
   wave_lds_sync();
   unsigned long long base = 0;
   if (lane == 0) base = atomicAdd(A.counters + C_HITS, (unsigned long long)cnt);
@@ -530,45 +548,109 @@ __device__ __forceinline__ void narrow(const d2& a, vec2<float>& o) { o = {(floa
 __device__ __forceinline__ void narrow(const d3& a, d3& o) { o = a; }
 __device__ __forceinline__ void narrow(const d2& a, d2& o) { o = a; }
 
+// A stencil's vertices at t0 and t1 (narrowed to T) and its gap.
+template <int D, typename T, typename VT>
+__device__ __forceinline__ double load_stencil(const AccdArgs& A, const int4 sv, VT* v0, VT* v1) {
+  using V = typename VLoad<D>::V;
+  const int id[4] = {sv.x, sv.y, sv.z, sv.w};
+#pragma unroll
+  for (int k = 0; k < D + 1; ++k) {
+    V a, b;
+    VLoad<D>::load(A.rec, id[k], a, b);
+    narrow(a, v0[k]);
+    narrow(b, v1[k]);
+  }
+  return A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
+}
+
 // T = double: the FP64 path; T = float: BVHTree<float,DIM>'s float ACCD on the
 // float-valued vertex records of a float-precision tree (exact conversions).
-template <int D, bool VF, typename T> __global__ __launch_bounds__(AC_WAVES * kWave) void k_accd(AccdArgs A) {
-  using V = typename VLoad<D>::V;
+//
+// Lock-step waves run every stencil of a chunk for at most three distance
+// evaluations (most stencils need exactly three); the stencils still running
+// then are pushed on the wave's LDS stack of suspended stencils, and as soon as
+// it holds 64 the wave resumes them as one chunk (their vertex records were
+// read a few chunks earlier and are still cached).  A chunk therefore no
+// longer waits for its longest stencil.  Stencils whose iteration 0 was not
+// the no-op step run to completion in place; the results are those of the
+// uninterrupted loop (Accd::resume).  Without suspension (A.suspend = 0, as
+// whenever per-chunk hit counts are written) every stencil runs in place.
+template <int D, bool VF, typename T>
+__global__ __launch_bounds__(AC_WAVES * kWave) __attribute__((amdgpu_waves_per_eu(4))) void k_accd(AccdArgs A) {
   using VT = typename VecOf<T>::template V<D>;
   __shared__ HitStage<ac_stage<D>()> S;
+  __shared__ SuspendStack Q;
   const int w = threadIdx.x / kWave, lane = lane_id();
   long long evals = 0;
   int staged = 0;
+  int held = 0;  // suspended stencils on the wave's stack (wave-uniform)
+  const bool sus = A.suspend != 0;
   const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * AC_WAVES;
-  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
-    const long long i = ch * kWave + lane;
-    int hit = 0;
-    double toi = -1.0;
-    if (i < n) {
-      const int4 sv = A.q[i];
-      const int id[4] = {sv.x, sv.y, sv.z, sv.w};
-      VT v0[D + 1], v1[D + 1];
-#pragma unroll
-      for (int k = 0; k < D + 1; ++k) {
-        V a, b;
-        VLoad<D>::load(A.rec, id[k], a, b);
-        narrow(a, v0[k]);
-        narrow(b, v1[k]);
+  long long ch = (long long)xcd_block() * AC_WAVES + w;
+  while (ch < nchunks || held > 0) {
+    // the next chunk: 64 suspended stencils once the stack holds them (or the
+    // rest of the stack after the last fresh chunk), else 64 fresh stencils
+    const bool resume = held >= kWave || ch >= nchunks;
+    long long i = 0;
+    bool valid;
+    int st = ACCD_MISS;
+    Accd<D, VF, T> a;
+    if (resume) {
+      const int cnt = held < kWave ? held : kWave;
+      held -= cnt;
+      valid = lane < cnt;
+      if (valid) {
+        const int k = held + lane;  // the top cnt entries
+        i = (long long)Q.idx[w][k];
+        VT v0[D + 1], v1[D + 1];
+        const double eta = load_stencil<D, T>(A, A.q[i], v0, v1);
+        a.resume(v0, v1, (T)eta, (T)Q.t1[w][k], (T)Q.t2[w][k], (T)Q.t3[w][k], (T)Q.g[w][k]);
+        st = ACCD_RUN;
       }
-      const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
-      int ev = 0;
-      T t = T(-1);
-      hit = accd<D, VF, T>(v0, v1, (T)eta, t, ev);
-      toi = (double)t;
-      if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
-      evals += ev;
+      wave_lds_sync();  // the entries are read before later chunks push over them
+    } else {
+      i = ch * kWave + lane;
+      valid = i < n;
+      if (valid) {
+        VT v0[D + 1], v1[D + 1];
+        const double eta = load_stencil<D, T>(A, A.q[i], v0, v1);
+        st = a.template start<kAccdSuspendIter>(v0, v1, (T)eta);
+      }
+      if (sus) {
+        const bool push = st == ACCD_RUN && a.replayable;
+        const uint64_t m = __ballot(push);
+        if (push) {
+          const int k = held + popc_below(m);
+          Q.idx[w][k] = (uint32_t)i;
+          Q.t1[w][k] = (double)a.t1;
+          Q.t2[w][k] = (double)a.t2;
+          Q.t3[w][k] = (double)a.tl;
+          Q.g[w][k] = (double)a.g;
+          st = ACCD_SUSPENDED;
+        }
+        held += __popcll(m);
+        wave_lds_sync();
+      }
     }
-    const uint64_t m = __ballot(hit != 0);
-    if (lane == 0 && A.chunk_hits) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
-    if (A.hit_sten) stage_hits<VF>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
+    if (st == ACCD_RUN) st = a.run();
+    const bool hit = st == ACCD_HIT;
+    const double toi = hit ? (double)a.toi : -1.0;
+    if (valid) {
+      evals += a.evals;
+      if (A.cand_toi && st != ACCD_SUSPENDED) __builtin_nontemporal_store(toi, A.cand_toi + i);
+    }
+    if (!resume) {
+      if (A.chunk_hits) {  // only without suspension (the launcher's rule)
+        const uint64_t m = __ballot(hit);
+        if (lane == 0) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+      }
+      ch += stride;
+    }
+    if (A.hit_sten) stage_hits<VF>(S, w, staged, hit, i, toi, A, false);
   }
+  if (A.hit_sten && staged) flush_hits<VF>(S, w, staged, A);
   // one atomic per block (same-address atomics serialise)
   __shared__ long long s_ev[AC_WAVES];
   for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
@@ -714,6 +796,7 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   const long long items = L.n_hint >= 0 ? L.n_hint : (long long)L.cap;
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);
+  A.suspend = direct && !A.chunk_hits && !L.no_suspend;  // suspended hits would miss their chunk's count
   if (vf && L.one_sided) {
     // the host libm's values, as the reference computes them (rpoly.h:74-77)
     static const double rot = 94.0 * 0.017453293, cosr = std::cos(rot), sinr = std::sin(rot);
