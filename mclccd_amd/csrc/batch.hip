@@ -104,7 +104,7 @@ static int batch_buffers(Ctx* c) {
   CHECK(c->grow((void**)&c->d_nodes, &c->cap_nodes, node * (size_t)(2 * np - 1), "nodes"));
   CHECK(c->grow(&c->d_cp, &c->cap_cp, child_pair_bytes(dim) * (size_t)std::max(np - 1, 1), "child pairs"));
   CHECK(c->grow((void**)&c->d_root, &c->cap_root, sizeof(int), "root"));
-  CHECK(c->grow(&c->d_sboxes, &c->cap_sboxes, box * (size_t)np, "sorted leaf boxes"));
+  CHECK(c->grow(&c->d_sboxes, &c->cap_sboxes, sorted_leaf_bytes(dim) * (size_t)np, "sorted leaf boxes"));
   CHECK(c->grow(&c->d_face, &c->cap_face, face_bytes((size_t)np), "face arrays"));
   CHECK(c->grow((void**)&c->d_v1, &c->cap_v1, sizeof(double) * v1_stride(dim) * (size_t)nv, "V1"));
   const size_t ni = (size_t)std::max(np - 1, 1);

@@ -706,7 +706,7 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
                                                      const uint32_t* __restrict__ order, int n,
                                                      const LeafBox<D>* __restrict__ boxes,
                                                      const uint8_t* __restrict__ act, NodeF<D>* nodes,
-                                                     LeafBox<D>* __restrict__ sboxes, int* lchild,
+                                                     SortedLeaf<D>* __restrict__ sboxes, int* lchild,
                                                      int* rchild, int* rng_l, int* rng_r, int* visits, int4* frontier,
                                                      unsigned long long* nfrontier, SceneArgs S) {
   __shared__ NodeF<D> s_node[2 * LB_T];  // [0, LB_T): leaves, [LB_T, 2 LB_T): internal nodes
@@ -718,8 +718,17 @@ __global__ __launch_bounds__(LB_T) void k_lbvh_local(const uint32_t* __restrict_
   if (s < n) {
     const uint32_t prim = order[s];
     const LeafBox<D> lb = boxes[prim];
-    sboxes[s] = lb;
     NodeF<D> L = leaf_node<D>(lb, act[prim] != 0);
+    SortedLeaf<D> sl;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      sl.t0_lo[k] = lb.t0_lo[k]; sl.t0_hi[k] = lb.t0_hi[k];
+      sl.t1_lo[k] = lb.t1_lo[k]; sl.t1_hi[k] = lb.t1_hi[k];
+      sl.sw_lo[k] = lb.sw_lo[k]; sl.sw_hi[k] = lb.sw_hi[k];
+    }
+    sl.child = L.child;
+    sl.meta = L.meta;
+    sboxes[s] = sl;
     L.first = L.last = s;
     nodes[(n - 1) + s] = L;
     s_node[t] = L;
@@ -829,7 +838,7 @@ void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32
   const int gf = (lbvh_frontier_cap(n) + 255) / 256;
   if (dim == 3) {
     k_lbvh_local<3><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<3>*)boxes, act, (NodeF<3>*)nodes,
-                                       (LeafBox<3>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
+                                       (SortedLeaf<3>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
                                        nfrontier, S);
     if (n > 1) k_lbvh_cross<3><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<3>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);
@@ -838,7 +847,7 @@ void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32
                                                           (ChildPair<3>*)cp, root);
   } else {
     k_lbvh_local<2><<<g, LB_T, 0, s>>>(keys, hi, order, n, (const LeafBox<2>*)boxes, act, (NodeF<2>*)nodes,
-                                       (LeafBox<2>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
+                                       (SortedLeaf<2>*)sboxes, lchild, rchild, rng_l, rng_r, visits, frontier,
                                        nfrontier, S);
     if (n > 1) k_lbvh_cross<2><<<gf, 256, 0, s>>>(keys, hi, n, (NodeF<2>*)nodes, lchild, rchild, rng_l, rng_r, visits,
                                                   frontier, nfrontier, S);

@@ -271,7 +271,7 @@ int Ctx::update(const double* V0, const double* V1, int nv_, const int* P, int n
   CHECK(grow((void**)&d_visits, &cap_visits, sizeof(int) * ni, "visits"));
   CHECK(grow((void**)&d_frontier, &cap_frontier, sizeof(int4) * (size_t)lbvh_frontier_cap(np), "frontier"));
   CHECK(grow((void**)&d_nfrontier, &cap_nfrontier, sizeof(unsigned long long), "frontier"));
-  CHECK(grow(&d_sboxes, &cap_sboxes, ldbox_bytes(dim) * (size_t)np, "sorted leaf boxes"));
+  CHECK(grow(&d_sboxes, &cap_sboxes, sorted_leaf_bytes(dim) * (size_t)np, "sorted leaf boxes"));
   CHECK(grow(&d_cp, &cap_cp, child_pair_bytes(dim) * ni, "child pairs"));
   CHECK(grow((void**)&d_root, &cap_root, sizeof(int), "root"));
   launch_lbvh(dim, d_keys, nullptr, d_order, np, d_boxes, d_act, d_nodes, d_sboxes, d_lchild, d_rchild, d_rngl, d_rngr,

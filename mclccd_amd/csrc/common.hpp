@@ -20,6 +20,17 @@ template <> struct VRec<2> { static constexpr int stride = 4; static constexpr i
 // mcl::BoundingBox (BVHLeaf.hpp:12-29) flattened.
 template <int D> struct LeafBox { double sw_lo[D], sw_hi[D], t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D]; };
 
+// A leaf in sorted order as the traversal reads it: the FP64 t0 and t1 boxes
+// and the leaf's flags (child: 1 if its t0 or t1 box is empty; meta:
+// active<<31) in the first two 64 B lines, the swept box behind them for the
+// empty-box fallback.  192 B (3D) / 128 B (2D), 64 B aligned.
+template <int D> struct alignas(64) SortedLeaf {
+  double t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D];
+  int child;
+  uint32_t meta;
+  double sw_lo[D], sw_hi[D];
+};
+
 // Tree node (64 B, one aligned half line): the t0 and t1 boxes rounded outward
 // to FP32 (conservative for the culling test), meta = active<<31 (the child-pair
 // copies below put the node's index in the low bits), and the range of sorted
@@ -45,6 +56,7 @@ template <> struct alignas(16) NodeF<2> {
 };
 
 static_assert(sizeof(NodeF<3>) == 64 && sizeof(NodeF<2>) == 48, "node layout");
+static_assert(sizeof(SortedLeaf<3>) == 192 && sizeof(SortedLeaf<2>) == 128, "sorted leaf layout");
 
 // Both children of internal node p side by side (one 128 B line in 3D), as the
 // traversal reads them: copies of their NodeF with meta = (index+1) |

@@ -48,7 +48,9 @@ void launch_scan_u64(unsigned long long* data, long long n, unsigned long long* 
 void launch_radix_sort(uint32_t* keys, uint32_t* vals, uint32_t* ktmp, uint32_t* vtmp, uint32_t* hist, int n, int nbits,
                        hipStream_t s);
 int lbvh_frontier_cap(int n);
-// Also writes sboxes[s] = boxes[order[s]] (FP64 leaf boxes in sorted order)
+inline size_t sorted_leaf_bytes(int dim) { return dim == 3 ? sizeof(SortedLeaf<3>) : sizeof(SortedLeaf<2>); }
+// Also writes sboxes[s] = boxes[order[s]] with the leaf's flags (SortedLeaf<dim>,
+// sorted_leaf_bytes(dim) each)
 // and, when cp is given and n > 1, the child-pair records (ChildPair<dim> per
 // internal node) and *root.
 void launch_lbvh(int dim, const uint32_t* keys, const uint32_t* hi, const uint32_t* order, int n, const void* boxes,

@@ -47,7 +47,8 @@ template <int D> __device__ __forceinline__ bool cull_f32(const NodeF<D>& q, con
 // separates both at t0 and at t1.  The traversal's fallback for pairs with an
 // empty box, kept out of line and small in registers.
 template <int D>
-__device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bool la, const LeafBox<D>* r, bool ra) {
+__device__ __attribute__((noinline)) bool leaf_test_fp64(const SortedLeaf<D>* l, bool la, const SortedLeaf<D>* r,
+                                                         bool ra) {
   bool ov = la | ra, emp = false, sep = false;
 #pragma nounroll
   for (int k = 0; k < D; ++k) {
@@ -61,7 +62,7 @@ __device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bo
   return ov & (emp | !sep);
 }
 
-// The t0 and t1 FP64 boxes of a sorted leaf (the tail of its LeafBox).  With
+// The t0 and t1 FP64 boxes of a sorted leaf (the head of its record).  With
 // both boxes non-empty the exact test is (la | ra) & !sep: disjoint swept boxes
 // on an axis separate that axis at t0 and at t1, so the swept overlap is implied.
 // FP64 boxes hold no NaN (box extension skips NaN coordinates; an axis of NaNs
@@ -69,7 +70,7 @@ __device__ __attribute__((noinline)) bool leaf_test_fp64(const LeafBox<D>* l, bo
 template <int D> struct Box01 {
   double t0_lo[D], t0_hi[D], t1_lo[D], t1_hi[D];
 };
-template <int D> __device__ __forceinline__ const Box01<D>* box01(const LeafBox<D>* b) {
+template <int D> __device__ __forceinline__ const Box01<D>* box01(const SortedLeaf<D>* b) {
   return reinterpret_cast<const Box01<D>*>(b->t0_lo);
 }
 template <int D> __device__ __forceinline__ bool separated(const Box01<D>& q, const Box01<D>& b) {
@@ -78,6 +79,19 @@ template <int D> __device__ __forceinline__ bool separated(const Box01<D>& q, co
   for (int a = 0; a < D; ++a)
     sep |= ((q.t0_lo[a] > b.t0_hi[a]) & (q.t1_lo[a] > b.t1_hi[a])) | ((q.t0_hi[a] < b.t0_lo[a]) & (q.t1_hi[a] < b.t1_lo[a]));
   return sep;
+}
+
+// The same cull with the query's exact FP64 t0 / t1 boxes against a node's
+// outward-rounded FP32 boxes (converted exactly): the FP64 box lies inside the
+// query's FP32 box, so this culls at least as much and never a pair the exact
+// test accepts.  A query with an empty FP64 t0 or t1 box (qemp) never culls.
+template <int D> __device__ __forceinline__ bool cull_q64(const Box01<D>& q, bool qa, bool qemp, const NodeF<D>& b) {
+  bool sep = false;
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    sep |= ((q.t0_lo[k] > (double)b.t0_hi[k]) & (q.t1_lo[k] > (double)b.t1_hi[k])) |
+           ((q.t0_hi[k] < (double)b.t0_lo[k]) & (q.t1_hi[k] < (double)b.t1_lo[k]));
+  return (qa | ((b.meta & 0x80000000u) != 0)) & (qemp | !sep);
 }
 
 // Whole-record loads as 16-byte vectors (sizes are multiples of 16).  With
@@ -137,8 +151,9 @@ constexpr int TR_STACK = 96;  // deeper than any LBVH over 32-bit keys and int i
 // child-pair load, ballots, stack) costs more than a few leaf tests: at 1M
 // triangles the traversal (then testing leaves on FP32 bounds) took 11.7 ms
 // without buckets, 10.8 / 10.2 / 10.1 / 10.1 ms with buckets of 4 / 8 / 16 /
-// 32 leaves (12.1 ms with 2).
-constexpr int TR_BUCKET = 16;
+// 32 leaves (12.1 ms with 2); with the FP64 leaf records, 10.5 / 9.6 / 9.3 /
+// 9.0 / 9.2 / 9.3 ms with 4 / 8 / 16 / 32 / 64 / 128.
+constexpr int TR_BUCKET = 32;
 
 // Stages the lanes' pairs (s, k) of one leaf k; flushes a full buffer.
 __device__ __forceinline__ void stage_pairs(bool emit, int s, int k, int2* buf, int& cnt, int2* __restrict__ pairs,
@@ -156,7 +171,7 @@ __device__ __forceinline__ void stage_pairs(bool emit, int s, int k, int2* buf, 
 template <int D>
 __global__ __launch_bounds__(TR_MAX_WAVES * kWave)
 void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restrict__ cp, const int* __restrict__ root,
-                const LeafBox<D>* __restrict__ sboxes, int n, int2* __restrict__ pairs, unsigned long long cap,
+                const SortedLeaf<D>* __restrict__ sboxes, int n, int2* __restrict__ pairs, unsigned long long cap,
                 unsigned long long* counters, const uint32_t* __restrict__ scene_of_sorted,
                 const int* __restrict__ scene_off) {
   __shared__ int2 s_buf[TR_MAX_WAVES][TR_BUF];
@@ -165,15 +180,14 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
   const int s = xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = s < n - 1;
   if (__ballot(valid) == 0) return;  // wave-uniform: the wave starts at or after the last leaf
-  NodeF<D> q;
   Box01<D> qd;
   bool qa = false, qemp = false;
   int last = -1;
   if (valid) {
-    q = load16(nodes + (n - 1) + s);
+    const int2 cm = *reinterpret_cast<const int2*>(&sboxes[s].child);
     qd = load16(box01<D>(sboxes + s));
-    qa = (q.meta >> 31) != 0;
-    qemp = q.child != 0;
+    qa = ((uint32_t)cm.y >> 31) != 0;
+    qemp = cm.x != 0;
     last = scene_off ? scene_off[scene_of_sorted[s] + 1] - 1 : n - 1;
   }
   int wave_last = last;
@@ -200,7 +214,7 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
       const NodeF<D>& c = pr.c[j];
-      const bool want = valid && c.first <= wave_last && c.last > s && c.first <= last && cull_f32<D>(q, c);
+      const bool want = valid && c.first <= wave_last && c.last > s && c.first <= last && cull_q64<D>(qd, qa, qemp, c);
       g[j] = __ballot(want) != 0;
     }
     // leaves and small subtrees: their sorted leaves are tested here, on the
@@ -210,10 +224,10 @@ void k_traverse(const NodeF<D>* __restrict__ nodes, const ChildPair<D>* __restri
       const NodeF<D>& c = pr.c[j];
       if (!g[j] || c.last - c.first >= TR_BUCKET) continue;
       for (int k = c.first; k <= c.last; ++k) {
-        const Box01<D> B = *box01<D>(sboxes + k);
-        const NodeF<D>& Ln = nodes[(n - 1) + k];
-        const uint32_t bm = Ln.meta;
-        const int bemp = Ln.child;
+        const SortedLeaf<D>& Lk = sboxes[k];  // two 64 B lines
+        const Box01<D> B = *box01<D>(&Lk);
+        const uint32_t bm = Lk.meta;
+        const int bemp = Lk.child;
         const bool in = valid & (k > s) & (k <= last);
         const bool ba = (bm >> 31) != 0;
         bool emit = in & (qa | ba) & !separated<D>(qd, B);
@@ -257,11 +271,11 @@ void launch_traverse(int dim, const void* nodes, const void* cp, const int* root
   const int g = xcd_grid(n - 1, waves * kWave);
   if (dim == 3)
     k_traverse<3><<<g, waves * kWave, 0, s>>>((const NodeF<3>*)nodes, (const ChildPair<3>*)cp, root,
-                                                 (const LeafBox<3>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
+                                                 (const SortedLeaf<3>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
                                                  scene_off);
   else
     k_traverse<2><<<g, waves * kWave, 0, s>>>((const NodeF<2>*)nodes, (const ChildPair<2>*)cp, root,
-                                                 (const LeafBox<2>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
+                                                 (const SortedLeaf<2>*)sboxes, n, pairs, cap, counters, scene_of_sorted,
                                                  scene_off);
 }
 
