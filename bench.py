@@ -491,7 +491,7 @@ def main():
                               "no copy to the host and no callback replay inside the timed region",
                    "leaf_pairs": c["leaf_pairs"], "candidates": c["candidates"], "hits": c["hits"],
                    "discrete": c["discrete"], "dist_evals": c["dist_evals"],
-                   "traverse_wave_steps": c["traverse_steps"],
+                   "traverse_wave_steps": c["traverse_steps"], "accd_suspended": c["suspended"],
                    "phase_ms": {k: round(v, 4) for k, v in avg.items()}},
         "roofline": roof,
         "kernels": others,

@@ -125,6 +125,8 @@ typedef struct mclg_counts {
   int64_t ee_dist_evals;  /* of dist_evals: those of edge-edge stencils */
   int64_t order_dependent; /* leaf pairs whose discrete result depends on the argument order (MCLG_CHECK_ORDER) */
   int64_t traverse_steps;  /* traversal wave-steps: child-pair records read, summed over the 64-leaf packets */
+  int64_t suspended;       /* ACCD stencils suspended after three iterations and resumed later (same results;
+                              0 on frames whose hit list may still outgrow its capacity) */
 } mclg_counts;
 int mclg_get_counts(const mclg_ctx* ctx, mclg_counts* out);
 

@@ -43,7 +43,7 @@ class Counts(C.Structure):
     _fields_ = [("leaf_pairs", C.c_int64), ("candidates", C.c_int64), ("hits", C.c_int64),
                 ("discrete", C.c_int64), ("dist_evals", C.c_int64),
                 ("vf_candidates", C.c_int64), ("ee_dist_evals", C.c_int64), ("order_dependent", C.c_int64),
-                ("traverse_steps", C.c_int64)]
+                ("traverse_steps", C.c_int64), ("suspended", C.c_int64)]
 
 
 class DeviceResults(C.Structure):

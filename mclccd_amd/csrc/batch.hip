@@ -257,6 +257,7 @@ static int batch_frame(Ctx* c) {
   c->counts.hits = (int64_t)h[C_HITS];
   c->counts.discrete = (int64_t)h[C_DISCRETE];
   c->counts.dist_evals = (int64_t)h[C_DIST_EVALS];
+  c->counts.suspended = (int64_t)h[C_SUSPENDED];
   c->counts.vf_candidates = (int64_t)h[C_CAND_VF];
   c->counts.ee_dist_evals = (int64_t)h[C_EE_EVALS];
   c->counts.traverse_steps = (int64_t)h[C_TR_STEPS];

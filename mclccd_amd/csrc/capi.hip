@@ -408,6 +408,7 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
       }
       launch_fill_u64(d_counters + C_HITS, 1, 0ull, stream);
       launch_fill_u64(d_counters + C_DIST_EVALS, 1, 0ull, stream);
+      launch_fill_u64(d_counters + C_SUSPENDED, 1, 0ull, stream);
       AccdLaunch L;
       L.rec = d_rec; L.counters = d_counters; L.num_cus = num_cus; L.blocks_per_cu = blocks_per_cu; 
       L.vert_scene = nullptr; L.scene_eta = nullptr; L.f32 = hits_f32;
@@ -450,13 +451,14 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
         CHECK(read_counters());
         if (h_counters[C_HITS] <= hit_cap) break;
         if (no_overflow) return fail(MCLG_ERR_STATE, "narrow phase: more hits than candidates");
-        // hits never outnumber candidates: a list that size never needs this pass again
-        hit_cap = std::min<unsigned long long>(h_counters[C_HITS] + h_counters[C_HITS] / 4,
-                                               (unsigned long long)(n_vf + n_ee));
+        // hits never outnumber candidates: a list that size never needs this pass
+        // again, and lets the ACCD kernels suspend long-running stencils
+        hit_cap = (unsigned long long)(n_vf + n_ee);
         if (attempt > 4) return fail(MCLG_ERR_STATE, "narrow phase: sizes did not converge");
       }
       counts.hits = (int64_t)h_counters[C_HITS];
       counts.dist_evals = (int64_t)h_counters[C_DIST_EVALS];
+      counts.suspended = (int64_t)h_counters[C_SUSPENDED];
       counts.ee_dist_evals = (int64_t)h_counters[C_EE_EVALS];
     }
   }

@@ -104,7 +104,8 @@ enum Counter : int {
   C_EE_EVALS,   // distance evaluations of edge-edge stencils (also counted in C_DIST_EVALS)
   C_ORDER_DEP,  // leaf pairs whose discrete result depends on the argument order (when checked)
   C_TR_STEPS,   // traversal wave-steps (child-pair records visited, summed over waves)
-  C_COUNT = 10
+  C_SUSPENDED,  // ACCD stencils suspended after three iterations and resumed
+  C_COUNT = 11
 };
 
 // ---- wave helpers ---------------------------------------------------------

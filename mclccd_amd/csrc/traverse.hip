@@ -598,6 +598,7 @@ __global__ __launch_bounds__(AC_WAVES * kWave) __attribute__((amdgpu_waves_per_e
   long long evals = 0;
   int staged = 0;
   int held = 0;  // suspended stencils on the wave's stack (wave-uniform)
+  unsigned pushed = 0;  // stencils this wave suspended (wave-uniform)
   const bool sus = A.suspend != 0;
   const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
@@ -645,6 +646,7 @@ __global__ __launch_bounds__(AC_WAVES * kWave) __attribute__((amdgpu_waves_per_e
           st = ACCD_SUSPENDED;
         }
         held += __popcll(m);
+        pushed += (unsigned)__popcll(m);
         wave_lds_sync();
       }
     }
@@ -665,17 +667,26 @@ __global__ __launch_bounds__(AC_WAVES * kWave) __attribute__((amdgpu_waves_per_e
     if (A.hit_sten) stage_hits<VF>(S, w, staged, hit, i, toi, A, false);
   }
   if (A.hit_sten && staged) flush_hits<VF>(S, w, staged, A);
-  // one atomic per block (same-address atomics serialise)
+  // one atomic per block and counter (same-address atomics serialise)
   __shared__ long long s_ev[AC_WAVES];
+  __shared__ unsigned s_sus[AC_WAVES];
   for (int o = 32; o > 0; o >>= 1) evals += __shfl_xor(evals, o);
-  if (lane == 0) s_ev[w] = evals;
+  if (lane == 0) {
+    s_ev[w] = evals;
+    s_sus[w] = pushed;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < AC_WAVES; ++k) evals += s_ev[k];
+    unsigned long long sus = pushed;
+    for (int k = 1; k < AC_WAVES; ++k) {
+      evals += s_ev[k];
+      sus += s_sus[k];
+    }
     if (evals) {
       atomicAdd(A.counters + C_DIST_EVALS, (unsigned long long)evals);
       if (!VF) atomicAdd(A.counters + C_EE_EVALS, (unsigned long long)evals);
     }
+    if (sus) atomicAdd(A.counters + C_SUSPENDED, sus);
   }
 }
 

@@ -5,7 +5,8 @@ the host (tests/cpp/accd_resume_check.cpp, hipcc host-only) and checks, on
 every candidate stencil of generated scenes, that both the uninterrupted run
 and suspend + resume on a fresh state give the oracle's additive_ccd hit and
 TOI bit for bit (NarrowPhase.cpp:634-711), with the same number of distance
-evaluations."""
+evaluations, in double and in float (the float tree's ACCD on the
+float-rounded coordinates)."""
 import os
 import shutil
 import subprocess
@@ -49,7 +50,7 @@ def _run(exe, path, dim, V0, V1, sten, is_vf, eta):
     line = [l for l in r.stdout.splitlines() if l.startswith("accd_resume_check:")]
     assert r.returncode == 0 and line, r.stdout[-2000:] + r.stderr[-2000:]
     parts = line[-1].split()
-    return int(parts[1]), int(parts[3])
+    return int(parts[1]), int(parts[3]), int(parts[7])
 
 
 SCENES = {
@@ -68,9 +69,9 @@ def test_suspended_stencils_resume_to_the_oracle_toi(checker, tmp_path, name):
     dim = V0.shape[1]
     if dim == 2:
         vf = np.ones(len(sten), np.uint8)
-    n, suspended = _run(checker, str(tmp_path / "in.bin"), dim, V0, V1, sten, vf, th)
+    n, suspended, suspended_f32 = _run(checker, str(tmp_path / "in.bin"), dim, V0, V1, sten, vf, th)
     assert n == len(sten) > 1000
-    assert suspended > 0  # the resume path is exercised on every scene
+    assert suspended > 0 and suspended_f32 > 0  # the resume path is exercised on every scene, in both precisions
 
 
 def test_long_iterations_resume(checker, tmp_path):
@@ -94,15 +95,15 @@ def test_long_iterations_resume(checker, tmp_path):
         st.append(base + np.arange(4))
         vf.append(1 if is_vf else 0)
         base += 4
-    n, suspended = _run(checker, str(tmp_path / "long.bin"), 3, np.concatenate(V0s), np.concatenate(V1s),
-                        np.array(st, np.int32), np.array(vf, np.uint8), 1e-3)
-    assert n == 4000 and suspended > 100
+    n, suspended, suspended_f32 = _run(checker, str(tmp_path / "long.bin"), 3, np.concatenate(V0s),
+                                       np.concatenate(V1s), np.array(st, np.int32), np.array(vf, np.uint8), 1e-3)
+    assert n == 4000 and suspended > 100 and suspended_f32 > 100
     R = 5000.0
     ang = np.array([0, 2 * np.pi / 3, 4 * np.pi / 3])
     face = np.stack([R * np.cos(ang), R * np.sin(ang), np.zeros(3)], 1)
     face -= face.mean(0)
     v0 = np.vstack([[0, 0, 1.0], face])
     v1 = np.vstack([[0, 0, 2.0], -face])
-    n, suspended = _run(checker, str(tmp_path / "max.bin"), 3, v0, v1, np.array([[0, 1, 2, 3]], np.int32),
-                        np.array([1], np.uint8), 1e-12)
+    n, suspended, _ = _run(checker, str(tmp_path / "max.bin"), 3, v0, v1, np.array([[0, 1, 2, 3]], np.int32),
+                           np.array([1], np.uint8), 1e-12)
     assert n == 1 and suspended == 1

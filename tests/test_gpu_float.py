@@ -83,6 +83,29 @@ def test_float_tree_equals_oracle_float_tree(gpu, case):
         assert canon_pairs(t2.discrete()) == canon_pairs(o.discrete())
 
 
+@pytest.mark.parametrize("case", ["layers", "disk"])
+def test_float_tree_later_frames(gpu, case):
+    # Once a frame has outgrown the first hit list, the list holds every
+    # candidate, so the float ACCD kernels append hits directly and suspend the
+    # stencils still running after three iterations (resumed 64 at a time):
+    # same float TOIs bit for bit.
+    V0, V1, F, th = CASES[case]()
+    dim = V0.shape[1]
+    o = run_scene(V0, V1, F, th, f32=True)
+    rs, rv, rt = o.hits()
+    t = _tree(dim, th)
+    for frame in range(3):
+        print(f"[float] later frames: case {case} frame {frame}", flush=True)
+        t.update(V0, V1, F)
+        t.run(V0, V1, F)
+        hs, hv, ht = t.hits()
+        assert _records(hs, hv, ht) == _records(rs, rv, rt), f"frame {frame}"
+        c = t.counts()
+        if frame == 0:
+            grew = c["hits"] > max(1024, c["candidates"] // 2)  # the first list holds half the candidates
+        assert (c["suspended"] > 0) == (grew and frame > 0), (frame, c)
+
+
 @pytest.mark.parametrize("dim", [3, 2])
 def test_float_ctcd(gpu, dim):
     if dim == 3:

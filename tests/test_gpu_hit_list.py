@@ -51,6 +51,8 @@ def test_fallback_then_direct(gpu, scene):
         if frame == 0:
             overflow = c["hits"] > max(1024, c["candidates"] // 2)
             assert overflow == (scene == "layers"), (c["hits"], c["candidates"])
+        # after an overflow the list holds every candidate: long-running stencils are suspended
+        assert (c["suspended"] > 0) == (overflow and frame > 0), (frame, c)
         _check(t, o, f"{scene} frame {frame}")
     print(f"[hits] {scene}: {c['candidates']} candidates, {c['hits']} hits")
 

@@ -198,7 +198,8 @@ def test_tiny(gpu, nprim):
     c = t.counts()
     if nprim <= 1:
         assert c == {"leaf_pairs": 0, "candidates": 0, "hits": 0, "discrete": 0, "dist_evals": 0,
-                     "vf_candidates": 0, "ee_dist_evals": 0, "order_dependent": 0, "traverse_steps": 0}
+                     "vf_candidates": 0, "ee_dist_evals": 0, "order_dependent": 0, "traverse_steps": 0,
+                     "suspended": 0}
     else:
         o = run_scene(V0, V1, F, 1e-2)
         check_parity(t, o, strict=False)
