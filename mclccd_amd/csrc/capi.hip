@@ -417,6 +417,11 @@ int Ctx::traverse(const double* V0, const double* V1, const int* P, unsigned fla
         CHECK(grow((void**)&d_hit_vf, &cap_hit_vf, hit_cap, "hits"));
         CHECK(grow((void**)&d_hit_toi, &cap_hit_toi, sizeof(double) * hit_cap, "hits"));
         L.hit_sten = d_hit_sten; L.hit_vf = d_hit_vf; L.hit_toi = d_hit_toi; L.hit_cap = hit_cap;
+        if (no_overflow && opt.vf_one_sided && dim == 3) {  // CTCD on the prefilter's survivors only
+          CHECK(grow((void**)&d_ctcd_sel, &cap_ctcd_sel, sizeof(uint32_t) * (size_t)vf_cap, "CTCD selection"));
+          launch_fill_u64(d_counters + C_CTCD_SEL, 1, 0ull, stream);
+          L.ctcd_sel = d_ctcd_sel;
+        }
       }
       mark(12);
       L.q = d_vfq; L.n_dev = d_counters + C_CAND_VF; L.cap = vf_cap; L.n_hint = n_vf;
@@ -494,7 +499,7 @@ Ctx::~Ctx() {
   if (stream) (void)hipStreamSynchronize(stream);
   void* bufs[] = {d_rec, d_flags, d_owner, d_ekeys, d_evals, d_boxes, d_act, d_cbounds, d_partial, d_keys, d_order, d_ktmp,
                   d_vtmp, d_hist, d_sboxes, d_face, d_v1, d_cp, d_root, d_ex_off, d_ex_part, d_hit_off, d_hit_part, d_nodes, d_lchild, d_rchild, d_rngl, d_rngr, d_visits, d_frontier, d_nfrontier, d_counters, d_pairs,
-                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_odep, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
+                  d_hit_sten, d_hit_vf, d_hit_toi, d_disc, d_odep, d_vfq, d_eeq, d_vf_toi, d_ee_toi, d_ctcd_sel, d_rank, d_minmax, d_adj_w, d_adj_g, d_adj_q, d_adj_n, d_adj_keys, d_adj_vals, d_adj_ktmp, d_adj_vtmp, d_adj_hist,
                   d_stage_v0, d_stage_v1, d_stage_p, d_stage_act};
   for (void* p : bufs)
     if (p) (void)hipFree(p);

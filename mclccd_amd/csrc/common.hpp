@@ -105,7 +105,8 @@ enum Counter : int {
   C_ORDER_DEP,  // leaf pairs whose discrete result depends on the argument order (when checked)
   C_TR_STEPS,   // traversal wave-steps (child-pair records visited, summed over waves)
   C_SUSPENDED,  // ACCD stencils suspended after three iterations and resumed
-  C_COUNT = 11
+  C_CTCD_SEL,   // VF stencils that pass the CTCD prefilter (the selection pass's list length)
+  C_COUNT = 12
 };
 
 // ---- wave helpers ---------------------------------------------------------

@@ -107,6 +107,7 @@ struct Ctx {
   unsigned long long vf_cap = 0, ee_cap = 0;
   double* d_vf_toi = nullptr; size_t cap_vf_toi = 0;
   double* d_ee_toi = nullptr; size_t cap_ee_toi = 0;
+  uint32_t* d_ctcd_sel = nullptr; size_t cap_ctcd_sel = 0;  // 3D CTCD: stencils that pass the prefilter
   long long n_vf = 0, n_ee = 0;
   int* d_rank = nullptr; size_t cap_rank = 0;
   int* d_minmax = nullptr; size_t cap_minmax = 0;

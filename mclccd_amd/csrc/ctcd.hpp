@@ -1036,7 +1036,9 @@ template <> struct Query<3> {
     return col;
   }
 
-  __device__ static int run(const d3* q0, const d3* q1, double eta, double& toi, double cosr, double sinr) {
+  // The query's prefilter: the vertex's path against the face's box over both
+  // time steps, padded by eta; false means no hit (run returns 0).
+  __device__ static bool prefilter(const d3* q0, const d3* q1, double eta) {
     d3 lo = {1.79769313486231570815e+308, 1.79769313486231570815e+308, 1.79769313486231570815e+308};
     d3 hi = {-lo.x, -lo.y, -lo.z};
 #pragma unroll
@@ -1049,7 +1051,11 @@ template <> struct Query<3> {
     }
     lo = {lo.x - eta, lo.y - eta, lo.z - eta};
     hi = {hi.x + eta, hi.y + eta, hi.z + eta};
-    if (!ray_box(q0[0], q1[0], lo, hi)) return 0;
+    return ray_box(q0[0], q1[0], lo, hi);
+  }
+
+  __device__ static int run(const d3* q0, const d3* q1, double eta, double& toi, double cosr, double sinr) {
+    if (!prefilter(q0, q1, eta)) return 0;
     const double minD = eta * eta;
     Times T;
     T.n = 0;

@@ -128,6 +128,10 @@ struct AccdLaunch {
   // running after three iterations and resumes them in dense chunks (same
   // results); no_suspend runs every stencil in place.
   int no_suspend = 0;
+  // 3D CTCD (one_sided) with the direct hit list and no chunk_hits: cap entries
+  // for the queue indices of the stencils that pass the prefilter, listed by a
+  // selection pass (counters[C_CTCD_SEL], cleared by the caller); null: one pass.
+  uint32_t* ctcd_sel = nullptr;
 };
 // Whether launch_accd appends hits directly for a queue of this capacity.
 inline bool accd_direct_hits(unsigned long long cap) { return cap < (1ull << 32); }

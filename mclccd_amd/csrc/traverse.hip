@@ -482,6 +482,7 @@ struct AccdArgs {
   unsigned long long hit_cap;
   int f32;                  // float-precision tree (CTCD: TOI rounded to float)
   int suspend;               // suspend stencils still running after three iterations (see k_accd)
+  uint32_t* sel;             // 3D CTCD: queue indices of the stencils that pass the prefilter (null: none)
 };
 
 constexpr int AC_WAVES = 4;
@@ -692,20 +693,27 @@ __global__ __launch_bounds__(AC_WAVES * kWave) __attribute__((amdgpu_waves_per_e
 
 // CTCD vertex-face narrow phase (options.vf_one_sided; BVHTree.cpp:446-447):
 // the VF queue, outputs and per-chunk hit counts of the VF ACCD kernel.  cosr,
-// sinr: the root finder's shift rotation, cos/sin(94 * 0.017453293).
-template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(AccdArgs A, double cosr, double sinr) {
+// sinr: the root finder's shift rotation, cos/sin(94 * 0.017453293).  SEL: only
+// the stencils listed in A.sel (counters[C_CTCD_SEL] of them) by the selection
+// pass; lock-step waves then hold no lane that the prefilter would retire at
+// once while the others run the root finder.
+template <int D, bool SEL>
+__global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(AccdArgs A, double cosr, double sinr) {
   using V = typename VLoad<D>::V;
   __shared__ HitStage<ac_stage<D>()> S;
   const int w = threadIdx.x / kWave, lane = lane_id();
   int staged = 0;
-  const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
+  const unsigned long long cnt = SEL ? A.counters[C_CTCD_SEL] : *A.n_dev;
+  const long long n = (long long)(cnt < A.cap ? cnt : A.cap);
   const long long nchunks = (n + kWave - 1) / kWave;
   const long long stride = (long long)gridDim.x * AC_WAVES;
   for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
-    const long long i = ch * kWave + lane;
+    const long long j = ch * kWave + lane;
+    long long i = j;
     int hit = 0;
     double toi = -1.0;
-    if (i < n) {
+    if (j < n) {
+      if (SEL) i = (long long)A.sel[j];
       const int4 sv = A.q[i];
       const int id[4] = {sv.x, sv.y, sv.z, sv.w};
       V v0[D + 1], v1[D + 1];
@@ -716,10 +724,42 @@ template <int D> __global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd(Accd
       if (A.f32 && hit) toi = (double)(float)toi;  // NarrowPhase.cpp:425-444: t_impact = t (a float)
       if (A.cand_toi) __builtin_nontemporal_store(hit ? toi : -1.0, A.cand_toi + i);
     }
-    const uint64_t m = __ballot(hit != 0);
-    if (lane == 0 && A.chunk_hits) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    if (!SEL) {
+      const uint64_t m = __ballot(hit != 0);
+      if (lane == 0 && A.chunk_hits) A.chunk_hits[ch] = (unsigned long long)__popcll(m);
+    }
     if (A.hit_sten) stage_hits<true>(S, w, staged, hit != 0, i, toi, A, ch + stride >= nchunks);
   }
+}
+
+// The selection pass of the 3D CTCD: the prefilter per VF stencil; the ones
+// that pass are appended to A.sel (per-wave LDS staging, one counter atomic per
+// flush), the others are misses.
+constexpr int CT_SEL_STAGE = 512;
+__global__ __launch_bounds__(AC_WAVES * kWave) void k_ctcd_select(AccdArgs A) {
+  using V = typename VLoad<3>::V;
+  __shared__ uint32_t s_sel[AC_WAVES][CT_SEL_STAGE];
+  const int w = threadIdx.x / kWave, lane = lane_id();
+  int staged = 0;
+  const long long n = (long long)(*A.n_dev < A.cap ? *A.n_dev : A.cap);
+  const long long nchunks = (n + kWave - 1) / kWave;
+  const long long stride = (long long)gridDim.x * AC_WAVES;
+  for (long long ch = (long long)xcd_block() * AC_WAVES + w; ch < nchunks; ch += stride) {
+    const long long i = ch * kWave + lane;
+    bool pass = false;
+    if (i < n) {
+      const int4 sv = A.q[i];
+      const int id[4] = {sv.x, sv.y, sv.z, sv.w};
+      V v0[4], v1[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) VLoad<3>::load(A.rec, id[k], v0[k], v1[k]);
+      const double eta = A.vert_scene ? A.scene_eta[A.vert_scene[sv.x]] : A.eta;
+      pass = ctcd::Query<3>::prefilter(v0, v1, eta);
+      if (!pass && A.cand_toi) __builtin_nontemporal_store(-1.0, A.cand_toi + i);
+    }
+    stage_push<uint32_t, CT_SEL_STAGE>(s_sel[w], staged, pass, (uint32_t)i, A.sel, A.cap, A.counters + C_CTCD_SEL);
+  }
+  if (staged) stage_flush(s_sel[w], staged, A.sel, A.cap, A.counters + C_CTCD_SEL);
 }
 
 __global__ void k_hits_total(const unsigned long long* off, long long last, unsigned long long* counters) {
@@ -822,11 +862,20 @@ void launch_accd(int dim, bool vf, const AccdLaunch& L, hipStream_t s) {
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave, AC_WAVES, L.num_cus, L.blocks_per_cu);
   A.suspend = direct && !A.chunk_hits && !L.no_suspend;  // suspended hits would miss their chunk's count
+  // 3D CTCD on the prefilter's survivors only (the same condition: hits of a
+  // listed stencil would miss their chunk's count)
+  A.sel = (dim == 3 && direct && !A.chunk_hits && L.ctcd_sel) ? L.ctcd_sel : nullptr;
   if (vf && L.one_sided) {
     // the host libm's values, as the reference computes them (rpoly.h:74-77)
     static const double rot = 94.0 * 0.017453293, cosr = std::cos(rot), sinr = std::sin(rot);
-    if (dim == 3) k_ctcd<3><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
-    else k_ctcd<2><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    if (A.sel) {  // counters[C_CTCD_SEL] was cleared by the caller
+      k_ctcd_select<<<g, AC_WAVES * kWave, 0, s>>>(A);
+      k_ctcd<3, true><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    } else if (dim == 3) {
+      k_ctcd<3, false><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    } else {
+      k_ctcd<2, false><<<g, AC_WAVES * kWave, 0, s>>>(A, cosr, sinr);
+    }
     return;
   }
   if (L.f32) {

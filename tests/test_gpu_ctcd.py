@@ -54,12 +54,21 @@ def test_ctcd_candidates_match_oracle(gpu, scene):
     t.options.box_eta = t.options.vf_ccd_eta = t.options.ee_ccd_eta = th
     t.options.vf_one_sided = True
     t.update(V0, V1, F)
-    t.run(V0, V1, F, keep_candidates=True)
-    sten, toi = _vf_candidates(t)
-    assert len(sten) > 100
-    want = O.ctcd_list(V0, V1, sten, th)
-    ndiff = _compare_toi(toi, want)
-    print(f"[ctcd] {scene}: {len(sten)} VF candidates, {int((want >= 0).sum())} hits, {ndiff} TOIs differ in the last bits")
+    for frame in range(3):
+        # Once the hit list holds every candidate (after a frame that outgrew it:
+        # the EE ACCD then suspends stencils), the VF CTCD runs only on the
+        # stencils that pass its prefilter, listed by a selection pass.
+        t.run(V0, V1, F, keep_candidates=True)
+        c = t.counts()
+        if frame == 0:
+            grew = c["hits"] > max(1024, c["candidates"] // 2)
+        assert (c["suspended"] > 0) == (grew and frame > 0), (frame, c)
+        sten, toi = _vf_candidates(t)
+        assert len(sten) > 100
+        want = O.ctcd_list(V0, V1, sten, th)  # the candidate order differs from frame to frame
+        ndiff = _compare_toi(toi, want)
+        print(f"[ctcd] {scene} frame {frame}: {len(sten)} VF candidates, {int((want >= 0).sum())} hits, "
+              f"{ndiff} TOIs differ in the last bits (selection pass: {grew and frame > 0})", flush=True)
 
 
 @pytest.mark.parametrize("strict", [True, False])
