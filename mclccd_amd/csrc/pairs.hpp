@@ -71,9 +71,8 @@ template <int D> __device__ __forceinline__ FaceIds<D> face_ids(int4 h) {
 // pair once in the KdBVH's order; the rank reproduces that orientation); pa,
 // pb receive their sorted positions.
 template <int D>
-__device__ __forceinline__ void load_pair_faces(const int2* __restrict__ pairs, const FaceArrays& F, long long i,
-                                                FaceIds<D>& fa, FaceIds<D>& fb, int& pa, int& pb) {
-  const int2 sp = pairs[i];
+__device__ __forceinline__ void pair_faces(const FaceArrays& F, const int2 sp, FaceIds<D>& fa, FaceIds<D>& fb, int& pa,
+                                           int& pb) {
   const int4 x = F.head[sp.x], y = F.head[sp.y];
   const int rx = F.rank[sp.x], ry = F.rank[sp.y];
   const bool keep = rx < ry;
@@ -81,6 +80,11 @@ __device__ __forceinline__ void load_pair_faces(const int2* __restrict__ pairs, 
   fb = face_ids<D>(keep ? y : x);
   pa = keep ? sp.x : sp.y;
   pb = keep ? sp.y : sp.x;
+}
+template <int D>
+__device__ __forceinline__ void load_pair_faces(const int2* __restrict__ pairs, const FaceArrays& F, long long i,
+                                                FaceIds<D>& fa, FaceIds<D>& fb, int& pa, int& pb) {
+  pair_faces<D>(F, pairs[i], fa, fb, pa, pb);
 }
 
 // BVHTree.cpp:327-397: VF f0->f1 (codes 0..2), f1->f0 (3..5), EE (3i + j)

@@ -417,11 +417,19 @@ __device__ __forceinline__ void write_stencils(int4* stage, uint32_t mask, int a
   wave_lds_sync();
 }
 
+// The t-th chunk a wave takes: A.chunk consecutive chunks per grid-stride step.
+__device__ __forceinline__ long long expand_chunk(const ExpandArgs& A, int w, long long t) {
+  const long long stride = (long long)gridDim.x * EX_WAVES * A.chunk;
+  return ((long long)xcd_block() * EX_WAVES + w) * A.chunk + (t / A.chunk) * stride + t % A.chunk;
+}
+
 // Pass 2: the discrete test and the candidates, written at the scanned offsets
 // (each chunk's stencils in lane order: the queues follow the pair list).
 // ODEP: also evaluate the discrete test with the two primitives swapped and
 // list the pairs whose result changes (its staging buffer only exists then,
 // so the default kernel keeps the LDS size that allows four blocks per CU).
+// Each chunk's pairs are loaded one chunk ahead, before the stores they would
+// otherwise queue behind.
 template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand(ExpandArgs A) {
   __shared__ int2 s_d[EX_WAVES][EX_DSTAGE];
   __shared__ int2 s_o[EX_WAVES][ODEP ? EX_DSTAGE : 1];
@@ -430,14 +438,18 @@ template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void 
   int nd = 0, no = 0;  // staged discrete / order-dependent records (wave-uniform)
   const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
   const long long nchunks = (npairs + kWave - 1) / kWave;
-  const long long stride = (long long)gridDim.x * EX_WAVES * A.chunk;
-  for (long long c0 = ((long long)xcd_block() * EX_WAVES + w) * A.chunk; c0 < nchunks; c0 += stride)
-  for (long long ch = c0; ch < c0 + A.chunk && ch < nchunks; ++ch) {
+  long long ch = expand_chunk(A, w, 0);
+  int2 next = make_int2(0, 0);
+  if (ch < nchunks && ch * kWave + lane < npairs) next = A.pairs[ch * kWave + lane];
+  for (long long t = 0; ch < nchunks; ++t) {
     const long long i = ch * kWave + lane;
     const bool valid = i < npairs;
+    const int2 sp = next;
+    const long long cn = expand_chunk(A, w, t + 1);
+    if (cn < nchunks && cn * kWave + lane < npairs) next = A.pairs[cn * kWave + lane];
     FaceIds<D> ha = {}, hb = {};
     int pa = 0, pb = 0;
-    if (valid) load_pair_faces<D>(A.pairs, A.faces, i, ha, hb, pa, pb);
+    if (valid) pair_faces<D>(A.faces, sp, ha, hb, pa, pb);
     if (A.discrete) {  // BVHTree.cpp:464-500
       int dh, dh_swap;
       discrete_test<D, ODEP>(A.faces.v1, ha, hb, valid, dh, dh_swap);
@@ -447,13 +459,17 @@ template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void 
       if constexpr (ODEP)
         stage_push<int2, EX_DSTAGE>(s_o[w], no, dh != dh_swap, rec, A.odep, A.odep_cap, A.counters + C_ORDER_DEP);
     }
-    if (!A.continuous) continue;
+    if (!A.continuous) {
+      ch = cn;
+      continue;
+    }
     uint32_t vfm = 0, eem = 0;
     if (valid) candidate_masks<D>(ha, hb, vfm, eem);
     const int x0 = ha.p[0], x1 = ha.p[1], x2 = ha.p[D - 1], y0 = hb.p[0], y1 = hb.p[1], y2 = hb.p[D - 1];
     write_stencils<D, true>(s_st[w], vfm, x0, x1, x2, y0, y1, y2, A.vfq, A.off_vf[ch], A.vf_cap);
     if constexpr (D == 3)
       write_stencils<D, false>(s_st[w], eem, x0, x1, x2, y0, y1, y2, A.eeq, A.off_ee[ch], A.ee_cap);
+    ch = cn;
   }
   if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
   if (ODEP && no) stage_flush(s_o[w], no, A.odep, A.odep_cap, A.counters + C_ORDER_DEP);
