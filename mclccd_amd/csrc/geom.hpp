@@ -41,8 +41,31 @@ template <typename T> __device__ __forceinline__ T smax(T a, T b) { return (a < 
 template <typename T> __device__ __forceinline__ T smin(T a, T b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double cr_div(double a, double b) { return a / b; }
 __device__ __forceinline__ float cr_div(float a, float b) { return (float)((double)a / (double)b); }
-__device__ __forceinline__ double cr_sqrt(double a) { return __builtin_sqrt(a); }
-__device__ __forceinline__ float cr_sqrt(float a) { return (float)__builtin_sqrt((double)a); }
+// Correctly rounded FP64 square root.  On gfx950 sqrt lowers to rsq, one
+// Goldschmidt step for (g, h) ~ (sqrt x, 1/(2 sqrt x)) and two Markstein
+// corrections g += (x - g*g)*h, wrapped in a scaling of x by 2^256 (result by
+// 2^-128) when x < 2^-767 and a select that returns x for +-0 and +inf.  For x
+// in [2^-767, DBL_MAX] the scalings are by 2^0 and the select keeps the
+// iteration's value, so the bare iteration below gives the same bits with
+// eight fewer vector instructions; other inputs take the full sequence.  The
+// host build of this code (the CPU checks) uses the C library's sqrt.
+__device__ __forceinline__ double cr_sqrt(double a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  if (a >= 0x1p-767 && a <= 0x1.fffffffffffffp+1023) {
+    const double y = __builtin_amdgcn_rsq(a);
+    double g = a * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, a);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, a);
+    return __builtin_fma(d, h, g);
+  }
+#endif
+  return __builtin_sqrt(a);
+}
+__device__ __forceinline__ float cr_sqrt(float a) { return (float)cr_sqrt((double)a); }
 
 // ---------------------------------------------------------------------------
 // VF distance: |q - closest point of triangle (a,b,c)|
