@@ -190,4 +190,20 @@ inline int xcd_grid(long long items, int per_block) {
   return (int)((b + 7) / 8 * 8);
 }
 
+// A wave-uniform 64-bit index as the compiler can see it: the same value read
+// from the first active lane.
+__device__ __forceinline__ long long uniform_index(long long i) {
+  return ((long long)__builtin_amdgcn_readfirstlane((int)(i >> 32)) << 32) |
+         (unsigned)__builtin_amdgcn_readfirstlane((int)i);
+}
+
+// A load at a wave-uniform address through the scalar cache (constant address
+// space).  Only for data that no thread of the running kernel writes: the
+// scalar cache does not see this kernel's vector stores (it is invalidated at
+// every kernel launch).  Unlike a vector load it does not wait behind the
+// wave's earlier vector stores, which share the vector loads' in-order counter.
+template <typename T> __device__ __forceinline__ T scalar_load(const T* p) {
+  return *(const __attribute__((address_space(4))) T*)p;
+}
+
 }  // namespace mclg

@@ -478,9 +478,15 @@ template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void 
     uint32_t vfm = 0, eem = 0;
     if (valid) candidate_masks<D>(ha, hb, vfm, eem);
     const int x0 = ha.p[0], x1 = ha.p[1], x2 = ha.p[D - 1], y0 = hb.p[0], y1 = hb.p[1], y2 = hb.p[D - 1];
-    write_stencils<D, true>(s_st[w], vfm, x0, x1, x2, y0, y1, y2, A.vfq, A.off_vf[ch], A.vf_cap);
+    // the chunk's offsets (written by the scan) through the scalar cache: the EE
+    // offset then does not wait for the VF stencils' stores (1M triangles:
+    // 8.5 -> 8.1 ms)
+    const long long chu = uniform_index(ch);
+    const unsigned long long ov = scalar_load(A.off_vf + chu);
+    const unsigned long long oe = D == 3 ? scalar_load(A.off_ee + chu) : 0ull;
+    write_stencils<D, true>(s_st[w], vfm, x0, x1, x2, y0, y1, y2, A.vfq, ov, A.vf_cap);
     if constexpr (D == 3)
-      write_stencils<D, false>(s_st[w], eem, x0, x1, x2, y0, y1, y2, A.eeq, A.off_ee[ch], A.ee_cap);
+      write_stencils<D, false>(s_st[w], eem, x0, x1, x2, y0, y1, y2, A.eeq, oe, A.ee_cap);
     ch = cn;
   }
   if (nd) stage_flush(s_d[w], nd, A.disc, A.disc_cap, A.counters + C_DISCRETE);
