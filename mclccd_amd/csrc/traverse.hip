@@ -362,38 +362,52 @@ struct ExpandArgs {
 
 constexpr int EX_WAVES = 4, EX_DSTAGE = 128;
 
-// Pass 1: candidate counts per 64-pair chunk (zero past the last pair).
+// Pass 1: candidate counts per 64-pair chunk (zero past the last pair).  A
+// wave takes EC_U consecutive chunks per step and issues all their pair loads,
+// then all their head loads, before using any: the pass is bound by the
+// latency of those two dependent loads, not by bandwidth.
+constexpr int EC_U = 4;
 template <int D> __global__ __launch_bounds__(EX_WAVES * kWave) void k_expand_count(ExpandArgs A) {
   const int w = threadIdx.x / kWave, lane = lane_id();
   const long long npairs = (long long)(*A.npairs_dev < A.pair_cap ? *A.npairs_dev : A.pair_cap);
-  const long long nchunks = (npairs + kWave - 1) / kWave;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     A.off_vf[A.nchunks_cap] = 0;
     A.off_ee[A.nchunks_cap] = 0;
   }
-  const long long stride = (long long)gridDim.x * EX_WAVES;
-  for (long long ch = (long long)xcd_block() * EX_WAVES + w; ch < A.nchunks_cap; ch += stride) {
-    uint32_t cv = 0, ce = 0;
-    if (ch < nchunks && A.continuous) {
-      const long long i = ch * kWave + lane;
-      if (i < npairs) {
-        // counts do not depend on orientation: the heads suffice
-        const int2 sp = A.pairs[i];
-        const FaceIds<D> ha = face_ids<D>(A.faces.head[sp.x]), hb = face_ids<D>(A.faces.head[sp.y]);
+  const long long stride = (long long)gridDim.x * EX_WAVES * EC_U;
+  for (long long c0 = ((long long)xcd_block() * EX_WAVES + w) * EC_U; c0 < A.nchunks_cap; c0 += stride) {
+    // unconditional loads (indices clamped into the list; the selects come after
+    // every load was issued), so that all of them are in flight together
+    int2 sp[EC_U];
+    bool in[EC_U];
+#pragma unroll
+    for (int u = 0; u < EC_U; ++u) {
+      const long long i = (c0 + u) * kWave + lane;
+      in[u] = A.continuous && i < npairs;
+      sp[u] = A.pairs[i & -(long long)(i < npairs)];  // masks, not selects: no branch per load
+    }
+    int4 ha[EC_U], hb[EC_U];
+#pragma unroll
+    for (int u = 0; u < EC_U; ++u) {  // counts do not depend on orientation: the heads suffice
+      const int m = -(int)in[u];
+      ha[u] = A.faces.head[sp[u].x & m];
+      hb[u] = A.faces.head[sp[u].y & m];
+    }
+#pragma unroll
+    for (int u = 0; u < EC_U; ++u) {
+      uint32_t c = 0;  // VF count in the low half, EE count in the high half (at most 6 and 9 per lane)
+      if (in[u]) {
         uint32_t vfm, eem;
-        candidate_masks<D>(ha, hb, vfm, eem);
-        cv = (uint32_t)__popc(vfm);
-        ce = (uint32_t)__popc(eem);
+        candidate_masks<D>(face_ids<D>(ha[u]), face_ids<D>(hb[u]), vfm, eem);
+        c = (uint32_t)__popc(vfm) | ((uint32_t)__popc(eem) << 16);
       }
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        cv += __shfl_xor(cv, o);
-        ce += __shfl_xor(ce, o);
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+      const long long ch = c0 + u;
+      if (lane == 0 && ch < A.nchunks_cap) {
+        A.off_vf[ch] = c & 0xffffu;
+        A.off_ee[ch] = c >> 16;
       }
-    }
-    if (lane == 0) {
-      A.off_vf[ch] = cv;
-      A.off_ee[ch] = ce;
     }
   }
 }
@@ -865,7 +879,7 @@ void launch_expand(int dim, const ExpandLaunch& L, hipStream_t s) {
   if (items <= 0) return;
   const int g = grid_for((items + kWave - 1) / kWave / A.chunk + 1, EX_WAVES, L.num_cus, L.blocks_per_cu);
   if (L.continuous) {  // pass 1 + scans: exact output offsets, no shared queue counter
-    const int gc = grid_for(A.nchunks_cap, EX_WAVES, L.num_cus, L.blocks_per_cu);
+    const int gc = grid_for((A.nchunks_cap + EC_U - 1) / EC_U, EX_WAVES, L.num_cus, L.blocks_per_cu);
     if (dim == 3) k_expand_count<3><<<gc, EX_WAVES * kWave, 0, s>>>(A);
     else k_expand_count<2><<<gc, EX_WAVES * kWave, 0, s>>>(A);
     launch_scan_u64(A.off_vf, A.nchunks_cap + 1, L.scan_part, s);
