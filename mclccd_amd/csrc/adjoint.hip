@@ -402,8 +402,11 @@ __device__ bool toi_adjoint_k1(const int* sv, const double* rec, double xsi, dou
 // order[k] (or hit k when order is null); skey[k] (sorted (kind, first
 // vertex) keys, kind at bit kind_shift) lets blocks of the other kind leave at
 // once.  Longer hits are appended to queue (at most qcap; *qn counts all).
+// Three waves per SIMD: the vertex-face kernel needs 172 VGPRs unconstrained
+// (two waves); capped at 168 it spills 8 bytes and the adjoint drops from ~78
+// to ~75 ms at 8.3e8 hits.  The LDS table allows three blocks per CU either way.
 template <int D, bool VF>
-__global__ __launch_bounds__(ADJ_T) void k_toi_adjoint_k1(const int4* __restrict__ sten,
+__global__ __launch_bounds__(ADJ_T) __attribute__((amdgpu_waves_per_eu(3))) void k_toi_adjoint_k1(const int4* __restrict__ sten,
                                                           const uint8_t* __restrict__ is_vf,
                                                           const double* __restrict__ w,
                                                           const uint32_t* __restrict__ order,
