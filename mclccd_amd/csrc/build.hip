@@ -21,8 +21,8 @@ __global__ void k_pack(const double* __restrict__ V0, const double* __restrict__
   double* r = rec + (size_t)v * VRec<D>::stride;
   if constexpr (D == 3) {
     const double x1 = rd(V1[3 * (size_t)v]), y1 = rd(V1[3 * (size_t)v + 1]), z1 = rd(V1[3 * (size_t)v + 2]);
-    r[0] = rd(V0[3 * (size_t)v]); r[1] = rd(V0[3 * (size_t)v + 1]); r[2] = rd(V0[3 * (size_t)v + 2]); r[3] = 0.0;
-    r[4] = x1; r[5] = y1; r[6] = z1; r[7] = 0.0;
+    r[0] = rd(V0[3 * (size_t)v]); r[1] = rd(V0[3 * (size_t)v + 1]); r[2] = rd(V0[3 * (size_t)v + 2]);
+    r[3] = x1; r[4] = y1; r[5] = z1; r[6] = 0.0; r[7] = 0.0;
     if (v1) {
       double* o = v1 + (size_t)v * 4;
       o[0] = x1; o[1] = y1; o[2] = z1; o[3] = 0.0;

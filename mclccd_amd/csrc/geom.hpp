@@ -534,16 +534,17 @@ __device__ __forceinline__ int edge_edge2(const d2& p0, const d2& p1, const d2& 
 template <int D> struct VLoad;
 template <> struct VLoad<3> {
   using V = d3;
+  // record {x0, x1, pad}: the six values in three 16 B loads
   __device__ __forceinline__ static void load(const double* rec, int v, d3& a, d3& b) {
     const double2* r = reinterpret_cast<const double2*>(rec + (size_t)v * 8);
-    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
+    const double2 r0 = r[0], r1 = r[1], r2 = r[2];
     a = {r0.x, r0.y, r1.x};
-    b = {r2.x, r2.y, r3.x};
+    b = {r1.y, r2.x, r2.y};
   }
   __device__ __forceinline__ static d3 load_t1(const double* rec, int v) {
-    const double2* r = reinterpret_cast<const double2*>(rec + (size_t)v * 8);
-    const double2 r2 = r[2], r3 = r[3];
-    return {r2.x, r2.y, r3.x};
+    const double* r = rec + (size_t)v * 8;
+    const double2 r2 = reinterpret_cast<const double2*>(r)[2];
+    return {r[3], r2.x, r2.y};
   }
 };
 template <> struct VLoad<2> {
