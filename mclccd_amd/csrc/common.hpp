@@ -76,17 +76,13 @@ inline size_t child_pair_bytes(int dim) { return dim == 3 ? sizeof(ChildPair<3>)
 //   rank[s]  orients pairs (lower rank first); the primitive index unless a rank was set
 //   prim[s]  primitive index (discrete hits, kept leaf pairs)
 //   v1[v]    V1 of vertex v: {x, y, z, 0} (3D) / {x, y} (2D)
-//   fv1[s]   3D: V1 of the leaf's three vertices {p0.xyz, p1.xyz, p2.xyz, 0} (80 B:
-//            the discrete test reads a face in five loads instead of six)
 struct FaceArrays {
   int4* head = nullptr;
   int* rank = nullptr;
   int* prim = nullptr;
-  double* fv1 = nullptr;
   double* v1 = nullptr;
 };
-inline size_t face_fv1_offset(size_t np) { return ((sizeof(int4) + 2 * sizeof(int)) * np + 15) / 16 * 16; }
-inline size_t face_bytes(size_t np) { return face_fv1_offset(np) + 10 * sizeof(double) * np; }
+inline size_t face_bytes(size_t np) { return (sizeof(int4) + 2 * sizeof(int)) * np; }
 inline size_t v1_stride(int dim) { return dim == 3 ? 4 : 2; }
 // The leaf arrays inside one allocation of face_bytes(np) bytes.
 inline FaceArrays face_arrays(void* base, size_t np, double* v1) {
@@ -94,7 +90,6 @@ inline FaceArrays face_arrays(void* base, size_t np, double* v1) {
   f.head = static_cast<int4*>(base);
   f.rank = reinterpret_cast<int*>(f.head + np);
   f.prim = f.rank + np;
-  f.fv1 = reinterpret_cast<double*>(static_cast<char*>(base) + face_fv1_offset(np));
   f.v1 = v1;
   return f;
 }

@@ -155,26 +155,11 @@ template <int D> __device__ __forceinline__ void face_v1(const double* v1, const
   }
 }
 
-// V1 of the face at sorted position s: the leaf's face record in 3D (five
-// loads), the vertices' V1 in 2D.
-template <int D>
-__device__ __forceinline__ void leaf_v1(const FaceArrays& F, int s, const FaceIds<D>& f, double* x) {
-  if constexpr (D == 3) {
-    const double2* r = reinterpret_cast<const double2*>(F.fv1 + 10 * (size_t)s);
-    const double2 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3];
-    const double r4 = F.fv1[10 * (size_t)s + 8];
-    x[0] = r0.x; x[1] = r0.y; x[2] = r1.x; x[3] = r1.y; x[4] = r2.x; x[5] = r2.y; x[6] = r3.x; x[7] = r3.y; x[8] = r4;
-  } else {
-    face_v1<D>(F.v1, f, x);
-  }
-}
-
-// The discrete test at V1 (BVHTree.cpp:464-500) of a pair (faces at sorted
-// positions pa, pb), and with swap the same test with the two primitives
-// exchanged (0 when not asked for).
+// The discrete test at V1 (BVHTree.cpp:464-500) of a pair, and with swap the
+// same test with the two primitives exchanged (0 when not asked for).
 template <int D, bool SWAP>
-__device__ __forceinline__ void discrete_test(const FaceArrays& F, int pa, int pb, const FaceIds<D>& ha,
-                                              const FaceIds<D>& hb, bool valid, int& dh, int& dh_swap) {
+__device__ __forceinline__ void discrete_test(const double* v1, const FaceIds<D>& ha, const FaceIds<D>& hb, bool valid,
+                                              int& dh, int& dh_swap) {
   bool shared = false;
 #pragma unroll
   for (int x = 0; x < D; ++x)
@@ -184,8 +169,8 @@ __device__ __forceinline__ void discrete_test(const FaceArrays& F, int pa, int p
   dh_swap = 0;
   if (valid && !shared) {
     double a[3 * D], b[3 * D];
-    leaf_v1<D>(F, pa, ha, a);
-    leaf_v1<D>(F, pb, hb, b);
+    face_v1<D>(v1, ha, a);
+    face_v1<D>(v1, hb, b);
     if constexpr (D == 3) {
       const d3 p1{a[0], a[1], a[2]}, q1{a[3], a[4], a[5]}, r1{a[6], a[7], a[8]};
       const d3 p2{b[0], b[1], b[2]}, q2{b[3], b[4], b[5]}, r2{b[6], b[7], b[8]};

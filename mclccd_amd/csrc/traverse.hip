@@ -317,17 +317,6 @@ __global__ void k_face_records(const uint32_t* __restrict__ order, int n, const 
   F.head[s] = D == 3 ? make_int4(p[0], p[1], p[D - 1], fl) : make_int4(p[0], p[1], fl, 0);
   F.rank[s] = rank ? rank[prim] : prim;
   F.prim[s] = prim;
-  if constexpr (D == 3) {  // V1 (packed beforehand) of the three vertices
-    double* o = F.fv1 + (size_t)s * 10;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const double* r = F.v1 + 4 * (size_t)p[j];
-      o[3 * j] = r[0];
-      o[3 * j + 1] = r[1];
-      o[3 * j + 2] = r[2];
-    }
-    o[9] = 0.0;
-  }
 }
 
 void launch_face_records(int dim, const uint32_t* order, int n, const int* P, const uint8_t* flags, const int* rank,
@@ -501,7 +490,7 @@ template <int D, bool ODEP> __global__ __launch_bounds__(EX_WAVES * kWave) void 
     if (valid) pair_faces<D>(A.faces, sp, ha, hb, pa, pb);
     if (A.discrete) {  // BVHTree.cpp:464-500
       int dh, dh_swap;
-      discrete_test<D, ODEP>(A.faces, pa, pb, ha, hb, valid, dh, dh_swap);
+      discrete_test<D, ODEP>(A.faces.v1, ha, hb, valid, dh, dh_swap);
       int2 rec = make_int2(0, 0);
       if (dh || dh_swap) rec = make_int2(A.faces.prim[pa], A.faces.prim[pb]);
       stage_push<int2, EX_DSTAGE>(s_d[w], nd, dh != 0, rec, A.disc, A.disc_cap, A.counters + C_DISCRETE);
